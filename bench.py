@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X force-and-integration path.
+
+Workload: the 10-element mixture (32,768 atoms on a 32^3 lattice, FP64,
+Lorentz-Berthelot LJ + truncated Coulomb sharing rc = 2.5 max sigma,
+Berendsen NVT at 60 K with tau = 100 fs, dt = 0.5 fs, g(r) with 200 bins
+to L/4 sampled every 100 ticks).  A step is one full tick of the reference
+schedule (kick, drift, wrap, cell-list rebuild, forces, kick, Berendsen) over
+the resident state; inputs are synthetic (seeded) and resident in HBM before
+the timed region.
+
+Prints one JSON line.  ``roofline`` is the pair kernel's FP64 roofline (its
+binding resource; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s),
+``roofline_hbm`` the same kernel against HBM, ``cpu_baseline`` the oracle's
+restatement of the reference CPU algorithm timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+# Algorithmic work, per atom-step (definitions in DESIGN.md):
+STEP_BYTES = 161.0          # x, v, a read+write (144) + species (1) + cell key (8) + charge (8)
+PAIR_BYTES = 57.0           # pair kernel: x, y, z, q, species read (33) + a written (24)
+FLOP_PER_PAIR = 82.0        # LJ + quintic switch + accumulate (70) + Coulomb (12), per ordered pair
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only: the path shards by spatial domain "
+                                   "(see DESIGN.md) but no multi-GPU figure is produced; run with --gpus 1"}))
+        return 2
+
+    from genesis_amd.inputs import mixture
+    from genesis_amd.universe import GpuUniverse
+
+    w = mixture()
+    ph, st = w.physics, w.state
+    n = st.n
+    gr_every, gr_bins, gr_rmax = w.extra["gr_every"], w.extra["gr_bins"], w.extra["gr_rmax"]
+    log(f"workload {w.name}: n={n} box={ph.box[0]:.4f} dims={ph.grid_dims()} steps={args.steps} "
+        f"warmup={args.warmup}")
+
+    u = GpuUniverse(ph, st)
+    # pairs in range (for the FLOP count): from one standalone force pass
+    u.forces()
+    pe, sabs = u.force_extras()
+
+    def run(k, observe=False):
+        done = 0
+        while done < k:
+            chunk = min(gr_every, k - done)
+            u.step(chunk, observe=observe)
+            done += chunk
+            u.radial_distribution(gr_rmax, gr_bins)
+    if args.warmup:
+        run(args.warmup)
+    u.timing(enable=True)
+    t0 = time.perf_counter()
+    run(args.steps)
+    t1 = time.perf_counter()
+    tm = u.timing()
+    elapsed = t1 - t0
+    value = n * args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    log(f"{value:.4e} atom-steps/s, {ms_per_step:.4f} ms/step; kernel phases per step: "
+        f"rebuild {tm['rebuild_ms'] / tm['steps']:.4f} ms, pair {tm['pair_ms'] / tm['steps']:.4f} ms, "
+        f"kick {tm['kick_ms'] / tm['steps']:.4f} ms")
+
+    # pair count of the final state (ordered pairs within rc)
+    fin = u.download()
+    pair_ms = tm["pair_ms"] / max(tm["steps"], 1)
+    # in-range ordered pairs per atom from a neighbour count on the final state
+    npairs = count_pairs(fin, ph)
+    flops_per_launch = FLOP_PER_PAIR * npairs
+    tflops = flops_per_launch / (pair_ms * 1e-3) / 1e12
+    gbs = PAIR_BYTES * n / (pair_ms * 1e-3) / 1e9
+    out = {
+        "metric": "atom-steps/s (force+Verlet), batched vs plain, at stated force tol; % HBM BW",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lattice + Maxwell-Boltzmann, no dataset)",
+        "config": {"workload": w.name, "atoms": n, "species": 10, "box": ph.box[0], "rc": ph.force_rc(),
+                   "cells": list(ph.grid_dims()), "dt_fs": 0.5, "thermostat": "berendsen 60 K tau 200 ticks",
+                   "coulomb": True, "gr": {"bins": gr_bins, "r_max": gr_rmax, "every": gr_every},
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "pipe": "fp64 VALU (pair forces are not a dense contraction; the MI355X "
+                                              "FP64 vector peak equals its FP64 matrix peak)",
+                     "kernel": "k_pair", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / FP64_PEAK_TFLOPS, "traffic": None,
+                     "per_launch": {"ordered_pairs": npairs, "flop_per_pair": FLOP_PER_PAIR,
+                                    "avg_ms": pair_ms}},
+        "roofline_hbm": {"bound": "hbm", "kernel": "k_pair", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "bytes_per_atom": PAIR_BYTES},
+        "step_roofline": {"hbm_frac": value * STEP_BYTES / (HBM_PEAK_GBS * 1e9),
+                          "fp64_frac": value * (FLOP_PER_PAIR * npairs / n) / (FP64_PEAK_TFLOPS * 1e12),
+                          "bytes_per_atom_step": STEP_BYTES, "flop_per_atom_step": FLOP_PER_PAIR * npairs / n},
+        "phases_ms_per_step": {k: tm[k] / max(tm["steps"], 1) for k in ("rebuild_ms", "pair_ms", "kick_ms")},
+    }
+    u.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def count_pairs(s, ph) -> int:
+    """Ordered pairs within the force cutoff (cell-binned numpy count)."""
+    rc = ph.force_rc()
+    box = np.asarray(ph.box)
+    p = np.stack([s.x, s.y, s.z], 1)
+    p = p - box * np.floor(p / box + 0.5)
+    dims = np.maximum(np.floor(box / rc).astype(int), 1)
+    cell = np.floor((p + 0.5 * box) / (box / dims)).astype(int) % dims
+    key = cell[:, 0] + dims[0] * (cell[:, 1] + dims[1] * cell[:, 2])
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    starts = np.searchsorted(ks, np.arange(dims.prod() + 1))
+    total = 0
+    for c in range(int(dims.prod())):
+        a = order[starts[c]:starts[c + 1]]
+        if a.size == 0:
+            continue
+        cx, cy, cz = c % dims[0], (c // dims[0]) % dims[1], c // (dims[0] * dims[1])
+        neigh = []
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    nc = ((cx + dx) % dims[0]) + dims[0] * (((cy + dy) % dims[1]) + dims[1] * ((cz + dz) % dims[2]))
+                    neigh.append(order[starts[nc]:starts[nc + 1]])
+        b = np.unique(np.concatenate(neigh))
+        d = p[a][:, None, :] - p[b][None, :, :]
+        d -= box * np.round(d / box)
+        r2 = (d * d).sum(-1)
+        total += int(((r2 < rc * rc) & (r2 > 0)).sum())
+    return total
+
+
+def cpu_baseline(w, seconds: float) -> dict:
+    """The oracle's restatement of the reference CPU algorithm (serial grid +
+    half-shell pair list, threaded fold with per-thread buffers merged in
+    order, serial kinetic sum) timed on this host on a bounded sample of the
+    same workload: as many full ticks as fit in ~``seconds``."""
+    from oracle import oracle as O
+    cores = max(1, min(os.cpu_count() or 1, 16))
+    st = w.state
+    O.run(w.physics, st, 1, nthreads=cores, observe=False)  # warm caches
+    steps, t_used = 0, 0.0
+    s = st
+    while t_used < seconds and steps < 400:
+        k = 1 if steps < 2 else max(1, min(20, int((seconds - t_used) / max(t_used / steps, 1e-3))))
+        t0 = time.perf_counter()
+        s, _ = O.run(w.physics, s, k, nthreads=cores, observe=False)
+        t_used += time.perf_counter() - t0
+        steps += k
+    v = st.n * steps / t_used
+    log(f"cpu baseline: {v:.4e} atom-steps/s over {steps} ticks ({t_used:.1f} s, {cores} threads)")
+    return {"value": v, "unit": "atom-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} full ticks of the {st.n}-atom mixture workload ({t_used:.1f} s), "
+                      f"C++ restatement of the reference rayon force pass with {cores} threads"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

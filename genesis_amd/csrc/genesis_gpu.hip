@@ -1,0 +1,1778 @@
+// genesis_gpu.hip — MI355X (gfx950) force-and-integration path of GENESIS
+// behind the C ABI of include/genesis_gpu.h.
+//
+// Resident state lives in HBM in cell-sorted order (structure of arrays):
+//   pos4[k] = (x, y, z, q)   double4, the pair kernel's tile format
+//   vx/vy/vz, ax/ay/az       double SoA, coalesced integrator traffic
+//   type[k] (u8), id[k] (u32 entity slot of sorted position k)
+// Every force pass re-sorts the atoms by cell: keys + counting (atomic u32
+// histogram), one exclusive scan, a scatter, then a per-cell ordering by
+// entity id that makes the layout independent of atomic arrival order.  The
+// pair kernel works cell by cell: one 256-thread workgroup per home cell
+// stages the 27-cell stencil in LDS (positions pre-shifted by the periodic
+// image), each wave takes home atoms in turn, sweeps the tile 64 candidates
+// at a time, compacts the in-range ones with a ballot into an LDS queue and
+// evaluates full 64-lane batches of the switched LJ + Coulomb law.  Each atom
+// owns its own sum (full shell), so no floating-point atomics exist and
+// results are bitwise reproducible run to run.
+#include "gs_device.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace gs;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) return fail(GS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int BLOCK = 256;
+constexpr int PAIR_WAVES = 4;
+constexpr int PAIR_THREADS = PAIR_WAVES * WAVE;
+constexpr int TILE_CAP = 1024;  // candidates staged per LDS chunk
+constexpr int QCAP = 128;       // per-wave compaction ring (u16 tile indices)
+
+inline unsigned nblocks(int64_t n, int b = BLOCK) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// =============================================================================
+// Kernels
+// =============================================================================
+
+// Cell key of a position on the force grid: SpatialGrid::cell_of + encode
+// (src/physics/grid.rs:228-246).
+__device__ __forceinline__ uint32_t cell_key(const KParams& P, double x, double y, double z) {
+    uint32_t cx = cell_axis(x, P.box[0], P.cell[0], P.dims[0]);
+    uint32_t cy = cell_axis(y, P.box[1], P.cell[1], P.dims[1]);
+    uint32_t cz = cell_axis(z, P.box[2], P.cell[2], P.dims[2]);
+    return cx + P.dims[0] * (cy + P.dims[1] * cz);
+}
+
+// Fused: thermostat scale of the previous tick (v *= lambda,
+// src/systems/thermostat.rs:68-70), half kick (src/systems/integrate.rs:38-44),
+// drift (:61-67), periodic wrap (src/systems/boundary.rs:28-34), then the cell
+// key and the atomic histogram of the rebuild.  Every expression keeps the
+// reference's operation order, so given the same accelerations the result
+// is bitwise the reference's.
+template <bool KICK_DRIFT>
+__global__ __launch_bounds__(BLOCK) void k_integrate_key(KParams P, double4* __restrict__ pos4,
+                                                         double* __restrict__ vx, double* __restrict__ vy,
+                                                         double* __restrict__ vz, const double* __restrict__ ax,
+                                                         const double* __restrict__ ay, const double* __restrict__ az,
+                                                         const double* __restrict__ lam_ptr,
+                                                         uint32_t* __restrict__ key, uint32_t* __restrict__ ticket,
+                                                         uint32_t* __restrict__ count) {
+    int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= P.n) return;
+    double4 p = pos4[k];
+    if (KICK_DRIFT) {
+        double lam = *lam_ptr;
+        double a = vx[k] * lam, b = vy[k] * lam, c = vz[k] * lam;
+        a = a + ax[k] * P.half_dt;
+        b = b + ay[k] * P.half_dt;
+        c = c + az[k] * P.half_dt;
+        vx[k] = a;
+        vy[k] = b;
+        vz[k] = c;
+        p.x = p.x + a * P.dt;
+        p.y = p.y + b * P.dt;
+        p.z = p.z + c * P.dt;
+        if (P.flags & F_WRAP) {
+            p.x = rem_euclid(p.x, 2.0 * P.half[0]) - P.half[0];
+            p.y = rem_euclid(p.y, 2.0 * P.half[1]) - P.half[1];
+            p.z = rem_euclid(p.z, 2.0 * P.half[2]) - P.half[2];
+        }
+        pos4[k] = p;
+    }
+    uint32_t kk = cell_key(P, p.x, p.y, p.z);
+    key[k] = kk;
+    ticket[k] = atomicAdd(&count[kk], 1u);
+}
+
+// Exclusive scan of the per-cell histogram into cell_start[0..ncell], and
+// reset of the histogram for the next rebuild.  One 1024-thread workgroup,
+// 16 cells per thread per pass, wave scans with shuffles.
+__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ count, uint32_t* __restrict__ start,
+                                               uint32_t ncell) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < ncell; base += 1024u * 16u) {
+        uint32_t v[16];
+        uint32_t s = 0;
+        const uint32_t i0 = base + (uint32_t)tid * 16u;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            uint32_t idx = i0 + e;
+            v[e] = idx < ncell ? count[idx] : 0u;
+            s += v[e];
+        }
+        uint32_t incl = s;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t t = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += t;
+        }
+        if (lane == 63) wsum[w] = incl;
+        __syncthreads();
+        if (w == 0) {
+            uint32_t x = lane < 16 ? wsum[lane] : 0u;
+            uint32_t xi = x;
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                uint32_t t = __shfl_up(xi, off, 64);
+                if (lane >= off) xi += t;
+            }
+            if (lane < 16) wsum[lane] = xi - x;
+        }
+        __syncthreads();
+        uint32_t excl = carry + wsum[w] + incl - s;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            uint32_t idx = i0 + e;
+            if (idx < ncell) {
+                start[idx] = excl;
+                count[idx] = 0u;
+            }
+            excl += v[e];
+        }
+        __syncthreads();
+        if (tid == 1023) carry = excl;
+        __syncthreads();
+    }
+    if (tid == 0) start[ncell] = carry;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scatter(int n, const uint32_t* __restrict__ key,
+                                                   const uint32_t* __restrict__ ticket,
+                                                   const uint32_t* __restrict__ start, uint32_t* __restrict__ perm) {
+    int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= n) return;
+    perm[start[key[k]] + ticket[k]] = (uint32_t)k;
+}
+
+// One wave per cell: order the cell's atoms by entity id (rank by counting;
+// ids are unique) and gather the state into the new sorted arrays.
+__global__ __launch_bounds__(BLOCK) void k_sort_gather(uint32_t ncell, const uint32_t* __restrict__ start,
+                                                       const uint32_t* __restrict__ perm,
+                                                       const double4* __restrict__ p_in,
+                                                       const double* __restrict__ vx_in,
+                                                       const double* __restrict__ vy_in,
+                                                       const double* __restrict__ vz_in,
+                                                       const uint8_t* __restrict__ t_in,
+                                                       const uint32_t* __restrict__ id_in, double4* __restrict__ p_out,
+                                                       double* __restrict__ vx_out, double* __restrict__ vy_out,
+                                                       double* __restrict__ vz_out, uint8_t* __restrict__ t_out,
+                                                       uint32_t* __restrict__ id_out) {
+    __shared__ uint32_t sid[BLOCK / WAVE][256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t c = blockIdx.x * (BLOCK / WAVE) + w;
+    const bool valid = c < ncell;
+    uint32_t s = 0, m = 0;
+    if (valid) {
+        s = start[c];
+        m = start[c + 1] - s;
+    }
+    const bool lds = m <= 256;
+    if (valid && lds)
+        for (uint32_t e = lane; e < m; e += 64) sid[w][e] = id_in[perm[s + e]];
+    __syncthreads();
+    if (!valid) return;
+    for (uint32_t e = lane; e < m; e += 64) {
+        const uint32_t k = perm[s + e];
+        const uint32_t myid = lds ? sid[w][e] : id_in[k];
+        uint32_t rank = 0;
+        for (uint32_t f = 0; f < m; ++f) {
+            uint32_t o = lds ? sid[w][f] : id_in[perm[s + f]];
+            rank += (o < myid) ? 1u : 0u;
+        }
+        const uint32_t p = s + rank;
+        p_out[p] = p_in[k];
+        vx_out[p] = vx_in[k];
+        vy_out[p] = vy_in[k];
+        vz_out[p] = vz_in[k];
+        t_out[p] = t_in[k];
+        id_out[p] = myid;
+    }
+}
+
+// ---- pair law ----------------------------------------------------------------
+
+// Quintic switch of smooth_cutoff (src/physics/forces.rs:302-317) with u
+// clamped to [0, 1]: sw = 1 - 10u^3 + 15u^4 - 6u^5, dsw = -30u^2(1-u)^2 / w.
+// u <= 0 gives (1, 0) and u >= 1 gives (0, 0), the reference's two early
+// returns, so no branch is needed.
+__device__ __forceinline__ void switch_q(double r, double r_on, double inv_w, double& sw, double& dsw) {
+    double u = (r - r_on) * inv_w;
+    u = fmin(fmax(u, 0.0), 1.0);
+    double u2 = u * u;
+    double u3 = u2 * u;
+    sw = fma(u3, fma(u, fma(-6.0, u, 15.0), -10.0), 1.0);
+    double om = 1.0 - u;
+    dsw = (-30.0 * inv_w) * (u2 * (om * om));
+}
+
+// Force magnitude along the normal and potential of one pair at distance d
+// (d = r2 * inv_d): the fold body of ForceSystem::run
+// (src/systems/forces.rs:270-299) composed of force_switched_scalar
+// (src/physics/forces.rs:270-275, hardened core at LJ_NUCLEUS sigma), the
+// Coulomb term with r = max(d, 0.5) switched at d, and gravity.
+template <bool COUL, bool SHARED>
+__device__ __forceinline__ void pair_law(const KParams& P, const PairCoef& pc, double d, double inv_d, double qq,
+                                         double mm, double& fmag, double& vv) {
+    const bool core = d < pc.rcore;
+    const double r = core ? pc.rcore : d;
+    const double inv_r = core ? pc.inv_rcore : inv_d;
+    const double s = pc.sigma * inv_r;
+    const double s2 = s * s;
+    const double s6 = s2 * s2 * s2;
+    const double s12 = s6 * s6;
+    const double vlj = pc.eps4 * (s12 - s6);
+    const double flj = (pc.c24 * s) * fma(2.0, s12, -s6);
+    double sw, dsw;
+    switch_q(r, P.lj_ron, P.lj_inv_w, sw, dsw);
+    if (COUL && SHARED) {
+        const bool cc = d < 0.5;
+        const double inv_c = cc ? 2.0 : inv_d;
+        const double vc = qq * inv_c;
+        const double fc = vc * inv_c;
+        const double fs = flj + fc, vs = vlj + vc;
+        fmag = fma(fs, sw, -vs * dsw);
+        vv = vs * sw;
+    } else {
+        fmag = fma(flj, sw, -vlj * dsw);
+        vv = vlj * sw;
+        if (COUL) {
+            const bool cc = d < 0.5;
+            const double inv_c = cc ? 2.0 : inv_d;
+            const double vc = qq * inv_c;
+            const double fc = vc * inv_c;
+            double swc, dswc;
+            switch_q(d, P.c_ron, P.c_inv_w, swc, dswc);
+            fmag += fma(fc, swc, -vc * dswc);
+            vv += vc * swc;
+        }
+    }
+    if (P.flags & F_GRAVITY) {
+        const bool cc = d < 0.5;
+        const double inv_g = cc ? 2.0 : inv_d;
+        const double vg = -(P.gconst * mm) * inv_g;
+        const double fg = vg * inv_g;
+        double swg, dswg;
+        switch_q(d, P.g_ron, P.g_inv_w, swg, dswg);
+        fmag += fma(fg, swg, -vg * dswg);
+        vv += vg * swg;
+    }
+}
+
+// Full-shell cell-tile pair kernel (the rayon force pass of
+// src/systems/forces.rs:228-345 reorganised for a GPU: every atom sums its
+// own pairs, so the ordered merge of :330-341 disappears).  Requires >= 3
+// cells on every axis so the 27 stencil cells are distinct and each pair is
+// seen once per end; smaller boxes use k_pair_brute.
+template <bool COUL, bool SHARED, bool EXTRAS, bool SINGLE>
+__global__ __launch_bounds__(PAIR_THREADS) void k_pair(KParams P, const uint32_t* __restrict__ cstart,
+                                                       const double4* __restrict__ pos4,
+                                                       const uint8_t* __restrict__ type,
+                                                       const PairCoef* __restrict__ coef, double* __restrict__ ax,
+                                                       double* __restrict__ ay, double* __restrict__ az,
+                                                       double* __restrict__ pe_atom, double* __restrict__ s_abs,
+                                                       double* __restrict__ vpart) {
+    __shared__ double4 tile[TILE_CAP];
+    __shared__ uint8_t ttype[TILE_CAP];
+    __shared__ PairCoef sc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];
+    __shared__ uint32_t seg_base[27];
+    __shared__ uint32_t seg_pref[28];
+    __shared__ double seg_sh[27][3];
+    __shared__ uint16_t queue[PAIR_WAVES][QCAP];
+    __shared__ double wv[PAIR_WAVES];
+
+    const uint32_t c = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t h0 = cstart[c], nh = cstart[c + 1] - h0;
+    if (nh == 0) {
+        if (tid == 0) vpart[c] = 0.0;
+        return;
+    }
+    if (SINGLE) {
+        if (tid == 0) sc[0] = coef[0];
+    } else {
+        for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += PAIR_THREADS) sc[t] = coef[t];
+    }
+    if (tid < 27) {
+        const uint32_t nx = P.dims[0], ny = P.dims[1], nz = P.dims[2];
+        const int cx = (int)(c % nx), cy = (int)((c / nx) % ny), cz = (int)(c / (nx * ny));
+        int ix = cx + tid % 3 - 1, iy = cy + (tid / 3) % 3 - 1, iz = cz + tid / 9 - 1;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        // An atom of the wrapped neighbour is moved by one period so that the
+        // plain difference is the minimum image (valid for >= 3 cells/axis).
+        if (ix < 0) { ix += nx; sx = -P.box[0]; } else if (ix >= (int)nx) { ix -= nx; sx = P.box[0]; }
+        if (iy < 0) { iy += ny; sy = -P.box[1]; } else if (iy >= (int)ny) { iy -= ny; sy = P.box[1]; }
+        if (iz < 0) { iz += nz; sz = -P.box[2]; } else if (iz >= (int)nz) { iz -= nz; sz = P.box[2]; }
+        const uint32_t nc = (uint32_t)ix + nx * ((uint32_t)iy + ny * (uint32_t)iz);
+        const uint32_t b = cstart[nc];
+        seg_base[tid] = b;
+        seg_pref[tid + 1] = cstart[nc + 1] - b;
+        seg_sh[tid][0] = sx;
+        seg_sh[tid][1] = sy;
+        seg_sh[tid][2] = sz;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        seg_pref[0] = 0;
+        for (int s = 0; s < 27; ++s) seg_pref[s + 1] += seg_pref[s];
+    }
+    __syncthreads();
+    const uint32_t ncand = seg_pref[27];
+    const int nchunks = (int)((ncand + TILE_CAP - 1) / TILE_CAP);
+    double vlane = 0.0;
+
+    for (int ch = 0; ch < nchunks; ++ch) {
+        const uint32_t c0 = (uint32_t)ch * TILE_CAP;
+        const uint32_t len = min((uint32_t)TILE_CAP, ncand - c0);
+        __syncthreads();
+        for (uint32_t t = tid; t < len; t += PAIR_THREADS) {
+            const uint32_t idx = c0 + t;
+            int s = 0;
+            while (seg_pref[s + 1] <= idx) ++s;
+            const uint32_t j = seg_base[s] + (idx - seg_pref[s]);
+            double4 pj = pos4[j];
+            pj.x += seg_sh[s][0];
+            pj.y += seg_sh[s][1];
+            pj.z += seg_sh[s][2];
+            tile[t] = pj;
+            ttype[t] = type[j];
+        }
+        __syncthreads();
+
+        for (uint32_t i = w; i < nh; i += PAIR_WAVES) {
+            const double4 pi = pos4[h0 + i];
+            const int ti = type[h0 + i];
+            const double kqi = P.ke * pi.w;
+            const double mi = P.mass[ti];
+            double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, sa = 0.0;
+            uint32_t qn = 0, qh = 0;
+
+            auto batch = [&](uint32_t cnt) {
+                if ((uint32_t)lane < cnt) {
+                    const uint32_t j = queue[w][(qh + lane) & (QCAP - 1)];
+                    const double4 pj = tile[j];
+                    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                    const double inv_d = rsqrt_refined(r2);
+                    const double d = r2 * inv_d;
+                    const int tj = ttype[j];
+                    const PairCoef& pc = SINGLE ? sc[0] : sc[ti * GS_NSPECIES + tj];
+                    double fmag, vv;
+                    pair_law<COUL, SHARED>(P, pc, d, inv_d, kqi * pj.w, mi * P.mass[tj], fmag, vv);
+                    const double fs = fmag * inv_d;
+                    fx = fma(fs, dx, fx);
+                    fy = fma(fs, dy, fy);
+                    fz = fma(fs, dz, fz);
+                    vlane += vv;
+                    if (EXTRAS) {
+                        pe += vv;
+                        sa += fabs(fmag);
+                    }
+                }
+                qh += cnt;
+            };
+
+            for (uint32_t base = 0; base < len; base += 64) {
+                const uint32_t j = base + lane;
+                bool in = false;
+                if (j < len) {
+                    const double4 pj = tile[j];
+                    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                    in = (r2 < P.rc2) && (r2 > P.eps2);
+                }
+                const uint64_t m = __ballot(in);
+                if (in) queue[w][(qn + mask_rank(m)) & (QCAP - 1)] = (uint16_t)j;
+                qn += (uint32_t)__popcll(m);
+                __builtin_amdgcn_wave_barrier();
+                if (qn - qh >= 64) batch(64);
+            }
+            if (qn > qh) batch(qn - qh);
+
+            fx = wave_sum(fx);
+            fy = wave_sum(fy);
+            fz = wave_sum(fz);
+            if (EXTRAS) {
+                pe = wave_sum(pe);
+                sa = wave_sum(sa);
+            }
+            if (lane == 0) {
+                const uint32_t k = h0 + i;
+                const double im = P.inv_mass[ti];
+                if (nchunks == 1) {
+                    ax[k] = fx * im;
+                    ay[k] = fy * im;
+                    az[k] = fz * im;
+                    if (EXTRAS) {
+                        pe_atom[k] = 0.5 * pe;
+                        s_abs[k] = sa;
+                    }
+                } else {
+                    // Several LDS chunks: partial sums travel through the
+                    // output arrays (this wave owns atom k for every chunk).
+                    if (ch > 0) {
+                        fx += ax[k];
+                        fy += ay[k];
+                        fz += az[k];
+                        if (EXTRAS) {
+                            pe += pe_atom[k];
+                            sa += s_abs[k];
+                        }
+                    }
+                    const bool last = ch == nchunks - 1;
+                    ax[k] = last ? fx * im : fx;
+                    ay[k] = last ? fy * im : fy;
+                    az[k] = last ? fz * im : fz;
+                    if (EXTRAS) {
+                        pe_atom[k] = last ? 0.5 * pe : pe;
+                        s_abs[k] = sa;
+                    }
+                }
+            }
+        }
+    }
+    vlane = wave_sum(vlane);
+    if (lane == 0) wv[w] = vlane;
+    __syncthreads();
+    if (tid == 0) vpart[c] = 0.5 * ((wv[0] + wv[1]) + (wv[2] + wv[3]));
+}
+
+// Small-box pair kernel (any axis with < 3 cells): every ordered pair, the
+// reference's min_image, and in GS_GRID_REFERENCE mode the multiplicity the
+// reference grid gives a pair on an axis with exactly 2 cells (each wrapping
+// offset reaches the other cell, src/physics/grid.rs:134-151,275-295): x2 per
+// such axis on which the two atoms' cells differ.
+template <bool COUL, bool EXTRAS>
+__global__ __launch_bounds__(BLOCK) void k_pair_brute(KParams P, const double4* __restrict__ pos4,
+                                                      const uint8_t* __restrict__ type,
+                                                      const PairCoef* __restrict__ coef, double* __restrict__ ax,
+                                                      double* __restrict__ ay, double* __restrict__ az,
+                                                      double* __restrict__ pe_atom, double* __restrict__ s_abs,
+                                                      double* __restrict__ vpart) {
+    __shared__ double4 tp[BLOCK];
+    __shared__ uint8_t tt[BLOCK];
+    __shared__ uint32_t tk[BLOCK];
+    __shared__ PairCoef sc[GS_NSPECIES * GS_NSPECIES];
+    __shared__ double wv[BLOCK / WAVE];
+    const int tid = threadIdx.x;
+    for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += BLOCK) sc[t] = coef[t];
+    const int i = blockIdx.x * BLOCK + tid;
+    const bool act = i < P.n;
+    double4 pi = act ? pos4[i] : make_double4(0, 0, 0, 0);
+    const int ti = act ? type[i] : 0;
+    const uint32_t ki = act ? cell_key(P, pi.x, pi.y, pi.z) : 0;
+    const uint32_t cxi = ki % P.dims[0], cyi = (ki / P.dims[0]) % P.dims[1], czi = ki / (P.dims[0] * P.dims[1]);
+    const bool refm = (P.flags & F_REF_MULT) != 0;
+    double fx = 0, fy = 0, fz = 0, pe = 0, sa = 0, vl = 0;
+    for (int base = 0; base < P.n; base += BLOCK) {
+        __syncthreads();
+        if (base + tid < P.n) {
+            tp[tid] = pos4[base + tid];
+            tt[tid] = type[base + tid];
+            tk[tid] = cell_key(P, tp[tid].x, tp[tid].y, tp[tid].z);
+        }
+        __syncthreads();
+        const int lim = min(BLOCK, P.n - base);
+        if (act) {
+            for (int jj = 0; jj < lim; ++jj) {
+                const int j = base + jj;
+                if (j == i) continue;
+                const double4 pj = tp[jj];
+                const double dx = min_image1(pi.x - pj.x, P.box[0]);
+                const double dy = min_image1(pi.y - pj.y, P.box[1]);
+                const double dz = min_image1(pi.z - pj.z, P.box[2]);
+                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                if (!(r2 < P.rc2 && r2 > P.eps2)) continue;
+                double mult = 1.0;
+                if (refm) {
+                    const uint32_t kj = tk[jj];
+                    const uint32_t cxj = kj % P.dims[0], cyj = (kj / P.dims[0]) % P.dims[1],
+                                   czj = kj / (P.dims[0] * P.dims[1]);
+                    if (P.dims[0] == 2 && cxi != cxj) mult *= 2.0;
+                    if (P.dims[1] == 2 && cyi != cyj) mult *= 2.0;
+                    if (P.dims[2] == 2 && czi != czj) mult *= 2.0;
+                }
+                const double inv_d = rsqrt_refined(r2);
+                const double d = r2 * inv_d;
+                const int tj = tt[jj];
+                double fmag, vv;
+                pair_law<COUL, false>(P, sc[ti * GS_NSPECIES + tj], d, inv_d, P.ke * pi.w * pj.w,
+                                      P.mass[ti] * P.mass[tj], fmag, vv);
+                fmag *= mult;
+                vv *= mult;
+                const double fs = fmag * inv_d;
+                fx = fma(fs, dx, fx);
+                fy = fma(fs, dy, fy);
+                fz = fma(fs, dz, fz);
+                vl += vv;
+                pe += vv;
+                sa += fabs(fmag);
+            }
+        }
+    }
+    if (act) {
+        const double im = P.inv_mass[ti];
+        ax[i] = fx * im;
+        ay[i] = fy * im;
+        az[i] = fz * im;
+        if (EXTRAS) {
+            pe_atom[i] = 0.5 * pe;
+            s_abs[i] = sa;
+        }
+    }
+    vl = wave_sum(vl);
+    if ((tid & 63) == 0) wv[tid >> 6] = vl;
+    __syncthreads();
+    if (tid == 0) vpart[blockIdx.x] = 0.5 * ((wv[0] + wv[1]) + (wv[2] + wv[3]));
+}
+
+// Deterministic block sum: fixed per-thread strides then a fixed tree.
+__device__ double block_sum_fixed(const double* __restrict__ a, uint32_t n, double* red) {
+    const int tid = threadIdx.x;
+    double s = 0.0;
+    for (uint32_t k = tid; k < n; k += BLOCK) s += a[k];
+    s = wave_sum(s);
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    __syncthreads();
+    double t = (red[0] + red[1]) + (red[2] + red[3]);
+    __syncthreads();
+    return t;
+}
+
+// Second half kick (src/systems/integrate.rs:38-44) fused with the kinetic
+// energy reduction, the Berendsen lambda (src/systems/thermostat.rs:47-71,
+// applied at the start of the next tick or by k_apply_lambda) and the tick's
+// observables (src/systems/aggregate.rs:86-121).  The last workgroup to
+// finish reduces the per-block partials in a fixed order.
+template <bool KICK>
+__global__ __launch_bounds__(BLOCK) void k_kick_ke(KParams P, double* __restrict__ vx, double* __restrict__ vy,
+                                                   double* __restrict__ vz, const double* __restrict__ ax,
+                                                   const double* __restrict__ ay, const double* __restrict__ az,
+                                                   const uint8_t* __restrict__ type, double* __restrict__ kpart,
+                                                   uint32_t* __restrict__ counter, const double* __restrict__ vpart,
+                                                   uint32_t nvpart, double* __restrict__ lam_out,
+                                                   gs_observables* __restrict__ obs, int thermostat, double target,
+                                                   double tau, double kconst) {
+    __shared__ double red[BLOCK / WAVE];
+    __shared__ bool last;
+    const int tid = threadIdx.x;
+    const int k = blockIdx.x * BLOCK + tid;
+    double ke = 0.0;
+    if (k < P.n) {
+        double a = vx[k], b = vy[k], c = vz[k];
+        if (KICK) {
+            a = a + ax[k] * P.half_dt;
+            b = b + ay[k] * P.half_dt;
+            c = c + az[k] * P.half_dt;
+            vx[k] = a;
+            vy[k] = b;
+            vz[k] = c;
+        }
+        ke = 0.5 * P.mass[type[k]] * (a * a + b * b + c * c);
+    }
+    ke = wave_sum(ke);
+    if ((tid & 63) == 0) red[tid >> 6] = ke;
+    __syncthreads();
+    if (tid == 0) {
+        kpart[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+        __threadfence();
+        const uint32_t t = atomicAdd(counter, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const double K = block_sum_fixed(kpart, gridDim.x, red);
+    const double V = vpart ? block_sum_fixed(vpart, nvpart, red) : 0.0;
+    if (tid == 0) {
+        double lam = 1.0;
+        const double T = (2.0 / 3.0) * (K / (double)P.n) / kconst;
+        if (thermostat && target > 0.0 && P.n > 0) {
+            if (isfinite(T) && T > 0.0) {
+                const double l2 = 1.0 + (1.0 / tau) * (target / T - 1.0);
+                lam = sqrt(fmax(l2, 0.0));
+            }
+        }
+        *lam_out = lam;
+        if (obs) {
+            const double Ka = lam * lam * K;
+            obs->kinetic = Ka;
+            obs->potential = V;
+            obs->bond_energy = 0.0;
+            obs->temperature = kconst > 0.0 ? (2.0 / 3.0) * (Ka / (double)P.n) / kconst : 0.0;
+            obs->lambda = lam;
+            obs->energy_total = Ka + V;
+        }
+        *counter = 0u;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_apply_lambda(int n, double* __restrict__ vx, double* __restrict__ vy,
+                                                        double* __restrict__ vz, double* __restrict__ lam) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    const double l = *lam;
+    if (k < n) {
+        vx[k] = vx[k] * l;
+        vy[k] = vy[k] * l;
+        vz[k] = vz[k] * l;
+    }
+}
+
+__global__ void k_set_scalar(double* p, double v) { *p = v; }
+
+__global__ __launch_bounds__(BLOCK) void k_reduce_to(const double* __restrict__ a, uint32_t n, double* __restrict__ out) {
+    __shared__ double red[BLOCK / WAVE];
+    const double s = block_sum_fixed(a, n, red);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// Individual systems.
+__global__ __launch_bounds__(BLOCK) void k_half_kick(KParams P, double* vx, double* vy, double* vz, const double* ax,
+                                                     const double* ay, const double* az) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= P.n) return;
+    vx[k] = vx[k] + ax[k] * P.half_dt;
+    vy[k] = vy[k] + ay[k] * P.half_dt;
+    vz[k] = vz[k] + az[k] * P.half_dt;
+}
+__global__ __launch_bounds__(BLOCK) void k_drift(KParams P, double4* pos4, const double* vx, const double* vy,
+                                                 const double* vz) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= P.n) return;
+    double4 p = pos4[k];
+    p.x = p.x + vx[k] * P.dt;
+    p.y = p.y + vy[k] * P.dt;
+    p.z = p.z + vz[k] * P.dt;
+    pos4[k] = p;
+}
+__global__ __launch_bounds__(BLOCK) void k_wrap(KParams P, double4* pos4) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= P.n) return;
+    double4 p = pos4[k];
+    p.x = rem_euclid(p.x, 2.0 * P.half[0]) - P.half[0];
+    p.y = rem_euclid(p.y, 2.0 * P.half[1]) - P.half[1];
+    p.z = rem_euclid(p.z, 2.0 * P.half[2]) - P.half[2];
+    pos4[k] = p;
+}
+
+// Sorted -> entity order for downloads.
+__global__ __launch_bounds__(BLOCK) void k_unpermute_state(int n, const uint32_t* __restrict__ id,
+                                                           const double4* __restrict__ pos4,
+                                                           const double* __restrict__ vx, const double* __restrict__ vy,
+                                                           const double* __restrict__ vz, const double* __restrict__ ax,
+                                                           const double* __restrict__ ay, const double* __restrict__ az,
+                                                           double* __restrict__ out) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t e = id[k];
+    const double4 p = pos4[k];
+    out[0 * (size_t)n + e] = p.x;
+    out[1 * (size_t)n + e] = p.y;
+    out[2 * (size_t)n + e] = p.z;
+    out[3 * (size_t)n + e] = vx[k];
+    out[4 * (size_t)n + e] = vy[k];
+    out[5 * (size_t)n + e] = vz[k];
+    out[6 * (size_t)n + e] = ax[k];
+    out[7 * (size_t)n + e] = ay[k];
+    out[8 * (size_t)n + e] = az[k];
+}
+__global__ __launch_bounds__(BLOCK) void k_unpermute2(int n, const uint32_t* __restrict__ id,
+                                                      const double* __restrict__ a, const double* __restrict__ b,
+                                                      double* __restrict__ out) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t e = id[k];
+    out[e] = a[k];
+    out[(size_t)n + e] = b[k];
+}
+__global__ __launch_bounds__(BLOCK) void k_cells_entity(KParams P, const uint32_t* __restrict__ id,
+                                                        const double4* __restrict__ pos4, uint32_t* __restrict__ out) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= P.n) return;
+    const double4 p = pos4[k];
+    out[id[k]] = cell_key(P, p.x, p.y, p.z);
+}
+
+// ---- observation lenses --------------------------------------------------------
+
+// Auxiliary grid of a lens: SpatialGrid::new(size, min_cell) over a point set
+// (src/physics/grid.rs:67-86).
+struct AuxGrid {
+    uint32_t dims[3];
+    double cell[3];
+    uint32_t ncell;
+};
+
+__device__ __forceinline__ uint32_t aux_key(const AuxGrid& G, const KParams& P, double x, double y, double z,
+                                            uint32_t c[3]) {
+    c[0] = cell_axis(x, P.box[0], G.cell[0], G.dims[0]);
+    c[1] = cell_axis(y, P.box[1], G.cell[1], G.dims[1]);
+    c[2] = cell_axis(z, P.box[2], G.cell[2], G.dims[2]);
+    return c[0] + G.dims[0] * (c[1] + G.dims[1] * c[2]);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_aux_key(int n, AuxGrid G, KParams P, const double4* __restrict__ pts,
+                                                   uint32_t* __restrict__ key, uint32_t* __restrict__ ticket,
+                                                   uint32_t* __restrict__ count) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= n) return;
+    const double4 p = pts[k];
+    uint32_t c[3];
+    const uint32_t kk = aux_key(G, P, p.x, p.y, p.z, c);
+    key[k] = kk;
+    ticket[k] = atomicAdd(&count[kk], 1u);
+}
+
+// Stencil offsets of one axis as the reference grid enumerates them:
+// 1 cell -> {0} (canonical_offset zeroes the axis); 2 cells -> {-1, 0, +1}
+// in reference mode, both non-zero offsets reaching the same cell (the
+// pair multiplicity of src/physics/grid.rs:134-151), or {0, +1} in min-image
+// mode; >= 3 cells -> {-1, 0, +1}.
+__device__ __forceinline__ int axis_offsets(uint32_t dims, bool refm, int* o) {
+    if (dims == 1) { o[0] = 0; return 1; }
+    if (dims == 2 && !refm) { o[0] = 0; o[1] = 1; return 2; }
+    o[0] = -1; o[1] = 0; o[2] = 1;
+    return 3;
+}
+
+// g(r) pair counts (radial_distribution, src/analysis/mod.rs:103-117): every
+// unordered pair of the subsample found by the grid of edge r_max, counted
+// from its lower-index end, with the reference's min_image, length and
+// (d / dr) as usize.  Integer counts are order independent, so the u32 LDS
+// histogram with atomics is exact.
+template <bool BETWEEN>
+__global__ __launch_bounds__(BLOCK) void k_gr_pairs(int n, AuxGrid G, KParams P, const double4* __restrict__ pts,
+                                                    const uint8_t* __restrict__ ptype,
+                                                    const uint32_t* __restrict__ start,
+                                                    const uint32_t* __restrict__ perm, double r_max, double dr,
+                                                    int nbins, int ta, int tb, int refm,
+                                                    unsigned long long* __restrict__ bins) {
+    extern __shared__ uint32_t hist[];
+    const bool lds = nbins <= 4096;
+    if (lds)
+        for (int b = threadIdx.x; b < nbins; b += BLOCK) hist[b] = 0u;
+    __syncthreads();
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) {
+        const double4 p = pts[i];
+        const int ti = BETWEEN ? ptype[i] : 0;
+        uint32_t c[3];
+        aux_key(G, P, p.x, p.y, p.z, c);
+        int ox[3], oy[3], oz[3];
+        const int nx = axis_offsets(G.dims[0], refm, ox), ny = axis_offsets(G.dims[1], refm, oy),
+                  nz = axis_offsets(G.dims[2], refm, oz);
+        const double cut2 = r_max * r_max;
+        for (int a = 0; a < nz; ++a)
+            for (int b = 0; b < ny; ++b)
+                for (int e = 0; e < nx; ++e) {
+                    const uint32_t ix = (c[0] + G.dims[0] + ox[e]) % G.dims[0];
+                    const uint32_t iy = (c[1] + G.dims[1] + oy[b]) % G.dims[1];
+                    const uint32_t iz = (c[2] + G.dims[2] + oz[a]) % G.dims[2];
+                    const uint32_t nc = ix + G.dims[0] * (iy + G.dims[1] * iz);
+                    for (uint32_t t = start[nc]; t < start[nc + 1]; ++t) {
+                        const uint32_t j = perm[t];
+                        if ((int)j <= i) continue;
+                        if (BETWEEN) {
+                            const int tj = ptype[j];
+                            if (!((ti == ta && tj == tb) || (ti == tb && tj == ta))) continue;
+                        }
+                        const double4 q = pts[j];
+                        const double dx = min_image1(p.x - q.x, P.box[0]);
+                        const double dy = min_image1(p.y - q.y, P.box[1]);
+                        const double dz = min_image1(p.z - q.z, P.box[2]);
+                        const double d2 = dx * dx + dy * dy + dz * dz;
+                        if (!(d2 < cut2)) continue;
+                        const double d = sqrt(d2);
+                        if (d <= F64_EPS || d >= r_max) continue;
+                        double fb = d / dr;
+                        uint32_t bin = fb >= (double)nbins ? (uint32_t)(nbins - 1) : (uint32_t)fb;
+                        if (lds)
+                            atomicAdd(&hist[bin], 1u);
+                        else
+                            atomicAdd(&bins[bin], 1ull);
+                    }
+                }
+    }
+    __syncthreads();
+    if (lds)
+        for (int b = threadIdx.x; b < nbins; b += BLOCK)
+            if (hist[b]) atomicAdd(&bins[b], (unsigned long long)hist[b]);
+}
+
+// Subsample of g_subsample (src/analysis/mod.rs:32-46) gathered from the
+// sorted state: entity e is kept when e % step == 0 and e / step < count.
+__global__ __launch_bounds__(BLOCK) void k_gr_gather(int n, const uint32_t* __restrict__ id,
+                                                     const double4* __restrict__ pos4,
+                                                     const uint8_t* __restrict__ type, uint32_t step, uint32_t count,
+                                                     double4* __restrict__ sub, uint8_t* __restrict__ stype) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t e = id[k];
+    if (e % step) return;
+    const uint32_t s = e / step;
+    if (s >= count) return;
+    sub[s] = pos4[k];
+    stype[s] = type[k];
+}
+
+// Friends-of-friends (clusters, src/analysis/mod.rs:221-284) as a concurrent
+// union-find over entity ids: a root is only ever hooked under a smaller
+// root, so each component ends rooted at its smallest entity id whatever the
+// order of the hooks (the canonical label), and the partition is exact.
+__device__ __forceinline__ uint32_t uf_find(uint32_t* parent, uint32_t x) {
+    uint32_t p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (p != x) {
+        x = p;
+        p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return x;
+}
+
+__device__ __forceinline__ void uf_union(uint32_t* parent, uint32_t a, uint32_t b) {
+    while (true) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) return;
+        if (a > b) {
+            const uint32_t t = a;
+            a = b;
+            b = t;
+        }
+        const uint32_t old = atomicCAS(&parent[b], b, a);
+        if (old == b) return;
+        b = old;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fof_edges(int n, AuxGrid G, KParams P, const double4* __restrict__ pos4,
+                                                     const uint8_t* __restrict__ type, const uint32_t* __restrict__ id,
+                                                     const uint32_t* __restrict__ start,
+                                                     const uint32_t* __restrict__ perm,
+                                                     const double* __restrict__ sigma, double bond_factor,
+                                                     double cutoff, int refm, uint32_t* __restrict__ parent,
+                                                     unsigned long long* __restrict__ bound) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    unsigned long long local = 0;
+    if (k < n) {
+        const double4 p = pos4[k];
+        const uint32_t ek = id[k];
+        const double sk = sigma[type[k]];
+        uint32_t c[3];
+        aux_key(G, P, p.x, p.y, p.z, c);
+        int ox[3], oy[3], oz[3];
+        const int nx = axis_offsets(G.dims[0], refm, ox), ny = axis_offsets(G.dims[1], refm, oy),
+                  nz = axis_offsets(G.dims[2], refm, oz);
+        const double cut2 = cutoff * cutoff;
+        for (int a = 0; a < nz; ++a)
+            for (int b = 0; b < ny; ++b)
+                for (int e = 0; e < nx; ++e) {
+                    const uint32_t ix = (c[0] + G.dims[0] + ox[e]) % G.dims[0];
+                    const uint32_t iy = (c[1] + G.dims[1] + oy[b]) % G.dims[1];
+                    const uint32_t iz = (c[2] + G.dims[2] + oz[a]) % G.dims[2];
+                    const uint32_t nc = ix + G.dims[0] * (iy + G.dims[1] * iz);
+                    for (uint32_t t = start[nc]; t < start[nc + 1]; ++t) {
+                        const uint32_t m = perm[t];
+                        const uint32_t em = id[m];
+                        if (em <= ek) continue;
+                        const double4 q = pos4[m];
+                        const double dx = min_image1(p.x - q.x, P.box[0]);
+                        const double dy = min_image1(p.y - q.y, P.box[1]);
+                        const double dz = min_image1(p.z - q.z, P.box[2]);
+                        const double d2 = dx * dx + dy * dy + dz * dz;
+                        if (!(d2 < cut2)) continue;
+                        const double d = sqrt(d2);
+                        if (d <= F64_EPS) continue;
+                        const double bond = bond_factor * 0.5 * (sk + sigma[type[m]]);
+                        if (d < bond) {
+                            local += 1;
+                            uf_union(parent, ek, em);
+                        }
+                    }
+                }
+    }
+    if (local) atomicAdd(bound, local);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_iota(int n, uint32_t* __restrict__ a) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k < n) a[k] = (uint32_t)k;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fof_finalize(int n, uint32_t* __restrict__ parent,
+                                                        uint32_t* __restrict__ size) {
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t r = uf_find(parent, (uint32_t)e);
+    atomicAdd(&size[r], 1u);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fof_labels(int n, uint32_t* __restrict__ parent,
+                                                      uint32_t* __restrict__ label) {
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= n) return;
+    label[e] = uf_find(parent, (uint32_t)e);
+}
+
+// monomers, aggregates, largest, roots (u64 each).
+__global__ __launch_bounds__(BLOCK) void k_fof_stats(int n, const uint32_t* __restrict__ size,
+                                                     unsigned long long* __restrict__ st) {
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t s = size[e];
+    if (s == 0) return;
+    atomicAdd(&st[3], 1ull);
+    if (s == 1)
+        atomicAdd(&st[0], 1ull);
+    else
+        atomicAdd(&st[1], 1ull);
+    atomicMax(&st[2], (unsigned long long)s);
+}
+
+// =============================================================================
+// Host side
+// =============================================================================
+
+struct gs_universe {
+    gs_params prm;
+    KParams kp;
+    int n = 0;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    bool have_state = false;
+    bool small_box = false;
+    bool extras_valid = false;
+    bool coul = false, shared_switch = false, single = false;
+    int cur = 0;
+    // resident state (double-buffered where the rebuild permutes)
+    double4* pos4[2] = {nullptr, nullptr};
+    double* v[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    uint8_t* type[2] = {nullptr, nullptr};
+    uint32_t* id[2] = {nullptr, nullptr};
+    double* a[3] = {nullptr, nullptr, nullptr};
+    // cell list
+    uint32_t *key = nullptr, *ticket = nullptr, *count = nullptr, *cstart = nullptr, *perm = nullptr;
+    // coefficients, reductions, scalars
+    PairCoef* coef = nullptr;
+    double* vpart = nullptr;
+    uint32_t nvpart = 0;
+    double* kpart = nullptr;
+    uint32_t* counter = nullptr;
+    double* scal = nullptr;  // [0] lambda, [1] V, [2] K, [3] spare
+    double *pe = nullptr, *sabs = nullptr;
+    double* stage = nullptr;  // 9n doubles
+    uint32_t* stage_u = nullptr;
+    gs_observables* obs = nullptr;
+    int64_t obs_cap = 0;
+    // lens scratch (allocated on first use)
+    double4* l_pts = nullptr;
+    uint8_t* l_type = nullptr;
+    uint32_t *l_key = nullptr, *l_ticket = nullptr, *l_perm = nullptr, *l_count = nullptr, *l_start = nullptr;
+    uint32_t l_ncell_cap = 0;
+    uint32_t* l_parent = nullptr;
+    uint32_t* l_size = nullptr;
+    unsigned long long* l_bins = nullptr;
+    int l_bins_cap = 0;
+    unsigned long long* l_u64 = nullptr;  // [0..3] FoF stats, [4] bound pairs
+    double* l_sigma = nullptr;
+    bool present[GS_NSPECIES] = {false};
+    // event timing of gs_step
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    gs_timing tm = {0, 0.0, 0.0, 0.0, 0.0, 0};
+};
+
+namespace {
+
+int validate_params(const gs_params* p, int64_t n) {
+    if (!p) return fail(GS_E_INVALID, "params is NULL");
+    if (n < 0 || n > (int64_t)0x7FFFFFFF) return fail(GS_E_INVALID, "n_atoms out of range");
+    for (int k = 0; k < 3; ++k)
+        if (!(p->box[k] > 0.0) || !std::isfinite(p->box[k])) return fail(GS_E_INVALID, "box must be finite and > 0");
+    if (!std::isfinite(p->dt)) return fail(GS_E_INVALID, "dt must be finite");
+    for (int s = 0; s < GS_NSPECIES; ++s) {
+        if (!(p->sigma[s] > 0.0) || !std::isfinite(p->sigma[s])) return fail(GS_E_INVALID, "sigma must be > 0");
+        if (!(p->epsilon_k[s] >= 0.0) || !std::isfinite(p->epsilon_k[s]))
+            return fail(GS_E_INVALID, "epsilon_k must be >= 0");
+        if (!(p->mass[s] > 0.0) || !std::isfinite(p->mass[s])) return fail(GS_E_INVALID, "mass must be > 0");
+    }
+    if (!(p->thermal_constant > 0.0)) return fail(GS_E_INVALID, "thermal_constant must be > 0");
+    if (p->grid_semantics != GS_GRID_REFERENCE && p->grid_semantics != GS_GRID_MINIMAGE)
+        return fail(GS_E_INVALID, "unknown grid_semantics");
+    return GS_OK;
+}
+
+// Host part of LjTable::new and ForceSystem::new
+// (src/physics/forces.rs:223-237, src/systems/forces.rs:100-142).
+std::vector<PairCoef> derive(gs_universe* u) {
+    const gs_params& p = u->prm;
+    KParams& k = u->kp;
+    std::memset(&k, 0, sizeof(k));
+    double max_sigma = 0.0;
+    for (int s = 0; s < GS_NSPECIES; ++s) max_sigma = std::fmax(max_sigma, p.sigma[s]);
+    double max_pair_sigma = 0.0;
+    std::vector<PairCoef> coef(GS_NSPECIES * GS_NSPECIES);
+    const double c2m13 = std::pow(2.0, -1.0 / 3.0), c2p16 = std::pow(2.0, 1.0 / 6.0);
+    double max_rcore = 0.0;
+    for (int i = 0; i < GS_NSPECIES; ++i)
+        for (int j = 0; j < GS_NSPECIES; ++j) {
+            const double sig = 0.5 * (p.sigma[i] + p.sigma[j]);
+            const double eps = p.thermal_constant * std::sqrt(p.epsilon_k[i] * p.epsilon_k[j]);
+            max_pair_sigma = std::fmax(max_pair_sigma, sig);
+            PairCoef& c = coef[i * GS_NSPECIES + j];
+            c.sigma = sig;
+            c.eps4 = 4.0 * eps;
+            c.c24 = 24.0 * eps / sig;
+            c.rcore = 0.5 * sig;
+            c.inv_rcore = 1.0 / c.rcore;
+            c.bond_k = 72.0 * c2m13 * eps / (sig * sig);
+            c.bond_req = sig * c2p16;
+            c.bond_rc = p.bond_k_bind * sig;
+            max_rcore = std::fmax(max_rcore, c.rcore);
+        }
+    const double lj_rc = std::fmax(p.lj_cutoff_factor, 1.0) * max_pair_sigma;
+    const double c_rc = p.enable_electrostatics ? std::fmax(p.coulomb_cutoff, 1.0) * max_sigma : 0.0;
+    const double g_rc = p.enable_gravity ? std::fmax(p.gravity_cutoff, 1.0) * max_sigma : 0.0;
+    const double rc = std::fmax(std::fmax(lj_rc, c_rc), g_rc);
+    for (int a = 0; a < 3; ++a) {
+        k.box[a] = p.box[a];
+        k.half[a] = p.box[a] * 0.5;
+        double nc = std::floor(p.box[a] / rc);
+        if (!(nc >= 1.0)) nc = 1.0;
+        if (nc > 1024.0) nc = 1024.0;
+        k.dims[a] = (uint32_t)nc;
+        k.cell[a] = p.box[a] / (double)k.dims[a];
+    }
+    k.ncell = k.dims[0] * k.dims[1] * k.dims[2];
+    k.n = u->n;
+    k.dt = p.dt;
+    k.half_dt = 0.5 * p.dt;
+    k.rc = rc;
+    k.rc2 = rc * rc;
+    k.eps2 = F64_EPS * F64_EPS;
+    k.lj_rc = lj_rc;
+    k.lj_ron = 0.9 * lj_rc;
+    k.lj_inv_w = 1.0 / (lj_rc - k.lj_ron);
+    k.c_rc = c_rc;
+    k.c_ron = 0.9 * c_rc;
+    k.c_inv_w = c_rc > 0 ? 1.0 / (c_rc - k.c_ron) : 0.0;
+    k.ke = p.enable_electrostatics ? p.coulomb_constant : 0.0;
+    k.g_rc = g_rc;
+    k.g_ron = 0.9 * g_rc;
+    k.g_inv_w = g_rc > 0 ? 1.0 / (g_rc - k.g_ron) : 0.0;
+    k.gconst = p.gravity_constant;
+    for (int s = 0; s < GS_NSPECIES; ++s) {
+        k.mass[s] = p.mass[s];
+        k.inv_mass[s] = 1.0 / p.mass[s];
+    }
+    int flags = 0;
+    if (p.enable_electrostatics) flags |= F_COULOMB;
+    if (p.enable_gravity) flags |= F_GRAVITY;
+    if (p.enable_boundaries) flags |= F_WRAP;
+    if (p.grid_semantics == GS_GRID_REFERENCE) flags |= F_REF_MULT;
+    u->coul = p.enable_electrostatics != 0;
+    // The Coulomb term may reuse the LJ switch when both laws share the
+    // cutoff and neither core clamp can reach the switching shell.
+    u->shared_switch = u->coul && c_rc == lj_rc && max_rcore < k.lj_ron && 0.5 < k.lj_ron;
+    if (u->shared_switch) flags |= F_SHARED_SWITCH;
+    k.flags = flags;
+    u->small_box = k.dims[0] < 3 || k.dims[1] < 3 || k.dims[2] < 3;
+    // Single-species fast path: every pair coefficient equal.
+    u->single = !u->coul && !p.enable_gravity;
+    if (u->single)
+        for (int s = 1; s < GS_NSPECIES * GS_NSPECIES; ++s)
+            if (std::memcmp(&coef[s], &coef[0], sizeof(PairCoef)) != 0) {
+                u->single = false;
+                break;
+            }
+    static_assert(sizeof(PairCoef) == 64, "PairCoef layout");
+    u->nvpart = u->small_box ? nblocks(u->n) : k.ncell;
+    if (u->nvpart == 0) u->nvpart = 1;
+    return coef;
+}
+
+template <typename T>
+int dalloc(T** p, size_t count) {
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) return fail(GS_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return GS_OK;
+}
+
+int check_state(gs_universe* u) {
+    if (!u) return fail(GS_E_INVALID, "universe handle is NULL");
+    if (!u->have_state) return fail(GS_E_STATE, "no state uploaded");
+    return GS_OK;
+}
+
+// Cell-list rebuild: key + histogram (optionally fused with kick/drift/wrap),
+// scan, scatter, per-cell id ordering + gather into the other buffer.
+int rebuild(gs_universe* u, bool kick_drift) {
+    const KParams& P = u->kp;
+    const int n = u->n;
+    if (n == 0) return GS_OK;
+    const int c = u->cur, o = 1 - c;
+    if (kick_drift)
+        k_integrate_key<true><<<nblocks(n), BLOCK, 0, u->stream>>>(P, u->pos4[c], u->v[c][0], u->v[c][1], u->v[c][2],
+                                                                  u->a[0], u->a[1], u->a[2], u->scal, u->key,
+                                                                  u->ticket, u->count);
+    else
+        k_integrate_key<false><<<nblocks(n), BLOCK, 0, u->stream>>>(P, u->pos4[c], u->v[c][0], u->v[c][1],
+                                                                   u->v[c][2], u->a[0], u->a[1], u->a[2], u->scal,
+                                                                   u->key, u->ticket, u->count);
+    k_scan<<<1, 1024, 0, u->stream>>>(u->count, u->cstart, P.ncell);
+    k_scatter<<<nblocks(n), BLOCK, 0, u->stream>>>(n, u->key, u->ticket, u->cstart, u->perm);
+    k_sort_gather<<<nblocks(P.ncell, BLOCK / WAVE), BLOCK, 0, u->stream>>>(
+        P.ncell, u->cstart, u->perm, u->pos4[c], u->v[c][0], u->v[c][1], u->v[c][2], u->type[c], u->id[c],
+        u->pos4[o], u->v[o][0], u->v[o][1], u->v[o][2], u->type[o], u->id[o]);
+    u->cur = o;
+    HIPCHK(hipGetLastError());
+    return GS_OK;
+}
+
+template <bool EXTRAS>
+int launch_pair(gs_universe* u) {
+    const KParams& P = u->kp;
+    const int c = u->cur;
+    if (u->small_box) {
+        if (u->coul)
+            k_pair_brute<true, EXTRAS><<<nblocks(u->n), BLOCK, 0, u->stream>>>(
+                P, u->pos4[c], u->type[c], u->coef, u->a[0], u->a[1], u->a[2], u->pe, u->sabs, u->vpart);
+        else
+            k_pair_brute<false, EXTRAS><<<nblocks(u->n), BLOCK, 0, u->stream>>>(
+                P, u->pos4[c], u->type[c], u->coef, u->a[0], u->a[1], u->a[2], u->pe, u->sabs, u->vpart);
+    } else {
+#define GS_PAIR(C, S, E, SG)                                                                                   \
+    k_pair<C, S, E, SG><<<P.ncell, PAIR_THREADS, 0, u->stream>>>(P, u->cstart, u->pos4[c], u->type[c], u->coef, \
+                                                                  u->a[0], u->a[1], u->a[2], u->pe, u->sabs,   \
+                                                                  u->vpart)
+        if (u->single)
+            GS_PAIR(false, false, EXTRAS, true);
+        else if (!u->coul)
+            GS_PAIR(false, false, EXTRAS, false);
+        else if (u->shared_switch)
+            GS_PAIR(true, true, EXTRAS, false);
+        else
+            GS_PAIR(true, false, EXTRAS, false);
+#undef GS_PAIR
+    }
+    HIPCHK(hipGetLastError());
+    return GS_OK;
+}
+
+}  // namespace
+
+// =============================================================================
+// C ABI
+// =============================================================================
+extern "C" {
+
+const char* gs_last_error(void) { return g_err.c_str(); }
+
+int gs_device_info(int* device, char* name, int name_len) {
+    int d = 0;
+    HIPCHK(hipGetDevice(&d));
+    if (device) *device = d;
+    if (name && name_len > 0) {
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, d));
+        std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    return GS_OK;
+}
+
+int gs_create(const gs_params* params, int64_t n_atoms, gs_universe** out) {
+    if (!out) return fail(GS_E_INVALID, "out is NULL");
+    *out = nullptr;
+    int rc = validate_params(params, n_atoms);
+    if (rc) return rc;
+    gs_universe* u = new gs_universe();
+    u->n = (int)n_atoms;
+    std::memcpy(&u->prm, params, sizeof(gs_params));
+    std::vector<PairCoef> coef = derive(u);
+    (void)hipGetDevice(&u->dev);
+    const size_t n = (size_t)std::max(1, u->n);
+    const KParams& P = u->kp;
+    auto bail = [&](int code) {
+        gs_destroy(u);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(GS_E_HIP, "hipStreamCreate failed"));
+    for (int b = 0; b < 2; ++b) {
+        if ((rc = dalloc(&u->pos4[b], n))) return bail(rc);
+        for (int a = 0; a < 3; ++a)
+            if ((rc = dalloc(&u->v[b][a], n))) return bail(rc);
+        if ((rc = dalloc(&u->type[b], n))) return bail(rc);
+        if ((rc = dalloc(&u->id[b], n))) return bail(rc);
+    }
+    for (int a = 0; a < 3; ++a)
+        if ((rc = dalloc(&u->a[a], n))) return bail(rc);
+    if ((rc = dalloc(&u->key, n)) || (rc = dalloc(&u->ticket, n)) || (rc = dalloc(&u->perm, n)) ||
+        (rc = dalloc(&u->count, (size_t)P.ncell)) || (rc = dalloc(&u->cstart, (size_t)P.ncell + 1)) ||
+        (rc = dalloc(&u->coef, coef.size())) || (rc = dalloc(&u->vpart, (size_t)u->nvpart)) ||
+        (rc = dalloc(&u->kpart, (size_t)nblocks(n))) || (rc = dalloc(&u->counter, 4)) ||
+        (rc = dalloc(&u->scal, 4)) || (rc = dalloc(&u->pe, n)) || (rc = dalloc(&u->sabs, n)) ||
+        (rc = dalloc(&u->stage, 9 * n)) || (rc = dalloc(&u->stage_u, n)))
+        return bail(rc);
+    if (hipMemsetAsync(u->count, 0, sizeof(uint32_t) * P.ncell, u->stream) != hipSuccess ||
+        hipMemsetAsync(u->counter, 0, sizeof(uint32_t) * 4, u->stream) != hipSuccess ||
+        hipMemsetAsync(u->vpart, 0, sizeof(double) * u->nvpart, u->stream) != hipSuccess ||
+        hipMemcpyAsync(u->coef, coef.data(), coef.size() * sizeof(PairCoef), hipMemcpyHostToDevice, u->stream) !=
+            hipSuccess)
+        return bail(fail(GS_E_HIP, "device initialisation failed"));
+    k_set_scalar<<<1, 1, 0, u->stream>>>(u->scal, 1.0);
+    if (hipStreamSynchronize(u->stream) != hipSuccess) return bail(fail(GS_E_HIP, "device initialisation failed"));
+    *out = u;
+    return GS_OK;
+}
+
+int gs_destroy(gs_universe* u) {
+    if (!u) return GS_OK;
+    if (u->stream) (void)hipStreamSynchronize(u->stream);
+    for (int b = 0; b < 2; ++b) {
+        (void)hipFree(u->pos4[b]);
+        for (int a = 0; a < 3; ++a) (void)hipFree(u->v[b][a]);
+        (void)hipFree(u->type[b]);
+        (void)hipFree(u->id[b]);
+    }
+    for (int a = 0; a < 3; ++a) (void)hipFree(u->a[a]);
+    (void)hipFree(u->key);
+    (void)hipFree(u->ticket);
+    (void)hipFree(u->perm);
+    (void)hipFree(u->count);
+    (void)hipFree(u->cstart);
+    (void)hipFree(u->coef);
+    (void)hipFree(u->vpart);
+    (void)hipFree(u->kpart);
+    (void)hipFree(u->counter);
+    (void)hipFree(u->scal);
+    (void)hipFree(u->pe);
+    (void)hipFree(u->sabs);
+    (void)hipFree(u->stage);
+    (void)hipFree(u->stage_u);
+    (void)hipFree(u->obs);
+    (void)hipFree(u->l_pts);
+    (void)hipFree(u->l_type);
+    (void)hipFree(u->l_key);
+    (void)hipFree(u->l_ticket);
+    (void)hipFree(u->l_perm);
+    (void)hipFree(u->l_count);
+    (void)hipFree(u->l_start);
+    (void)hipFree(u->l_parent);
+    (void)hipFree(u->l_size);
+    (void)hipFree(u->l_bins);
+    (void)hipFree(u->l_u64);
+    (void)hipFree(u->l_sigma);
+    for (hipEvent_t e : u->ev) (void)hipEventDestroy(e);
+    if (u->stream) (void)hipStreamDestroy(u->stream);
+    delete u;
+    return GS_OK;
+}
+
+int gs_upload_state(gs_universe* u, const double* x, const double* y, const double* z, const double* vx,
+                    const double* vy, const double* vz, const double* ax, const double* ay, const double* az,
+                    const uint8_t* species, const double* charge) {
+    if (!u) return fail(GS_E_INVALID, "universe handle is NULL");
+    const int n = u->n;
+    if (n > 0 && (!x || !y || !z || !vx || !vy || !vz || !species || !charge))
+        return fail(GS_E_INVALID, "state arrays must not be NULL");
+    std::vector<double4> p4((size_t)n);
+    std::vector<uint32_t> id((size_t)n);
+    bool present[GS_NSPECIES] = {false};
+    for (int i = 0; i < n; ++i) {
+        if (species[i] >= GS_NSPECIES) return fail(GS_E_INVALID, "species index out of range");
+        present[species[i]] = true;
+        if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || !std::isfinite(z[i]) || !std::isfinite(vx[i]) ||
+            !std::isfinite(vy[i]) || !std::isfinite(vz[i]) || !std::isfinite(charge[i]))
+            return fail(GS_E_NONFINITE, "non-finite value in the uploaded state");
+        p4[i] = make_double4(x[i], y[i], z[i], charge[i]);
+        id[i] = (uint32_t)i;
+    }
+    const int c = u->cur;
+    const size_t b = sizeof(double) * (size_t)n;
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(u->pos4[c], p4.data(), sizeof(double4) * (size_t)n, hipMemcpyHostToDevice, u->stream));
+        HIPCHK(hipMemcpyAsync(u->v[c][0], vx, b, hipMemcpyHostToDevice, u->stream));
+        HIPCHK(hipMemcpyAsync(u->v[c][1], vy, b, hipMemcpyHostToDevice, u->stream));
+        HIPCHK(hipMemcpyAsync(u->v[c][2], vz, b, hipMemcpyHostToDevice, u->stream));
+        const double* acc[3] = {ax, ay, az};
+        for (int a = 0; a < 3; ++a) {
+            if (acc[a])
+                HIPCHK(hipMemcpyAsync(u->a[a], acc[a], b, hipMemcpyHostToDevice, u->stream));
+            else
+                HIPCHK(hipMemsetAsync(u->a[a], 0, b, u->stream));
+        }
+        HIPCHK(hipMemcpyAsync(u->type[c], species, (size_t)n, hipMemcpyHostToDevice, u->stream));
+        HIPCHK(hipMemcpyAsync(u->id[c], id.data(), sizeof(uint32_t) * (size_t)n, hipMemcpyHostToDevice, u->stream));
+    }
+    k_set_scalar<<<1, 1, 0, u->stream>>>(u->scal, 1.0);
+    HIPCHK(hipStreamSynchronize(u->stream));
+    u->have_state = true;
+    u->extras_valid = false;
+    std::memcpy(u->present, present, sizeof(present));
+    return GS_OK;
+}
+
+int gs_download_state(gs_universe* u, double* x, double* y, double* z, double* vx, double* vy, double* vz, double* ax,
+                      double* ay, double* az) {
+    int rc = check_state(u);
+    if (rc) return rc;
+    const int n = u->n;
+    if (n == 0) return GS_OK;
+    const int c = u->cur;
+    k_unpermute_state<<<nblocks(n), BLOCK, 0, u->stream>>>(n, u->id[c], u->pos4[c], u->v[c][0], u->v[c][1],
+                                                           u->v[c][2], u->a[0], u->a[1], u->a[2], u->stage);
+    HIPCHK(hipGetLastError());
+    double* outs[9] = {x, y, z, vx, vy, vz, ax, ay, az};
+    for (int k = 0; k < 9; ++k)
+        if (outs[k])
+            HIPCHK(hipMemcpyAsync(outs[k], u->stage + (size_t)k * n, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost,
+                                  u->stream));
+    HIPCHK(hipStreamSynchronize(u->stream));
+    return GS_OK;
+}
+
+int gs_forces(gs_universe* u, double* potential, double* bond_energy) {
+    int rc = check_state(u);
+    if (rc) return rc;
+    if (u->n == 0) {
+        if (potential) *potential = 0.0;
+        if (bond_energy) *bond_energy = 0.0;
+        return GS_OK;
+    }
+    if ((rc = rebuild(u, false))) return rc;
+    if ((rc = launch_pair<true>(u))) return rc;
+    k_reduce_to<<<1, BLOCK, 0, u->stream>>>(u->vpart, u->nvpart, u->scal + 1);
+    HIPCHK(hipGetLastError());
+    double v = 0.0;
+    HIPCHK(hipMemcpyAsync(&v, u->scal + 1, sizeof(double), hipMemcpyDeviceToHost, u->stream));
+    HIPCHK(hipStreamSynchronize(u->stream));
+    if (potential) *potential = v;
+    if (bond_energy) *bond_energy = 0.0;
+    u->extras_valid = true;
+    return GS_OK;
+}
+
+int gs_download_force_extras(gs_universe* u, double* pe_atom, double* s_abs) {
+    int rc = check_state(u);
+    if (rc) return rc;
+    if (!u->extras_valid) return fail(GS_E_STATE, "no gs_forces result to download");
+    const int n = u->n;
+    if (n == 0) return GS_OK;
+    k_unpermute2<<<nblocks(n), BLOCK, 0, u->stream>>>(n, u->id[u->cur], u->pe, u->sabs, u->stage);
+    HIPCHK(hipGetLastError());
+    if (pe_atom)
+        HIPCHK(hipMemcpyAsync(pe_atom, u->stage, sizeof(double) * n, hipMemcpyDeviceToHost, u->stream));
+    if (s_abs)
+        HIPCHK(hipMemcpyAsync(s_abs, u->stage + n, sizeof(double) * n, hipMemcpyDeviceToHost, u->stream));
+    HIPCHK(hipStreamSynchronize(u->stream));
+    return GS_OK;
+}
+
+int gs_download_cells(gs_universe* u, uint32_t* cell, uint32_t* dims) {
+    int rc = check_state(u);
+    if (rc) return rc;
+    if (dims) {
+        dims[0] = u->kp.dims[0];
+        dims[1] = u->kp.dims[1];
+        dims[2] = u->kp.dims[2];
+    }
+    const int n = u->n;
+    if (n == 0 || !cell) return GS_OK;
+    k_cells_entity<<<nblocks(n), BLOCK, 0, u->stream>>>(u->kp, u->id[u->cur], u->pos4[u->cur], u->stage_u);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(cell, u->stage_u, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, u->stream));
+    HIPCHK(hipStreamSynchronize(u->stream));
+    return GS_OK;
+}
+
+int gs_half_kick(gs_universe* u) {
+    int rc = check_state(u);
+    if (rc) return rc;
+    if (u->n == 0) return GS_OK;
+    const int c = u->cur;
+    k_half_kick<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->kp, u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1],
+                                                         u->a[2]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(u->stream));
+    return GS_OK;
+}
+
+int gs_drift(gs_universe* u) {
+    int rc = check_state(u);
+    if (rc) return rc;
+    if (u->n == 0) return GS_OK;
+    const int c = u->cur;
+    k_drift<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->kp, u->pos4[c], u->v[c][0], u->v[c][1], u->v[c][2]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(u->stream));
+    return GS_OK;
+}
+
+int gs_wrap(gs_universe* u) {
+    int rc = check_state(u);
+    if (rc) return rc;
+    if (u->n == 0) return GS_OK;
+    k_wrap<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->kp, u->pos4[u->cur]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(u->stream));
+    return GS_OK;
+}
+
+static int reduce_ke(gs_universe* u, bool thermostat, gs_observables* obs_dev) {
+    const int c = u->cur;
+    const gs_params& p = u->prm;
+    k_kick_ke<false><<<nblocks(u->n), BLOCK, 0, u->stream>>>(
+        u->kp, u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->type[c], u->kpart, u->counter,
+        nullptr, 0, u->scal, obs_dev, thermostat ? 1 : 0, p.thermostat_temperature, std::fmax(p.thermostat_tau, 1.0),
+        p.thermal_constant);
+    HIPCHK(hipGetLastError());
+    return GS_OK;
+}
+
+int gs_thermostat(gs_universe* u, double* lambda) {
+    int rc = check_state(u);
+    if (rc) return rc;
+    if (u->n == 0) {
+        if (lambda) *lambda = 1.0;
+        return GS_OK;
+    }
+    if ((rc = reduce_ke(u, true, nullptr))) return rc;
+    const int c = u->cur;
+    k_apply_lambda<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->n, u->v[c][0], u->v[c][1], u->v[c][2], u->scal);
+    double l = 1.0;
+    HIPCHK(hipMemcpyAsync(&l, u->scal, sizeof(double), hipMemcpyDeviceToHost, u->stream));
+    k_set_scalar<<<1, 1, 0, u->stream>>>(u->scal, 1.0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(u->stream));
+    if (lambda) *lambda = l;
+    return GS_OK;
+}
+
+int gs_kinetic(gs_universe* u, double* kinetic) {
+    int rc = check_state(u);
+    if (rc) return rc;
+    if (u->n == 0) {
+        if (kinetic) *kinetic = 0.0;
+        return GS_OK;
+    }
+    if (!u->obs || u->obs_cap < 1) {
+        (void)hipFree(u->obs);
+        u->obs = nullptr;
+        u->obs_cap = 0;
+        if ((rc = dalloc(&u->obs, 1))) return rc;
+        u->obs_cap = 1;
+    }
+    if ((rc = reduce_ke(u, false, u->obs))) return rc;
+    k_set_scalar<<<1, 1, 0, u->stream>>>(u->scal, 1.0);
+    gs_observables o;
+    HIPCHK(hipMemcpyAsync(&o, u->obs, sizeof(o), hipMemcpyDeviceToHost, u->stream));
+    HIPCHK(hipStreamSynchronize(u->stream));
+    if (kinetic) *kinetic = o.kinetic;
+    return GS_OK;
+}
+
+int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
+    int rc = check_state(u);
+    if (rc) return rc;
+    if (n_steps < 0) return fail(GS_E_INVALID, "n_steps < 0");
+    if (n_steps == 0 || u->n == 0) return GS_OK;
+    if (obs && u->obs_cap < n_steps) {
+        (void)hipFree(u->obs);
+        u->obs = nullptr;
+        u->obs_cap = 0;
+        if ((rc = dalloc(&u->obs, (size_t)n_steps))) return rc;
+        u->obs_cap = n_steps;
+    }
+    const gs_params& p = u->prm;
+    const bool thermo = p.enable_thermostat != 0;
+    // Timing: 4 events per tick (start, after rebuild, after pair, end).
+    const size_t nev = u->timing ? (size_t)(4 * n_steps) : 0;
+    while (u->ev.size() < nev) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        u->ev.push_back(e);
+    }
+    for (int64_t s = 0; s < n_steps; ++s) {
+        if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 0], u->stream));
+        if ((rc = rebuild(u, true))) return rc;
+        if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 1], u->stream));
+        if ((rc = launch_pair<false>(u))) return rc;
+        if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 2], u->stream));
+        const int c = u->cur;
+        k_kick_ke<true><<<nblocks(u->n), BLOCK, 0, u->stream>>>(
+            u->kp, u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->type[c], u->kpart, u->counter,
+            u->vpart, u->nvpart, u->scal, obs ? u->obs + s : nullptr, thermo ? 1 : 0, p.thermostat_temperature,
+            std::fmax(p.thermostat_tau, 1.0), p.thermal_constant);
+        if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
+    }
+    const int c = u->cur;
+    k_apply_lambda<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->n, u->v[c][0], u->v[c][1], u->v[c][2], u->scal);
+    k_set_scalar<<<1, 1, 0, u->stream>>>(u->scal, 1.0);
+    HIPCHK(hipGetLastError());
+    if (obs)
+        HIPCHK(hipMemcpyAsync(obs, u->obs, sizeof(gs_observables) * (size_t)n_steps, hipMemcpyDeviceToHost, u->stream));
+    HIPCHK(hipStreamSynchronize(u->stream));
+    u->extras_valid = false;
+    if (u->timing) {
+        for (int64_t s = 0; s < n_steps; ++s) {
+            float a = 0.f, b = 0.f, c2 = 0.f;
+            HIPCHK(hipEventElapsedTime(&a, u->ev[4 * s + 0], u->ev[4 * s + 1]));
+            HIPCHK(hipEventElapsedTime(&b, u->ev[4 * s + 1], u->ev[4 * s + 2]));
+            HIPCHK(hipEventElapsedTime(&c2, u->ev[4 * s + 2], u->ev[4 * s + 3]));
+            u->tm.rebuild_ms += a;
+            u->tm.pair_ms += b;
+            u->tm.kick_ms += c2;
+            u->tm.step_ms += (double)a + b + c2;
+        }
+        u->tm.steps += n_steps;
+    }
+    return GS_OK;
+}
+
+int gs_timing_enable(gs_universe* u, int on) {
+    if (!u) return fail(GS_E_INVALID, "universe handle is NULL");
+    u->timing = on != 0;
+    u->tm = gs_timing{0, 0.0, 0.0, 0.0, 0.0, 0};
+    return GS_OK;
+}
+
+int gs_timing_read(gs_universe* u, gs_timing* out) {
+    if (!u || !out) return fail(GS_E_INVALID, "NULL argument");
+    *out = u->tm;
+    return GS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// SpatialGrid::new on the host (src/physics/grid.rs:67-86).  Axes with many
+// cells are coarsened so the cell count stays O(n): cells stay >= min_cell,
+// so the enumerated candidate set still contains every pair within the
+// cutoff, and an axis with 1 or 2 cells (where the reference's enumeration
+// differs) is never touched.
+AuxGrid aux_grid(const KParams& P, double min_cell, int64_t n) {
+    AuxGrid G;
+    for (int a = 0; a < 3; ++a) {
+        double nc = std::floor(P.box[a] / min_cell);
+        if (!(nc >= 1.0)) nc = 1.0;
+        if (nc > 1024.0) nc = 1024.0;
+        G.dims[a] = (uint32_t)nc;
+    }
+    const uint64_t cap = (uint64_t)std::max<int64_t>(n, 1) * 4 + 4096;
+    while ((uint64_t)G.dims[0] * G.dims[1] * G.dims[2] > cap) {
+        int a = 0;
+        for (int b = 1; b < 3; ++b)
+            if (G.dims[b] > G.dims[a]) a = b;
+        if (G.dims[a] <= 3) break;
+        G.dims[a] = std::max<uint32_t>(3, G.dims[a] / 2);
+    }
+    for (int a = 0; a < 3; ++a) G.cell[a] = P.box[a] / (double)G.dims[a];
+    G.ncell = G.dims[0] * G.dims[1] * G.dims[2];
+    return G;
+}
+
+int lens_alloc(gs_universe* u, uint32_t ncell) {
+    int rc;
+    const size_t n = (size_t)std::max(1, u->n);
+    if (!u->l_pts) {
+        if ((rc = dalloc(&u->l_pts, n)) || (rc = dalloc(&u->l_type, n)) || (rc = dalloc(&u->l_key, n)) ||
+            (rc = dalloc(&u->l_ticket, n)) || (rc = dalloc(&u->l_perm, n)) || (rc = dalloc(&u->l_parent, n)) ||
+            (rc = dalloc(&u->l_size, n)) || (rc = dalloc(&u->l_u64, 8)) || (rc = dalloc(&u->l_sigma, GS_NSPECIES)))
+            return rc;
+    }
+    if (u->l_ncell_cap < ncell) {
+        (void)hipFree(u->l_count);
+        (void)hipFree(u->l_start);
+        u->l_count = nullptr;
+        u->l_start = nullptr;
+        u->l_ncell_cap = 0;
+        if ((rc = dalloc(&u->l_count, ncell)) || (rc = dalloc(&u->l_start, (size_t)ncell + 1))) return rc;
+        u->l_ncell_cap = ncell;
+    }
+    HIPCHK(hipMemsetAsync(u->l_count, 0, sizeof(uint32_t) * ncell, u->stream));
+    return GS_OK;
+}
+
+// Counting sort of m points into the lens grid: l_start, l_perm.
+int lens_sort(gs_universe* u, const AuxGrid& G, const double4* pts, int m) {
+    k_aux_key<<<nblocks(m), BLOCK, 0, u->stream>>>(m, G, u->kp, pts, u->l_key, u->l_ticket, u->l_count);
+    k_scan<<<1, 1024, 0, u->stream>>>(u->l_count, u->l_start, G.ncell);
+    k_scatter<<<nblocks(m), BLOCK, 0, u->stream>>>(m, u->l_key, u->l_ticket, u->l_start, u->l_perm);
+    HIPCHK(hipGetLastError());
+    return GS_OK;
+}
+
+int radial_impl(gs_universe* u, bool between, int ta, int tb, double r_max_in, int nbins_in, uint64_t* counts,
+                double* g, double* r_max_out, double* dr_out) {
+    int rc = check_state(u);
+    if (rc) return rc;
+    if (between && (ta < 0 || ta >= GS_NSPECIES || tb < 0 || tb >= GS_NSPECIES))
+        return fail(GS_E_INVALID, "species index out of range");
+    // radial_distribution(_between), src/analysis/mod.rs:93-97 and :152-156.
+    const int nbins = std::min(std::max(nbins_in, 1), 1 << 16);
+    const KParams& P = u->kp;
+    const double half_min = std::min(std::min(P.box[0], P.box[1]), P.box[2]) * 0.5;
+    const double r_max = std::min(std::fmax(r_max_in, 1e-9), half_min);
+    const double dr = r_max / (double)nbins;
+    const double vol = P.box[0] * P.box[1] * P.box[2];
+    // g_subsample (src/analysis/mod.rs:32-46).
+    const uint32_t N = (uint32_t)u->n;
+    uint32_t step = 1, m = N;
+    if (N > 8192u) {
+        step = N / 8192u + 1u;
+        m = N / step;
+    }
+    std::vector<unsigned long long> hc((size_t)nbins, 0ull);
+    if (m > 0) {
+        const AuxGrid G = aux_grid(P, r_max, m);
+        if ((rc = lens_alloc(u, G.ncell))) return rc;
+        if (u->l_bins_cap < nbins) {
+            (void)hipFree(u->l_bins);
+            u->l_bins = nullptr;
+            u->l_bins_cap = 0;
+            if ((rc = dalloc(&u->l_bins, (size_t)nbins))) return rc;
+            u->l_bins_cap = nbins;
+        }
+        HIPCHK(hipMemsetAsync(u->l_bins, 0, sizeof(unsigned long long) * nbins, u->stream));
+        const int c = u->cur;
+        k_gr_gather<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->n, u->id[c], u->pos4[c], u->type[c], step, m,
+                                                             u->l_pts, u->l_type);
+        if ((rc = lens_sort(u, G, u->l_pts, (int)m))) return rc;
+        const int refm = u->prm.grid_semantics == GS_GRID_REFERENCE;
+        const size_t shm = nbins <= 4096 ? sizeof(uint32_t) * (size_t)nbins : 0;
+        if (between)
+            k_gr_pairs<true><<<nblocks(m), BLOCK, shm, u->stream>>>((int)m, G, P, u->l_pts, u->l_type, u->l_start,
+                                                                    u->l_perm, r_max, dr, nbins, ta, tb, refm,
+                                                                    u->l_bins);
+        else
+            k_gr_pairs<false><<<nblocks(m), BLOCK, shm, u->stream>>>((int)m, G, P, u->l_pts, u->l_type, u->l_start,
+                                                                     u->l_perm, r_max, dr, nbins, ta, tb, refm,
+                                                                     u->l_bins);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(hc.data(), u->l_bins, sizeof(unsigned long long) * nbins, hipMemcpyDeviceToHost,
+                              u->stream));
+        HIPCHK(hipStreamSynchronize(u->stream));
+    }
+    // Normalisation, src/analysis/mod.rs:119-132 / :184-200, on the host with
+    // the reference's operation order.
+    double pair_norm;
+    if (!between) {
+        pair_norm = m < 2 ? 0.0 : (double)((uint64_t)m * (uint64_t)(m - 1)) * 0.5;
+    } else {
+        // species counts of the subsample
+        std::vector<uint8_t> st((size_t)m);
+        if (m > 0) {
+            HIPCHK(hipMemcpyAsync(st.data(), u->l_type, m, hipMemcpyDeviceToHost, u->stream));
+            HIPCHK(hipStreamSynchronize(u->stream));
+        }
+        uint64_t na = 0, nb = 0;
+        for (uint8_t t : st) {
+            na += (t == ta);
+            nb += (t == tb);
+        }
+        pair_norm = ta == tb ? (double)na * (double)(na > 0 ? na - 1 : 0) * 0.5 : (double)na * (double)nb;
+    }
+    for (int i = 0; i < nbins; ++i) {
+        if (counts) counts[i] = (uint64_t)hc[i];
+        if (g) {
+            if (pair_norm <= 0.0) {
+                g[i] = 0.0;
+                continue;
+            }
+            const double r_lo = (double)i * dr;
+            const double r_hi = r_lo + dr;
+            const double shell = 4.0 / 3.0 * M_PI * (r_hi * r_hi * r_hi - r_lo * r_lo * r_lo);
+            const double expected = pair_norm * shell / vol;
+            g[i] = (double)hc[i] / std::fmax(expected, 2.2250738585072014e-308);
+        }
+    }
+    if (r_max_out) *r_max_out = r_max;
+    if (dr_out) *dr_out = dr;
+    return GS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gs_radial_distribution(gs_universe* u, double r_max, int32_t nbins, uint64_t* counts, double* g,
+                           double* r_max_out, double* dr_out) {
+    return radial_impl(u, false, -1, -1, r_max, nbins, counts, g, r_max_out, dr_out);
+}
+
+int gs_radial_distribution_between(gs_universe* u, int32_t species_a, int32_t species_b, double r_max,
+                                   int32_t nbins, uint64_t* counts, double* g, double* r_max_out, double* dr_out) {
+    return radial_impl(u, true, species_a, species_b, r_max, nbins, counts, g, r_max_out, dr_out);
+}
+
+int gs_clusters(gs_universe* u, double bond_factor, uint32_t* labels, gs_cluster_stats* stats) {
+    int rc = check_state(u);
+    if (rc) return rc;
+    gs_cluster_stats st = {0, 0, 0, 0.0, 0};
+    const int n = u->n;
+    if (n == 0) {
+        if (stats) *stats = st;
+        return GS_OK;
+    }
+    // clusters, src/analysis/mod.rs:235-243: sigma of the present species.
+    double max_sigma = 0.0;
+    for (int s = 0; s < GS_NSPECIES; ++s)
+        if (u->present[s]) max_sigma = std::fmax(max_sigma, u->prm.sigma[s]);
+    const double cutoff = std::fmax(std::fmax(bond_factor, 0.1) * max_sigma, 1e-9);
+    const KParams& P = u->kp;
+    const AuxGrid G = aux_grid(P, cutoff, n);
+    if ((rc = lens_alloc(u, G.ncell))) return rc;
+    const int c = u->cur;
+    HIPCHK(hipMemcpyAsync(u->l_sigma, u->prm.sigma, sizeof(double) * GS_NSPECIES, hipMemcpyHostToDevice, u->stream));
+    HIPCHK(hipMemsetAsync(u->l_u64, 0, sizeof(unsigned long long) * 8, u->stream));
+    HIPCHK(hipMemsetAsync(u->l_size, 0, sizeof(uint32_t) * (size_t)n, u->stream));
+    k_iota<<<nblocks(n), BLOCK, 0, u->stream>>>(n, u->l_parent);
+    if ((rc = lens_sort(u, G, u->pos4[c], n))) return rc;
+    const int refm = u->prm.grid_semantics == GS_GRID_REFERENCE;
+    k_fof_edges<<<nblocks(n), BLOCK, 0, u->stream>>>(n, G, P, u->pos4[c], u->type[c], u->id[c], u->l_start,
+                                                     u->l_perm, u->l_sigma, bond_factor, cutoff, refm, u->l_parent,
+                                                     u->l_u64 + 4);
+    k_fof_finalize<<<nblocks(n), BLOCK, 0, u->stream>>>(n, u->l_parent, u->l_size);
+    k_fof_stats<<<nblocks(n), BLOCK, 0, u->stream>>>(n, u->l_size, u->l_u64);
+    if (labels) k_fof_labels<<<nblocks(n), BLOCK, 0, u->stream>>>(n, u->l_parent, u->l_key);
+    HIPCHK(hipGetLastError());
+    unsigned long long h[8];
+    HIPCHK(hipMemcpyAsync(h, u->l_u64, sizeof(h), hipMemcpyDeviceToHost, u->stream));
+    if (labels) HIPCHK(hipMemcpyAsync(labels, u->l_key, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost, u->stream));
+    HIPCHK(hipStreamSynchronize(u->stream));
+    st.monomers = (int64_t)h[0];
+    st.aggregates = (int64_t)h[1];
+    st.largest = (int64_t)h[2];
+    st.mean_size = (double)n / (double)std::max<unsigned long long>(h[3], 1ull);
+    st.bound_pairs = (int64_t)h[4];
+    if (stats) *stats = st;
+    return GS_OK;
+}
+
+}  // extern "C"

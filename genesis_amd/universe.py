@@ -1,0 +1,163 @@
+"""Host-side handle over the resident GPU state (one ``gs_universe``).
+
+``GpuUniverse`` owns one C-ABI handle; its methods map one-to-one onto the
+entry points of include/genesis_gpu.h, with numpy arrays in entity (slot)
+order on the host side.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _lib
+from .inputs import State
+from .params import Physics
+
+
+def _dp(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+OBS_FIELDS = ("kinetic", "potential", "bond_energy", "temperature", "lambda", "energy_total")
+
+
+class GpuUniverse:
+    """Resident universe on the current HIP device."""
+
+    def __init__(self, physics: Physics, state: State):
+        self.lib = _lib.load()
+        self.physics = physics
+        self.n = state.n
+        self._params = physics.to_c()
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.gs_create(ctypes.byref(self._params), self.n, ctypes.byref(h)))
+        self.h = h
+        self.upload(state)
+
+    # -- lifecycle ---------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.gs_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state transfer ------------------------------------------------------------
+    def upload(self, s: State) -> None:
+        ax, ay, az = (None, None, None) if s.ax is None else (_f64(s.ax), _f64(s.ay), _f64(s.az))
+        arrs = [_f64(s.x), _f64(s.y), _f64(s.z), _f64(s.vx), _f64(s.vy), _f64(s.vz)]
+        sp = np.ascontiguousarray(s.species, dtype=np.uint8)
+        q = _f64(s.charge)
+        self._keep = (arrs, ax, ay, az, sp, q)
+        _lib.check(self.lib.gs_upload_state(self.h, *[_dp(a) for a in arrs], _dp(ax), _dp(ay), _dp(az),
+                                            sp.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _dp(q)))
+        self.species = sp.copy()
+        self.charge = q.copy()
+
+    def download(self) -> State:
+        out = [np.empty(self.n) for _ in range(9)]
+        _lib.check(self.lib.gs_download_state(self.h, *[_dp(a) for a in out]))
+        return State(out[0], out[1], out[2], out[3], out[4], out[5], self.species.copy(), self.charge.copy(),
+                     out[6], out[7], out[8])
+
+    # -- systems -------------------------------------------------------------------
+    def forces(self) -> Tuple[float, float]:
+        v, vb = ctypes.c_double(), ctypes.c_double()
+        _lib.check(self.lib.gs_forces(self.h, ctypes.byref(v), ctypes.byref(vb)))
+        return v.value, vb.value
+
+    def force_extras(self) -> Tuple[np.ndarray, np.ndarray]:
+        pe, sa = np.empty(self.n), np.empty(self.n)
+        _lib.check(self.lib.gs_download_force_extras(self.h, _dp(pe), _dp(sa)))
+        return pe, sa
+
+    def cells(self) -> Tuple[np.ndarray, tuple]:
+        c = np.empty(self.n, dtype=np.uint32)
+        d = np.zeros(3, dtype=np.uint32)
+        _lib.check(self.lib.gs_download_cells(self.h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                              d.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        return c, tuple(int(v) for v in d)
+
+    def half_kick(self) -> None:
+        _lib.check(self.lib.gs_half_kick(self.h))
+
+    def drift(self) -> None:
+        _lib.check(self.lib.gs_drift(self.h))
+
+    def wrap(self) -> None:
+        _lib.check(self.lib.gs_wrap(self.h))
+
+    def thermostat(self) -> float:
+        lam = ctypes.c_double()
+        _lib.check(self.lib.gs_thermostat(self.h, ctypes.byref(lam)))
+        return lam.value
+
+    def kinetic(self) -> float:
+        k = ctypes.c_double()
+        _lib.check(self.lib.gs_kinetic(self.h, ctypes.byref(k)))
+        return k.value
+
+    def step(self, n_steps: int, observe: bool = True) -> Optional[np.ndarray]:
+        """Universe::tick x n_steps on the device; returns (n_steps, 6) observables."""
+        if not observe:
+            _lib.check(self.lib.gs_step(self.h, n_steps, None))
+            return None
+        buf = (_lib.Observables * max(1, n_steps))()
+        _lib.check(self.lib.gs_step(self.h, n_steps, buf))
+        return np.ctypeslib.as_array(ctypes.cast(buf, ctypes.POINTER(ctypes.c_double)),
+                                     shape=(max(1, n_steps), 6))[:n_steps].copy()
+
+    def timing(self, enable: Optional[bool] = None) -> dict:
+        """Per-phase event timing of step(); enable=True resets the counters."""
+        if enable is not None:
+            _lib.check(self.lib.gs_timing_enable(self.h, 1 if enable else 0))
+        t = _lib.Timing()
+        _lib.check(self.lib.gs_timing_read(self.h, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in _lib.Timing._fields_}
+
+    # -- lenses --------------------------------------------------------------------
+    def radial_distribution(self, r_max: float, nbins: int):
+        counts = np.zeros(max(1, min(nbins, 1 << 16)), dtype=np.uint64)
+        g = np.zeros(counts.shape[0])
+        rm, dr = ctypes.c_double(), ctypes.c_double()
+        _lib.check(self.lib.gs_radial_distribution(self.h, r_max, nbins,
+                                                   counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), _dp(g),
+                                                   ctypes.byref(rm), ctypes.byref(dr)))
+        return counts, g, rm.value, dr.value
+
+    def radial_distribution_between(self, ta: int, tb: int, r_max: float, nbins: int):
+        counts = np.zeros(max(1, min(nbins, 1 << 16)), dtype=np.uint64)
+        g = np.zeros(counts.shape[0])
+        rm, dr = ctypes.c_double(), ctypes.c_double()
+        _lib.check(self.lib.gs_radial_distribution_between(
+            self.h, ta, tb, r_max, nbins, counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), _dp(g),
+            ctypes.byref(rm), ctypes.byref(dr)))
+        return counts, g, rm.value, dr.value
+
+    def clusters(self, bond_factor: float = 1.5):
+        labels = np.zeros(self.n, dtype=np.uint32)
+        st = _lib.ClusterStats()
+        _lib.check(self.lib.gs_clusters(self.h, bond_factor, labels.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                        ctypes.byref(st)))
+        stats = {"monomers": st.monomers, "aggregates": st.aggregates, "largest": st.largest,
+                 "mean_size": st.mean_size, "bound_pairs": st.bound_pairs}
+        return labels, stats

@@ -1,0 +1,197 @@
+/*
+ * genesis_gpu.h — C ABI of the MI355X force-and-integration path.
+ *
+ * Plain C linkage, POD structs, pointers and sizes only.  Every entry point
+ * returns an int status: 0 = ok, < 0 = error (see GS_E_*); gs_last_error()
+ * gives the message of the last failing call on the calling thread.
+ *
+ * Ownership: device buffers belong to the handle; host buffers passed in are
+ * caller-owned and copied.  One handle = one HIP stream.  Calls on one handle
+ * must be serialised by the caller (the reference scheduler runs systems
+ * sequentially, src/scheduler/mod.rs:83-89).  Results are deterministic run
+ * to run: no floating-point atomics are used anywhere.
+ *
+ * Units and semantics follow the reference crate (ValentinVaninetti/genesis):
+ * positions in simulation length units, epsilon in kelvin converted with
+ * thermal_constant, the Berendsen tau in ticks.  Species are element indices
+ * 0..9 in the order H, He, C, N, O, Na, Si, P, S, Fe
+ * (src/physics/forces.rs:176-189).
+ */
+#ifndef GENESIS_GPU_H
+#define GENESIS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS_NSPECIES 10
+
+/* status codes */
+#define GS_OK 0
+#define GS_E_INVALID (-1)     /* bad argument / validation failure           */
+#define GS_E_HIP (-2)         /* HIP runtime error                           */
+#define GS_E_STATE (-3)       /* call order error (e.g. no state uploaded)   */
+#define GS_E_OVERFLOW (-4)    /* a device capacity was exceeded              */
+#define GS_E_NONFINITE (-5)   /* a kernel saw a non-finite value             */
+
+/* grid_semantics */
+#define GS_GRID_REFERENCE 0   /* pair multiplicity of the reference grid: an
+                                 axis with exactly 2 cells reports a pair once
+                                 per wrapping offset (src/physics/grid.rs:134-151) */
+#define GS_GRID_MINIMAGE 1    /* every unordered pair once (= brute force,
+                                 src/physics/grid.rs:313-326)                 */
+
+/*
+ * Physics and schedule parameters.  Replaces the fields of Config that the
+ * force pass, the integrator and the thermostat read
+ * (src/config/mod.rs:17-260; ForceSystem::new src/systems/forces.rs:100-142;
+ * ThermostatSystem::new src/systems/thermostat.rs:29-35).
+ */
+typedef struct gs_params {
+    double box[3];                 /* universe.size (periodic torus extent)  */
+    double dt;                     /* universe.dt                            */
+    double sigma[GS_NSPECIES];     /* ElementTable sigma (after overrides)   */
+    double epsilon_k[GS_NSPECIES]; /* ElementTable epsilon in kelvin         */
+    double charge[GS_NSPECIES];    /* ElementTable charge (seeding default)  */
+    double mass[GS_NSPECIES];      /* per-species Mass component value       */
+    double thermal_constant;       /* physics.thermal_constant (k)           */
+    double lj_cutoff_factor;       /* LJ_CUTOFF_FACTOR = 2.5                 */
+    double coulomb_constant;       /* physics.coulomb_constant (k_e)         */
+    double coulomb_cutoff;         /* physics.coulomb_cutoff (x max sigma)   */
+    double gravity_constant;       /* physics.gravity_constant               */
+    double gravity_cutoff;         /* physics.gravity_cutoff (x max sigma)   */
+    double thermostat_temperature; /* physics.thermostat_temperature (K)     */
+    double thermostat_tau;         /* physics.thermostat_tau (ticks)         */
+    double bond_k_bind;            /* stats.bond_k_bind                      */
+    double bond_min_periods;       /* stats.bond_min_periods                 */
+    int32_t enable_boundaries;
+    int32_t enable_electrostatics;
+    int32_t enable_gravity;
+    int32_t enable_thermostat;
+    int32_t enable_bond_interaction;
+    int32_t enable_bond_observation;
+    int32_t grid_semantics;        /* GS_GRID_REFERENCE or GS_GRID_MINIMAGE  */
+    int32_t reserved0;
+} gs_params;
+
+/* Per-tick observables (StatsSystem energy part, src/systems/aggregate.rs:86-121). */
+typedef struct gs_observables {
+    double kinetic;        /* K = sum 1/2 m |v|^2 after the thermostat           */
+    double potential;      /* V (LJ + Coulomb + gravity) of this tick's forces   */
+    double bond_energy;    /* V_bond of this tick's forces                       */
+    double temperature;    /* (2/3)(K/N)/k                                        */
+    double lambda;         /* Berendsen scale applied this tick (1 if off)       */
+    double energy_total;   /* K + V + V_bond                                      */
+} gs_observables;
+
+/* FoF summary (ClusterStats, src/analysis/mod.rs:204-216). */
+typedef struct gs_cluster_stats {
+    int64_t monomers;
+    int64_t aggregates;
+    int64_t largest;
+    double mean_size;
+    int64_t bound_pairs;
+} gs_cluster_stats;
+
+/* Kernel timing of gs_step, measured with HIP events recorded on the
+ * handle's stream around each phase of every tick (no reference
+ * counterpart: the reference only times whole ticks, src/universe/mod.rs:143-158). */
+typedef struct gs_timing {
+    int64_t steps;          /* ticks timed since gs_timing_enable            */
+    double step_ms;         /* sum over ticks of the whole tick              */
+    double rebuild_ms;      /* kick/drift/wrap + cell-list rebuild           */
+    double pair_ms;         /* pair kernel (the dominant kernel)             */
+    double kick_ms;         /* second kick + kinetic reduction + Berendsen   */
+    int64_t pairs_in_range; /* ordered pairs within the cutoff, last tick    */
+} gs_timing;
+
+typedef struct gs_universe gs_universe;
+
+/* ---- lifecycle ------------------------------------------------------------ */
+
+/* ForceSystem::new + ThermostatSystem::new (src/systems/forces.rs:100-142,
+ * src/systems/thermostat.rs:29-35).  Allocates resident device state for
+ * n_atoms atoms on the current HIP device. */
+int gs_create(const gs_params* params, int64_t n_atoms, gs_universe** out);
+int gs_destroy(gs_universe* u);
+const char* gs_last_error(void);
+/* Which HIP device the library runs on and its name (for logs). */
+int gs_device_info(int* device, char* name, int name_len);
+
+/* ---- state transfer (ECS columns, src/components/) ---------------------
+ * SoA arrays of length n_atoms in entity (slot) order.  species[i] is the
+ * element index 0..9; charge is the per-entity Charge component.  Any of
+ * the acceleration pointers may be NULL on upload (then a = 0, which is the
+ * state spawn_atom leaves, src/universe/mod.rs:272). */
+int gs_upload_state(gs_universe* u,
+                    const double* x, const double* y, const double* z,
+                    const double* vx, const double* vy, const double* vz,
+                    const double* ax, const double* ay, const double* az,
+                    const uint8_t* species, const double* charge);
+int gs_download_state(gs_universe* u,
+                      double* x, double* y, double* z,
+                      double* vx, double* vy, double* vz,
+                      double* ax, double* ay, double* az);
+
+/* ---- systems on the hot path (one call = one System::run) ------------------ */
+
+/* ForceSystem::run (src/systems/forces.rs:163-353): rebuilds the cell list,
+ * evaluates every pair within the cutoff, writes the Acceleration column and
+ * returns PotentialEnergy / BondEnergy.  Any output pointer may be NULL. */
+int gs_forces(gs_universe* u, double* potential, double* bond_energy);
+/* Per-atom extras of the last gs_forces call, in slot order: per-atom
+ * potential energy (1/2 sum_j V_ij) and S_i = sum_j |f_ij|.  NULL skips. */
+int gs_download_force_extras(gs_universe* u, double* pe_atom, double* s_abs);
+/* Cell membership of the force grid for the current positions, in slot order:
+ * cell = ix + nx*(iy + ny*iz) with (ix, iy, iz) from SpatialGrid::cell_of and
+ * encode (src/physics/grid.rs:228-246) on a grid of edge >= the force cutoff
+ * (ForceSystem::new, src/systems/forces.rs:121-124).  dims (u32[3]) may be NULL. */
+int gs_download_cells(gs_universe* u, uint32_t* cell, uint32_t* dims);
+/* VelocityHalfKick::run (src/systems/integrate.rs:38-44). */
+int gs_half_kick(gs_universe* u);
+/* PositionDrift::run (src/systems/integrate.rs:61-67). */
+int gs_drift(gs_universe* u);
+/* BoundarySystem::run (src/systems/boundary.rs:22-35). */
+int gs_wrap(gs_universe* u);
+/* ThermostatSystem::run (src/systems/thermostat.rs:47-71); lambda may be NULL. */
+int gs_thermostat(gs_universe* u, double* lambda);
+/* Kinetic energy sum (src/systems/aggregate.rs:86-93). */
+int gs_kinetic(gs_universe* u, double* kinetic);
+
+/* Universe::tick x n_steps (src/universe/mod.rs:141-159) with the schedule of
+ * build_schedule (src/universe/mod.rs:355-392): kick, drift, [wrap], forces,
+ * kick, [thermostat].  Fused and resident: nothing crosses to the host but the
+ * observables.  obs may be NULL, else it receives n_steps records. */
+int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs);
+
+/* Enable (and reset) or disable per-phase event timing of gs_step. */
+int gs_timing_enable(gs_universe* u, int on);
+int gs_timing_read(gs_universe* u, gs_timing* out);
+
+/* ---- observation lenses ---------------------------------------------------- */
+
+/* radial_distribution (src/analysis/mod.rs:87-133), incl. the deterministic
+ * subsample above 8192 atoms (:28-46).  counts: u64[nbins] pair counts,
+ * g: f64[nbins]; r_max_out/dr_out receive the clipped r_max and bin width.
+ * Any output may be NULL. */
+int gs_radial_distribution(gs_universe* u, double r_max, int32_t nbins,
+                           uint64_t* counts, double* g,
+                           double* r_max_out, double* dr_out);
+/* radial_distribution_between (src/analysis/mod.rs:143-201). */
+int gs_radial_distribution_between(gs_universe* u, int32_t species_a, int32_t species_b,
+                                   double r_max, int32_t nbins,
+                                   uint64_t* counts, double* g,
+                                   double* r_max_out, double* dr_out);
+/* clusters (src/analysis/mod.rs:221-284): friends-of-friends with
+ * d < bond_factor * (sigma_a + sigma_b)/2.  labels (u32[n], may be NULL) get
+ * the canonical component label = smallest slot index in the component. */
+int gs_clusters(gs_universe* u, double bond_factor, uint32_t* labels,
+                gs_cluster_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GENESIS_GPU_H */

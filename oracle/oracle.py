@@ -1,0 +1,274 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so built from oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_D = ctypes.POINTER(ctypes.c_double)
+_U8 = ctypes.POINTER(ctypes.c_uint8)
+_U32 = ctypes.POINTER(ctypes.c_uint32)
+_U64 = ctypes.POINTER(ctypes.c_uint64)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-B", "-C", HERE], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        from genesis_amd.params import GsParams
+        P = ctypes.POINTER(GsParams)
+        sig = {
+            "or_lj_raw": (None, [ctypes.c_double] * 3 + [_D, _D]),
+            "or_smooth_cutoff": (None, [ctypes.c_double] * 3 + [_D, _D]),
+            "or_switched": (None, [ctypes.c_double] * 4 + [_D, _D]),
+            "or_coulomb_raw": (None, [ctypes.c_double] * 4 + [_D, _D]),
+            "or_gravity_raw": (None, [ctypes.c_double] * 4 + [_D, _D]),
+            "or_bond_harmonic_raw": (None, [ctypes.c_double] * 3 + [_D, _D]),
+            "or_well_curvature": (ctypes.c_double, [ctypes.c_double] * 2),
+            "or_vib_period": (ctypes.c_double, [ctypes.c_double] * 3),
+            "or_equilibrium_distance": (ctypes.c_double, [ctypes.c_double]),
+            "or_lj_table": (None, [P, _D, _D, _D]),
+            "or_force_switched_scalar": (None, [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D]),
+            "or_min_image": (None, [_D, _D, _D]),
+            "or_grid_cells": (None, [ctypes.c_int64, _D, _D, _D, _D, ctypes.c_double, _U32, _U32]),
+            "or_pairs": (ctypes.c_int64, [ctypes.c_int64, _D, _D, _D, _D, ctypes.c_double, ctypes.c_int,
+                                          ctypes.c_int64, _U32, _U32]),
+            "or_forces": (ctypes.c_int, [P, ctypes.c_int64, _D, _D, _D, _U8, _D, _I32, _I32, ctypes.c_int,
+                                         _D, _D, _D, _D, _D, _D, _D, _I64]),
+            "or_half_kick": (None, [ctypes.c_int64, _D, _D, _D, _D, _D, _D, ctypes.c_double]),
+            "or_drift": (None, [ctypes.c_int64, _D, _D, _D, _D, _D, _D, ctypes.c_double]),
+            "or_wrap": (None, [ctypes.c_int64, _D, _D, _D, _D]),
+            "or_kinetic": (ctypes.c_double, [ctypes.c_int64, _D, _D, _D, _U8, _D]),
+            "or_thermostat": (ctypes.c_double, [P, ctypes.c_int64, _D, _D, _D, _U8]),
+            "or_run": (ctypes.c_int, [P, ctypes.c_int64] + [_D] * 9 + [_U8, _D, ctypes.c_int64, ctypes.c_int, _D]),
+            "or_radial_distribution": (ctypes.c_int, [ctypes.c_int64, _D, _D, _D, _U8, _D, ctypes.c_double,
+                                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, _U64, _D, _D, _D]),
+            "or_clusters": (ctypes.c_int, [ctypes.c_int64, _D, _D, _D, _U8, _D, _D, ctypes.c_double, _U32, _D]),
+            "or_bond_components": (ctypes.c_int, [ctypes.c_int64, _U32, _U8, ctypes.c_int64, _U32, _U32, _I64, _U32,
+                                                  ctypes.c_char_p, ctypes.c_int64]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a, t=ctypes.c_double):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def out2(fn, *args):
+    a, b = ctypes.c_double(), ctypes.c_double()
+    fn(*args, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def lj_raw(sigma, eps, r):
+    return out2(lib().or_lj_raw, sigma, eps, r)
+
+
+def smooth_cutoff(r, r_on, rc):
+    return out2(lib().or_smooth_cutoff, r, r_on, rc)
+
+
+def switched(f, v, sw, dsw):
+    return out2(lib().or_switched, f, v, sw, dsw)
+
+
+def coulomb_raw(k, qa, qb, r):
+    return out2(lib().or_coulomb_raw, k, qa, qb, r)
+
+
+def gravity_raw(g, ma, mb, r):
+    return out2(lib().or_gravity_raw, g, ma, mb, r)
+
+
+def bond_harmonic_raw(k, r_eq, r):
+    return out2(lib().or_bond_harmonic_raw, k, r_eq, r)
+
+
+def well_curvature(eps, sigma):
+    return lib().or_well_curvature(eps, sigma)
+
+
+def vib_period(eps, sigma, mu):
+    return lib().or_vib_period(eps, sigma, mu)
+
+
+def equilibrium_distance(sigma):
+    return lib().or_equilibrium_distance(sigma)
+
+
+def lj_table(physics):
+    p = physics.to_c()
+    sig, eps, cut = np.zeros(100), np.zeros(100), np.zeros(6)
+    lib().or_lj_table(ctypes.byref(p), _p(sig), _p(eps), _p(cut))
+    return sig.reshape(10, 10), eps.reshape(10, 10), dict(zip(
+        ("lj_rc", "lj_ron", "rc", "coulomb_rc", "coulomb_ron", "gravity_rc"), cut.tolist()))
+
+
+def force_switched_scalar(physics, ta, tb, r):
+    p = physics.to_c()
+    return out2(lib().or_force_switched_scalar, ctypes.byref(p), ta, tb, r)
+
+
+def min_image(d, size):
+    d, s, o = _d(d), _d(size), np.zeros(3)
+    lib().or_min_image(_p(d), _p(s), _p(o))
+    return o
+
+
+def grid_cells(x, y, z, size, min_cell):
+    n = len(x)
+    x, y, z, s = _d(x), _d(y), _d(z), _d(size)
+    dims, cell = np.zeros(3, dtype=np.uint32), np.zeros(n, dtype=np.uint32)
+    lib().or_grid_cells(n, _p(x), _p(y), _p(z), _p(s), min_cell, _p(dims, ctypes.c_uint32), _p(cell, ctypes.c_uint32))
+    return cell, tuple(int(v) for v in dims)
+
+
+def pairs(x, y, z, size, cutoff, brute=False):
+    n = len(x)
+    x, y, z, s = _d(x), _d(y), _d(z), _d(size)
+    total = lib().or_pairs(n, _p(x), _p(y), _p(z), _p(s), cutoff, 1 if brute else 0, 0, None, None)
+    a, b = np.zeros(max(total, 1), dtype=np.uint32), np.zeros(max(total, 1), dtype=np.uint32)
+    lib().or_pairs(n, _p(x), _p(y), _p(z), _p(s), cutoff, 1 if brute else 0, total, _p(a, ctypes.c_uint32),
+                   _p(b, ctypes.c_uint32))
+    return a[:total], b[:total]
+
+
+def forces(physics, state, nthreads=1, bonds_csr=None, extras=True):
+    """ForceSystem::run on ``state``.  Returns dict with ax, ay, az, V, Vb,
+    pe, sabs, npairs (slot order)."""
+    n = state.n
+    p = physics.to_c()
+    x, y, z, q = _d(state.x), _d(state.y), _d(state.z), _d(state.charge)
+    sp = np.ascontiguousarray(state.species, dtype=np.uint8)
+    ax, ay, az = np.zeros(n), np.zeros(n), np.zeros(n)
+    pe = np.zeros(n) if extras else None
+    sa = np.zeros(n) if extras else None
+    V, Vb, npairs = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+    bp = bi = None
+    if bonds_csr is not None:
+        bp = np.ascontiguousarray(bonds_csr[0], dtype=np.int32)
+        bi = np.ascontiguousarray(bonds_csr[1], dtype=np.int32)
+    lib().or_forces(ctypes.byref(p), n, _p(x), _p(y), _p(z), _p(sp, ctypes.c_uint8), _p(q), _p(bp, ctypes.c_int32),
+                    _p(bi, ctypes.c_int32), nthreads, _p(ax), _p(ay), _p(az), ctypes.byref(V), ctypes.byref(Vb),
+                    _p(pe), _p(sa), ctypes.byref(npairs))
+    return {"ax": ax, "ay": ay, "az": az, "V": V.value, "Vb": Vb.value, "pe": pe, "sabs": sa,
+            "npairs": npairs.value}
+
+
+def run(physics, state, nsteps, nthreads=1, observe=True):
+    """Universe::tick x nsteps on a copy of ``state``; returns (state, obs)."""
+    s = state.copy()
+    n = s.n
+    if s.ax is None:
+        s.ax, s.ay, s.az = np.zeros(n), np.zeros(n), np.zeros(n)
+    arrs = [_d(a) for a in (s.x, s.y, s.z, s.vx, s.vy, s.vz, s.ax, s.ay, s.az)]
+    sp = np.ascontiguousarray(s.species, dtype=np.uint8)
+    q = _d(s.charge)
+    obs = np.zeros((max(nsteps, 1), 6)) if observe else None
+    p = physics.to_c()
+    lib().or_run(ctypes.byref(p), n, *[_p(a) for a in arrs], _p(sp, ctypes.c_uint8), _p(q), nsteps, nthreads,
+                 _p(obs))
+    s.x, s.y, s.z, s.vx, s.vy, s.vz, s.ax, s.ay, s.az = arrs
+    return s, (obs[:nsteps] if observe else None)
+
+
+def half_kick(state, dt):
+    lib().or_half_kick(state.n, _p(state.vx), _p(state.vy), _p(state.vz), _p(state.ax), _p(state.ay), _p(state.az),
+                       dt)
+
+
+def drift(state, dt):
+    lib().or_drift(state.n, _p(state.x), _p(state.y), _p(state.z), _p(state.vx), _p(state.vy), _p(state.vz), dt)
+
+
+def wrap(state, box):
+    b = _d(box)
+    lib().or_wrap(state.n, _p(state.x), _p(state.y), _p(state.z), _p(b))
+
+
+def kinetic(state, mass):
+    m = _d(mass)
+    sp = np.ascontiguousarray(state.species, dtype=np.uint8)
+    return lib().or_kinetic(state.n, _p(state.vx), _p(state.vy), _p(state.vz), _p(sp, ctypes.c_uint8), _p(m))
+
+
+def thermostat(physics, state):
+    p = physics.to_c()
+    sp = np.ascontiguousarray(state.species, dtype=np.uint8)
+    return lib().or_thermostat(ctypes.byref(p), state.n, _p(state.vx), _p(state.vy), _p(state.vz),
+                               _p(sp, ctypes.c_uint8))
+
+
+def radial_distribution(x, y, z, size, r_max, nbins, species=None, ta=-1, tb=-1):
+    n = len(x)
+    x, y, z, s = _d(x), _d(y), _d(z), _d(size)
+    sp = None if species is None else np.ascontiguousarray(species, dtype=np.uint8)
+    nb = max(1, min(nbins, 1 << 16))
+    counts, g = np.zeros(nb, dtype=np.uint64), np.zeros(nb)
+    rm, dr = ctypes.c_double(), ctypes.c_double()
+    lib().or_radial_distribution(n, _p(x), _p(y), _p(z), _p(sp, ctypes.c_uint8), _p(s), r_max, nbins, ta, tb,
+                                 _p(counts, ctypes.c_uint64), _p(g), ctypes.byref(rm), ctypes.byref(dr))
+    return counts, g, rm.value, dr.value
+
+
+def clusters(x, y, z, species, sigma_table, size, bond_factor=1.5):
+    n = len(x)
+    x, y, z, s, st = _d(x), _d(y), _d(z), _d(size), _d(sigma_table)
+    sp = np.ascontiguousarray(species, dtype=np.uint8)
+    labels, stats = np.zeros(max(n, 1), dtype=np.uint32), np.zeros(5)
+    lib().or_clusters(n, _p(x), _p(y), _p(z), _p(sp, ctypes.c_uint8), _p(st), _p(s), bond_factor,
+                      _p(labels, ctypes.c_uint32), _p(stats))
+    return labels[:n], {"monomers": int(stats[0]), "aggregates": int(stats[1]), "largest": int(stats[2]),
+                        "mean_size": float(stats[3]), "bound_pairs": int(stats[4])}
+
+
+def bond_components(entities, edges):
+    """entities: list of (id, species); edges: list of (a, b)."""
+    ne = len(entities)
+    ids = np.array([e[0] for e in entities], dtype=np.uint32) if ne else np.zeros(1, dtype=np.uint32)
+    sp = np.array([e[1] for e in entities], dtype=np.uint8) if ne else np.zeros(1, dtype=np.uint8)
+    ea = np.array([e[0] for e in edges], dtype=np.uint32) if edges else np.zeros(1, dtype=np.uint32)
+    eb = np.array([e[1] for e in edges], dtype=np.uint32) if edges else np.zeros(1, dtype=np.uint32)
+    sizes = np.zeros(max(ne, 1), dtype=np.int64)
+    members = np.zeros(max(ne, 1), dtype=np.uint32)
+    stride = 64
+    fbuf = ctypes.create_string_buffer(stride * max(ne, 1))
+    nc = lib().or_bond_components(ne, _p(ids, ctypes.c_uint32), _p(sp, ctypes.c_uint8), len(edges),
+                                  _p(ea, ctypes.c_uint32), _p(eb, ctypes.c_uint32), _p(sizes, ctypes.c_int64),
+                                  _p(members, ctypes.c_uint32), fbuf, stride)
+    out, off = [], 0
+    for k in range(nc):
+        sz = int(sizes[k])
+        f = fbuf.raw[k * stride:(k + 1) * stride].split(b"\0", 1)[0].decode()
+        out.append({"size": sz, "members": members[off:off + sz].tolist(), "formula": f})
+        off += sz
+    return out

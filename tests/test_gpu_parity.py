@@ -1,0 +1,279 @@
+"""HIP path vs the oracle on identical seeded inputs, through the C ABI.
+
+Tolerances (written here, as the bar the north star sets):
+  * per-atom acceleration: |a_gpu - a_cpu|_inf <= 1e-10 max(|a_cpu|_inf, S_i/m_i)
+    with S_i = sum_j |f_ij| from the oracle;
+  * total potential / kinetic energy and per-atom PE: 1e-10 relative;
+  * integrator steps, cell membership, g(r) bin counts, cluster labels:
+    bit-exact.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from genesis_amd.inputs import State, mixture, plain_lj, random_gas
+from genesis_amd.params import DEFAULT_SIGMA, GRID_MINIMAGE, Physics
+from oracle import oracle as O
+from tests.helpers import force_tolerance_ok, reference_like
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+def say(msg):
+    print(f"[gpu-parity] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def GU():
+    from genesis_amd.universe import GpuUniverse
+    return GpuUniverse
+
+
+def check_forces(GU, ph, st, label):
+    say(f"{label}: n={st.n} box={tuple(round(b, 3) for b in ph.box)} dims={ph.grid_dims()}")
+    ref = O.forces(ph, st)
+    with GU(ph, st) as u:
+        V, Vb = u.forces()
+        out = u.download()
+        pe, sa = u.force_extras()
+    m = np.asarray(ph.mass)[st.species]
+    ok, worst = force_tolerance_ok(out.ax, out.ay, out.az, ref, m, REL)
+    say(f"{label}: worst conditioned force error {worst:.3e}, V gpu {V:.17g} cpu {ref['V']:.17g}")
+    assert ok, worst
+    assert V == pytest.approx(ref["V"], rel=REL)
+    scale = np.maximum(np.abs(ref["pe"]), 1e-3 * np.abs(ref["pe"]).mean())
+    assert (np.abs(pe - ref["pe"]) <= REL * scale).all()
+    np.testing.assert_allclose(sa, ref["sabs"], rtol=REL, atol=0)
+    return out
+
+
+def test_forces_plain_lj_4096(GU):
+    w = plain_lj(n_side=16, steps=1)
+    check_forces(GU, w.physics, w.state, "plain-lj-4096")
+
+
+def test_forces_mixture_coulomb_4096(GU):
+    w = mixture(n=4096, seed=21, steps=1)
+    check_forces(GU, w.physics, w.state, "mix10-4096")
+
+
+def test_forces_mixture_coulomb_32768(GU):
+    w = mixture(n=32768, seed=1002, steps=1)
+    check_forces(GU, w.physics, w.state, "mix10-32768")
+
+
+def test_forces_reference_like_default_table(GU):
+    ph = Physics(box=(24.0, 24.0, 24.0), enable_electrostatics=True)
+    st = reference_like(8, 24.0, 100.0, seed=42)
+    check_forces(GU, ph, st, "reference-like-512")
+
+
+def test_forces_separate_coulomb_cutoff_and_gravity(GU):
+    # Coulomb cutoff 3.2 max sigma (its own switch), gravity on: the unshared path.
+    w = mixture(n=4096, seed=5, steps=1)
+    ph = Physics(**{**w.physics.__dict__, "coulomb_cutoff": 3.2, "enable_gravity": True, "gravity_constant": 1e-3})
+    check_forces(GU, ph, w.state, "mix10-coulomb3.2-gravity")
+
+
+def test_forces_dense_cells_use_several_lds_chunks(GU):
+    # ~1400 candidates per stencil > one LDS tile: exercises the chunked path.
+    ph = Physics(box=(18.0, 18.0, 18.0))
+    st = random_gas(3, 4000, ph.box, species_pool=(0, 1), min_sep=0.0)
+    check_forces(GU, ph, st, "dense-two-species")
+
+
+@pytest.mark.parametrize("semantics", [0, GRID_MINIMAGE])
+def test_forces_small_box_two_cells(GU, semantics):
+    # floor(L / rc) = 2 on every axis: the reference's 2-cell multiplicity
+    # (semantics 0) or one count per pair (min-image semantics).
+    ph = Physics(box=(12.0, 12.0, 12.0), grid_semantics=semantics)
+    st = random_gas(8, 300, ph.box, species_pool=(0, 1, 2), min_sep=1.0)
+    assert ph.grid_dims() == (2, 2, 2)
+    check_forces(GU, ph, st, f"small-box-semantics{semantics}")
+
+
+def test_forces_single_cell_axis(GU):
+    ph = Physics(box=(30.0, 30.0, 7.0))
+    st = random_gas(9, 500, ph.box, species_pool=(2, 4), min_sep=1.0)
+    assert ph.grid_dims()[2] == 1
+    check_forces(GU, ph, st, "one-cell-axis")
+
+
+def test_cell_membership_bit_exact(GU):
+    w = mixture(n=32768, seed=1002, steps=1)
+    ph, st = w.physics, w.state
+    say(f"cells: n={st.n}")
+    ref, dims = O.grid_cells(st.x, st.y, st.z, np.asarray(ph.box), ph.force_rc())
+    with GU(ph, st) as u:
+        cells, gdims = u.cells()
+    assert gdims == dims
+    assert np.array_equal(cells, ref)
+
+
+def test_integrator_systems_bit_exact(GU):
+    w = mixture(n=4096, seed=4, steps=1)
+    ph, st = w.physics, w.state
+    st.ax, st.ay, st.az = st.vx * 40.0, -st.vy * 30.0, st.vz * 3.0
+    st.x = st.x + 0.6 * ph.box[0]  # partly outside the box: wrap has work to do
+    say("integrator systems")
+    with GU(ph, st) as u:
+        u.half_kick()
+        u.drift()
+        u.wrap()
+        got = u.download()
+    ref = st.copy()
+    O.half_kick(ref, ph.dt)
+    O.drift(ref, ph.dt)
+    O.wrap(ref, ph.box)
+    for a in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(getattr(got, a), getattr(ref, a)), a
+
+
+def test_thermostat_and_kinetic(GU):
+    w = mixture(n=4096, seed=6, temperature=60.0, steps=1)
+    ph, st = w.physics, w.state
+    ph = Physics(**{**ph.__dict__, "thermostat_temperature": 90.0})
+    say("thermostat")
+    with GU(ph, st) as u:
+        k = u.kinetic()
+        lam = u.thermostat()
+        got = u.download()
+    ref = st.copy()
+    assert k == pytest.approx(O.kinetic(ref, ph.mass), rel=1e-13)
+    lam_ref = O.thermostat(ph, ref)
+    assert lam == pytest.approx(lam_ref, rel=1e-14)
+    np.testing.assert_allclose(got.vx, ref.vx, rtol=1e-14, atol=0)
+
+
+def test_one_tick_matches_oracle(GU):
+    # Tick 1 starts with a = 0 (src/universe/mod.rs:272,357): kick, drift and
+    # wrap are then bit-exact, forces and the second kick within tolerance.
+    w = mixture(n=4096, seed=8, steps=1)
+    ph, st = w.physics, w.state
+    say("one tick")
+    ref, obs_ref = O.run(ph, st, 1)
+    with GU(ph, st) as u:
+        obs = u.step(1)
+        got = u.download()
+    for a in ("x", "y", "z"):
+        assert np.array_equal(getattr(got, a), getattr(ref, a)), a
+    ref_f = O.forces(ph, State(ref.x, ref.y, ref.z, ref.vx, ref.vy, ref.vz, st.species, st.charge))
+    m = np.asarray(ph.mass)[st.species]
+    ok, worst = force_tolerance_ok(got.ax, got.ay, got.az, ref_f, m, REL)
+    assert ok, worst
+    np.testing.assert_allclose(got.vx, ref.vx, rtol=1e-9, atol=1e-12 * np.abs(ref.vx).max())
+    assert obs[0, 1] == pytest.approx(obs_ref[0, 1], rel=REL)  # V
+    assert obs[0, 0] == pytest.approx(obs_ref[0, 0], rel=1e-12)  # K
+    assert obs[0, 4] == pytest.approx(obs_ref[0, 4], rel=1e-13)  # lambda
+
+
+def test_short_trajectory_tracks_oracle(GU):
+    # 20 ticks of the plain LJ workload: trajectories stay within round-off
+    # growth of the oracle's (chaos amplifies ~1e-16 slowly over few steps).
+    w = plain_lj(n_side=10, steps=20)
+    ph, st = w.physics, w.state
+    say("20 ticks plain LJ n=1000")
+    ref, obs_ref = O.run(ph, st, 20)
+    with GU(ph, st) as u:
+        obs = u.step(20)
+        got = u.download()
+    np.testing.assert_allclose(got.x, ref.x, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(obs[:, 5], obs_ref[:, 5], rtol=1e-9)
+
+
+def test_nve_energy_and_momentum_conservation(GU):
+    w = plain_lj(n_side=16, steps=1000)
+    ph, st = w.physics, w.state
+    say("NVE 1000 ticks plain LJ n=4096")
+    with GU(ph, st) as u:
+        obs = u.step(1000)
+        got = u.download()
+    e = obs[:, 5]
+    drift = abs(e[-1] - e[100]) / abs(e[100])
+    say(f"NVE relative drift {drift:.3e}")
+    assert drift < 1e-4
+    m = np.asarray(ph.mass)[st.species]
+    p = np.array([(m * got.vx).sum(), (m * got.vy).sum(), (m * got.vz).sum()])
+    assert np.linalg.norm(p) < 1e-10 * (m * np.abs(got.vx)).sum()
+
+
+def test_step_is_bitwise_deterministic(GU):
+    w = mixture(n=32768, seed=1002, steps=50)
+    ph, st = w.physics, w.state
+    say("determinism 2 x 50 ticks n=32768")
+    outs = []
+    for _ in range(2):
+        with GU(ph, st) as u:
+            obs = u.step(50)
+            outs.append((u.download(), obs))
+    for a in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"):
+        assert np.array_equal(getattr(outs[0][0], a), getattr(outs[1][0], a)), a
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("n,rmax_frac,nbins", [(32768, 0.25, 200), (4096, 0.5, 100), (20000, 0.2, 5000)])
+def test_radial_distribution_bit_exact(GU, n, rmax_frac, nbins):
+    w = mixture(n=n, seed=n + 1, steps=1)
+    ph, st = w.physics, w.state
+    say(f"g(r) n={n} r_max={rmax_frac}L bins={nbins}")
+    r_max = rmax_frac * ph.box[0]
+    c_ref, g_ref, rm_ref, dr_ref = O.radial_distribution(st.x, st.y, st.z, np.asarray(ph.box), r_max, nbins)
+    with GU(ph, st) as u:
+        c, g, rm, dr = u.radial_distribution(r_max, nbins)
+    assert (rm, dr) == (rm_ref, dr_ref)
+    assert np.array_equal(c, c_ref)
+    assert np.array_equal(g, g_ref)
+
+
+def test_partial_radial_distribution_bit_exact(GU):
+    w = mixture(n=8000, seed=77, steps=1)
+    ph, st = w.physics, w.state
+    say("partial g(r)")
+    with GU(ph, st) as u:
+        for ta, tb in ((4, 5), (0, 0), (9, 2)):
+            c_ref, g_ref, _, _ = O.radial_distribution(st.x, st.y, st.z, np.asarray(ph.box), 6.0, 120,
+                                                       species=st.species, ta=ta, tb=tb)
+            c, g, _, _ = u.radial_distribution_between(ta, tb, 6.0, 120)
+            assert np.array_equal(c, c_ref), (ta, tb)
+            assert np.array_equal(g, g_ref)
+
+
+@pytest.mark.parametrize("n,density", [(32768, 1.0), (5000, 0.3)])
+def test_cluster_labels_bit_exact(GU, n, density):
+    w = mixture(n=n, seed=n + 3, steps=1)
+    ph, st = w.physics, w.state
+    if density != 1.0:
+        s = density ** (-1.0 / 3.0)
+        ph = Physics(**{**ph.__dict__, "box": tuple(b * s for b in ph.box)})
+        st.x, st.y, st.z = st.x * s, st.y * s, st.z * s
+    say(f"clusters n={n} density x{density}")
+    lab_ref, s_ref = O.clusters(st.x, st.y, st.z, st.species, DEFAULT_SIGMA, np.asarray(ph.box))
+    with GU(ph, st) as u:
+        lab, s = u.clusters(1.5)
+    assert np.array_equal(lab, lab_ref)
+    assert s == s_ref
+
+
+def test_cluster_fixture_dimers_and_monomers(GU):
+    pos = [(10, 10, 10), (12, 10, 10), (30, 10, 10), (32, 10, 10), (50, 50, 50), (5, 50, 50)]
+    x, y, z = [np.array([p[k] for p in pos], dtype=float) - 32.0 for k in range(3)]
+    st = State(x, y, z, np.zeros(6), np.zeros(6), np.zeros(6), np.zeros(6, dtype=np.uint8), np.zeros(6))
+    ph = Physics(box=(64.0, 64.0, 64.0))
+    with GU(ph, st) as u:
+        lab, s = u.clusters(1.5)
+    assert lab.tolist() == [0, 0, 2, 2, 4, 5]
+    assert (s["aggregates"], s["monomers"], s["largest"], s["bound_pairs"], s["mean_size"]) == (2, 2, 2, 2, 1.5)
+
+
+def test_empty_universe(GU):
+    z = np.zeros(0)
+    st = State(z, z, z, z, z, z, np.zeros(0, dtype=np.uint8), z)
+    with GU(Physics(box=(10.0, 10.0, 10.0)), st) as u:
+        assert u.forces() == (0.0, 0.0)
+        assert u.step(3).shape == (3, 6)
+        c, g, _, _ = u.radial_distribution(4.0, 10)
+        assert c.sum() == 0
