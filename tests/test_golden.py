@@ -1,0 +1,70 @@
+"""Golden fixtures (tests/golden/*.npz, written by tests/golden/make_golden.py).
+
+CPU: the oracle still reproduces them exactly (regression pin of the
+checker).  GPU: the HIP path matches them at the parity bar."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from genesis_amd.inputs import State
+from tests.golden.make_golden import physics_from_vector
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+
+
+def load(path):
+    d = dict(np.load(path, allow_pickle=False))
+    ph = physics_from_vector(d["physics"])
+    st = State(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["species"].astype(np.uint8), d["charge"])
+    return ph, st, d
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
+def test_oracle_reproduces_fixture(path):
+    from oracle import oracle as O
+    ph, st, d = load(path)
+    f = O.forces(ph, st)
+    for a in ("ax", "ay", "az", "pe", "sabs"):
+        assert np.array_equal(f[a], d[a]), a
+    assert f["V"] == float(d["V"])
+    fin, obs = O.run(ph, st, int(d["nsteps"]))
+    assert np.array_equal(obs, d["obs"])
+    assert np.array_equal(fin.x, d["fin_x"])
+    if "gr_counts" in d:
+        c, g, _, _ = O.radial_distribution(st.x, st.y, st.z, np.asarray(ph.box), float(d["gr_rmax"]),
+                                           len(d["gr_counts"]))
+        assert np.array_equal(c, d["gr_counts"]) and np.array_equal(g, d["gr_g"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
+def test_gpu_matches_fixture(path):
+    from genesis_amd.universe import GpuUniverse
+    ph, st, d = load(path)
+    print(f"[golden] {os.path.basename(path)} n={st.n} dims={ph.grid_dims()}", flush=True)
+    with GpuUniverse(ph, st) as u:
+        V, _ = u.forces()
+        got = u.download()
+        pe, sabs = u.force_extras()
+        if "gr_counts" in d:
+            c, g, _, _ = u.radial_distribution(float(d["gr_rmax"]), len(d["gr_counts"]))
+            lab, cs = u.clusters(1.5)
+        obs = u.step(int(d["nsteps"]))
+        fin = u.download()
+    m = np.asarray(ph.mass)[st.species]
+    a_ref = np.stack([d["ax"], d["ay"], d["az"]], 1)
+    a = np.stack([got.ax, got.ay, got.az], 1)
+    scale = np.maximum(np.abs(a_ref).max(1), d["sabs"] / m)
+    assert (np.abs(a - a_ref).max(1) <= 1e-10 * scale).all()
+    assert V == pytest.approx(float(d["V"]), rel=1e-10)
+    if "gr_counts" in d:
+        assert np.array_equal(c, d["gr_counts"]) and np.array_equal(g, d["gr_g"])
+        assert np.array_equal(lab, d["labels"])
+        cs_ref = d["cluster_stats"]
+        assert [cs["monomers"], cs["aggregates"], cs["largest"], cs["mean_size"], cs["bound_pairs"]] == \
+            cs_ref.tolist()
+    np.testing.assert_allclose(obs[:, 5], d["obs"][:, 5], rtol=1e-9)
+    np.testing.assert_allclose(fin.x, d["fin_x"], rtol=0, atol=1e-9)
