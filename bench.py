@@ -46,7 +46,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--pair-mode", type=int, default=-1, help="-1 autotune, 0 cell tiles, 1 Verlet list")
+    ap.add_argument("--lanes", type=int, default=4)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 256 MiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: the path shards by spatial domain "
                                    "(see DESIGN.md) but no multi-GPU figure is produced; run with --gpus 1"}))
@@ -63,9 +69,12 @@ def main() -> int:
         f"warmup={args.warmup}")
 
     u = GpuUniverse(ph, st)
-    # pairs in range (for the FLOP count): from one standalone force pass
-    u.forces()
-    pe, sabs = u.force_extras()
+    if args.pair_mode < 0:
+        tuning = u.autotune(40)
+        log(f"autotuned: {tuning}")
+    else:
+        u.set_tuning(args.pair_mode, args.lanes)
+        tuning = u.tuning()
 
     def run(k, observe=False):
         done = 0
@@ -76,6 +85,7 @@ def main() -> int:
             u.radial_distribution(gr_rmax, gr_bins)
     if args.warmup:
         run(args.warmup)
+    builds0 = u.tuning()["list_builds"]
     u.timing(enable=True)
     t0 = time.perf_counter()
     run(args.steps)
@@ -125,12 +135,28 @@ def main() -> int:
                           "fp64_frac": value * (FLOP_PER_PAIR * npairs / n) / (FP64_PEAK_TFLOPS * 1e12),
                           "bytes_per_atom_step": STEP_BYTES, "flop_per_atom_step": FLOP_PER_PAIR * npairs / n},
         "phases_ms_per_step": {k: tm[k] / max(tm["steps"], 1) for k in ("rebuild_ms", "pair_ms", "kick_ms")},
+        "tuning": {k: tuning[k] for k in ("pair_mode", "lanes_per_atom", "skin", "ms_per_tick")},
+        "list_builds_in_timed_region": u.tuning()["list_builds"] - builds0,
     }
     u.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def copy_probe() -> int:
+    """A plain device copy of known size, for scaling rocprofv3's FETCH_SIZE /
+    WRITE_SIZE counters (read 256 MiB, write 256 MiB per launch)."""
+    import torch
+    n = 256 * 1024 * 1024 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n, "launches": 5}))
     return 0
 
 

@@ -49,6 +49,11 @@ class Timing(ctypes.Structure):
                 ("pair_ms", ctypes.c_double), ("kick_ms", ctypes.c_double), ("pairs_in_range", ctypes.c_int64)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("pair_mode", ctypes.c_int32), ("lanes_per_atom", ctypes.c_int32), ("skin", ctypes.c_double),
+                ("list_life", ctypes.c_int64), ("list_builds", ctypes.c_int64), ("ms_per_tick", ctypes.c_double)]
+
+
 _D = ctypes.POINTER(ctypes.c_double)
 _U8 = ctypes.POINTER(ctypes.c_uint8)
 _U32 = ctypes.POINTER(ctypes.c_uint32)
@@ -75,6 +80,9 @@ _SIGNATURES = {
     "gs_radial_distribution": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_int32, _U64, _D, _D, _D]),
     "gs_radial_distribution_between": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                                       ctypes.c_int32, _U64, _D, _D, _D]),
+    "gs_get_tuning": (ctypes.c_int, [_H, ctypes.POINTER(Tuning)]),
+    "gs_set_tuning": (ctypes.c_int, [_H, ctypes.POINTER(Tuning)]),
+    "gs_autotune": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.POINTER(Tuning)]),
     "gs_timing_enable": (ctypes.c_int, [_H, ctypes.c_int]),
     "gs_timing_read": (ctypes.c_int, [_H, ctypes.POINTER(Timing)]),
     "gs_clusters": (ctypes.c_int, [_H, ctypes.c_double, _U32, ctypes.POINTER(ClusterStats)]),

@@ -717,6 +717,249 @@ __global__ __launch_bounds__(BLOCK) void k_cells_entity(KParams P, const uint32_
     out[id[k]] = cell_key(P, p.x, p.y, p.z);
 }
 
+
+// ---- Verlet-list path ---------------------------------------------------------
+//
+// The cell list is rebuilt on a grid of edge >= rc + skin and, from the same
+// LDS-tiled 27-cell stencil as k_pair, every atom gets the list of atoms
+// within rc + skin.  The lists are reused while no atom has moved more than
+// skin/2 from its position at the build (checked every tick), so every pair
+// within rc is in some list: the pair set, and hence the forces, are the
+// reference's; only the evaluation order differs.
+
+constexpr uint32_t NBR_IDX_MASK = 0x0FFFFFFFu;  // entry = (species << 28) | sorted index
+
+// Entry k of atom i with T lanes per atom: row k/T of an [rows][N][T] array,
+// so the T lanes of consecutive atoms read consecutive words.
+__device__ __forceinline__ size_t nbr_slot(uint32_t k, uint32_t i, uint32_t N, uint32_t logT) {
+    return (((size_t)(k >> logT) * N + i) << logT) + (k & ((1u << logT) - 1u));
+}
+
+__global__ __launch_bounds__(PAIR_THREADS) void k_list_build(KParams PL, const uint32_t* __restrict__ cstart,
+                                                             const double4* __restrict__ pos4,
+                                                             const uint8_t* __restrict__ type,
+                                                             uint32_t* __restrict__ nbr, uint32_t* __restrict__ nlen,
+                                                             uint32_t N, uint32_t logT, uint32_t cap, double cut2,
+                                                             uint32_t* __restrict__ overflow) {
+    __shared__ double4 tile[TILE_CAP];
+    __shared__ uint32_t tidx[TILE_CAP];
+    __shared__ uint32_t seg_base[27];
+    __shared__ uint32_t seg_pref[28];
+    __shared__ double seg_sh[27][3];
+    const uint32_t c = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t h0 = cstart[c], nh = cstart[c + 1] - h0;
+    if (nh == 0) return;
+    if (tid < 27) {
+        const uint32_t nx = PL.dims[0], ny = PL.dims[1], nz = PL.dims[2];
+        const int cx = (int)(c % nx), cy = (int)((c / nx) % ny), cz = (int)(c / (nx * ny));
+        int ix = cx + tid % 3 - 1, iy = cy + (tid / 3) % 3 - 1, iz = cz + tid / 9 - 1;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        if (ix < 0) { ix += nx; sx = -PL.box[0]; } else if (ix >= (int)nx) { ix -= nx; sx = PL.box[0]; }
+        if (iy < 0) { iy += ny; sy = -PL.box[1]; } else if (iy >= (int)ny) { iy -= ny; sy = PL.box[1]; }
+        if (iz < 0) { iz += nz; sz = -PL.box[2]; } else if (iz >= (int)nz) { iz -= nz; sz = PL.box[2]; }
+        const uint32_t nc = (uint32_t)ix + nx * ((uint32_t)iy + ny * (uint32_t)iz);
+        const uint32_t b = cstart[nc];
+        seg_base[tid] = b;
+        seg_pref[tid + 1] = cstart[nc + 1] - b;
+        seg_sh[tid][0] = sx;
+        seg_sh[tid][1] = sy;
+        seg_sh[tid][2] = sz;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        seg_pref[0] = 0;
+        for (int s2 = 0; s2 < 27; ++s2) seg_pref[s2 + 1] += seg_pref[s2];
+    }
+    __syncthreads();
+    const uint32_t ncand = seg_pref[27];
+    const int nchunks = (int)((ncand + TILE_CAP - 1) / TILE_CAP);
+    for (int ch = 0; ch < nchunks; ++ch) {
+        const uint32_t c0 = (uint32_t)ch * TILE_CAP;
+        const uint32_t len = min((uint32_t)TILE_CAP, ncand - c0);
+        __syncthreads();
+        for (uint32_t t = tid; t < len; t += PAIR_THREADS) {
+            const uint32_t idx = c0 + t;
+            int sg = 0;
+            while (seg_pref[sg + 1] <= idx) ++sg;
+            const uint32_t j = seg_base[sg] + (idx - seg_pref[sg]);
+            double4 pj = pos4[j];
+            pj.x += seg_sh[sg][0];
+            pj.y += seg_sh[sg][1];
+            pj.z += seg_sh[sg][2];
+            tile[t] = pj;
+            tidx[t] = ((uint32_t)type[j] << 28) | j;
+        }
+        __syncthreads();
+        for (uint32_t i = w; i < nh; i += PAIR_WAVES) {
+            const uint32_t gi = h0 + i;
+            const double4 pi = pos4[gi];
+            uint32_t cnt = ch == 0 ? 0u : nlen[gi];
+            for (uint32_t base = 0; base < len; base += 64) {
+                const uint32_t j = base + lane;
+                bool in = false;
+                uint32_t e = 0;
+                if (j < len) {
+                    const double4 pj = tile[j];
+                    e = tidx[j];
+                    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                    in = (r2 < cut2) && ((e & NBR_IDX_MASK) != gi);
+                }
+                const uint64_t m = __ballot(in);
+                if (in) {
+                    const uint32_t k = cnt + mask_rank(m);
+                    if (k < cap) nbr[nbr_slot(k, gi, N, logT)] = e;
+                }
+                cnt += (uint32_t)__popcll(m);
+            }
+            if (lane == 0) {
+                if (ch == nchunks - 1 && cnt > cap) {
+                    atomicOr(overflow, 1u);
+                    cnt = cap;
+                }
+                nlen[gi] = cnt;
+            }
+        }
+    }
+}
+
+// Per-atom list traversal: T lanes per atom split its list (entries s, s+T,
+// ...), sum with shuffles.  The minimum image is taken per pair
+// (dx - L rint(dx / L)); pairs of the skin shell beyond rc get exactly zero
+// from the clamped switch, coincident pairs are masked.
+template <int LOGT, bool COUL, bool SHARED, bool SINGLE>
+__global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, const double4* __restrict__ pos4,
+                                                     const uint8_t* __restrict__ type,
+                                                     const uint32_t* __restrict__ nbr,
+                                                     const uint32_t* __restrict__ nlen,
+                                                     const PairCoef* __restrict__ coef, double* __restrict__ ax,
+                                                     double* __restrict__ ay, double* __restrict__ az,
+                                                     double* __restrict__ vpart) {
+    constexpr int T = 1 << LOGT;
+    __shared__ PairCoef sc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];
+    __shared__ double wv[BLOCK / WAVE];
+    const int tid = threadIdx.x;
+    if (SINGLE) {
+        if (tid == 0) sc[0] = coef[0];
+    } else {
+        for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += BLOCK) sc[t] = coef[t];
+    }
+    __syncthreads();
+    const uint32_t g = blockIdx.x * BLOCK + tid;
+    const uint32_t i = g >> LOGT;
+    const uint32_t s = g & (T - 1);
+    double fx = 0.0, fy = 0.0, fz = 0.0, vl = 0.0;
+    int ti = 0;
+    if (i < N) {
+        const double4 pi = pos4[i];
+        ti = type[i];
+        const double kqi = P.ke * pi.w;
+        const double mi = P.mass[ti];
+        const double Lx = P.box[0], Ly = P.box[1], Lz = P.box[2];
+        const double iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
+        const uint32_t len = nlen[i];
+        const PairCoef* row = SINGLE ? sc : sc + ti * GS_NSPECIES;
+        for (uint32_t k = s; k < len; k += T) {
+            const uint32_t e = nbr[nbr_slot(k, i, N, LOGT)];
+            const uint32_t j = e & NBR_IDX_MASK;
+            const int tj = (int)(e >> 28);
+            const double4 pj = pos4[j];
+            double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            dx = fma(-Lx, rint(dx * iLx), dx);
+            dy = fma(-Ly, rint(dy * iLy), dy);
+            dz = fma(-Lz, rint(dz * iLz), dz);
+            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+            const bool ok = r2 > P.eps2;
+            const double r2s = ok ? r2 : 1.0;
+            const double inv_d = rsqrt_refined(r2s);
+            const double d = r2s * inv_d;
+            const PairCoef& pc = SINGLE ? row[0] : row[tj];
+            double fmag, vv;
+            pair_law<COUL, SHARED>(P, pc, d, inv_d, kqi * pj.w, mi * P.mass[tj], fmag, vv);
+            const double fs = ok ? fmag * inv_d : 0.0;
+            fx = fma(fs, dx, fx);
+            fy = fma(fs, dy, fy);
+            fz = fma(fs, dz, fz);
+            vl += ok ? vv : 0.0;
+        }
+    }
+#pragma unroll
+    for (int off = T / 2; off > 0; off >>= 1) {
+        fx += __shfl_xor(fx, off, WAVE);
+        fy += __shfl_xor(fy, off, WAVE);
+        fz += __shfl_xor(fz, off, WAVE);
+    }
+    if (i < N && s == 0) {
+        const double im = P.inv_mass[ti];
+        ax[i] = fx * im;
+        ay[i] = fy * im;
+        az[i] = fz * im;
+    }
+    vl = wave_sum(vl);
+    if ((tid & 63) == 0) wv[tid >> 6] = vl;
+    __syncthreads();
+    if (tid == 0) vpart[blockIdx.x] = 0.5 * ((wv[0] + wv[1]) + (wv[2] + wv[3]));
+}
+
+// List-mode tick start: lambda, kick, drift, wrap (as k_integrate_key) plus
+// the displacement from the list-build position: any atom beyond skin/2
+// flags the tick (the host then replays from its snapshot with a shorter
+// list lifetime); the largest displacement steers the lifetime.
+__global__ __launch_bounds__(BLOCK) void k_integrate_disp(KParams P, double4* __restrict__ pos4,
+                                                          double* __restrict__ vx, double* __restrict__ vy,
+                                                          double* __restrict__ vz, const double* __restrict__ ax,
+                                                          const double* __restrict__ ay,
+                                                          const double* __restrict__ az,
+                                                          const double* __restrict__ lam_ptr,
+                                                          const double4* __restrict__ xref, double thr2,
+                                                          uint32_t* __restrict__ violation,
+                                                          unsigned long long* __restrict__ dispmax) {
+    __shared__ double red[BLOCK / WAVE];
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    double d2 = 0.0;
+    if (k < P.n) {
+        double4 p = pos4[k];
+        const double lam = *lam_ptr;
+        double a = vx[k] * lam, b = vy[k] * lam, c = vz[k] * lam;
+        a = a + ax[k] * P.half_dt;
+        b = b + ay[k] * P.half_dt;
+        c = c + az[k] * P.half_dt;
+        vx[k] = a;
+        vy[k] = b;
+        vz[k] = c;
+        p.x = p.x + a * P.dt;
+        p.y = p.y + b * P.dt;
+        p.z = p.z + c * P.dt;
+        if (P.flags & F_WRAP) {
+            p.x = rem_euclid(p.x, 2.0 * P.half[0]) - P.half[0];
+            p.y = rem_euclid(p.y, 2.0 * P.half[1]) - P.half[1];
+            p.z = rem_euclid(p.z, 2.0 * P.half[2]) - P.half[2];
+        }
+        pos4[k] = p;
+        const double4 r = xref[k];
+        double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+        dx = fma(-P.box[0], rint(dx / P.box[0]), dx);
+        dy = fma(-P.box[1], rint(dy / P.box[1]), dy);
+        dz = fma(-P.box[2], rint(dz / P.box[2]), dz);
+        d2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        if (!(d2 <= thr2)) atomicOr(violation, 1u);  // also catches NaN
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off, WAVE));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = d2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        if (m > 0.0 && m == m) atomicMax(dispmax, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_copy_xref(int n, const double4* __restrict__ p, double4* __restrict__ r) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k < n) r[k] = p[k];
+}
+
 // ---- observation lenses --------------------------------------------------------
 
 // Auxiliary grid of a lens: SpatialGrid::new(size, min_cell) over a point set
@@ -979,6 +1222,7 @@ struct gs_universe {
     PairCoef* coef = nullptr;
     double* vpart = nullptr;
     uint32_t nvpart = 0;
+    uint32_t nvpart_list = 0;
     double* kpart = nullptr;
     uint32_t* counter = nullptr;
     double* scal = nullptr;  // [0] lambda, [1] V, [2] K, [3] spare
@@ -999,6 +1243,30 @@ struct gs_universe {
     unsigned long long* l_u64 = nullptr;  // [0..3] FoF stats, [4] bound pairs
     double* l_sigma = nullptr;
     bool present[GS_NSPECIES] = {false};
+    std::vector<uint8_t> species_h;  // host copies of the last upload
+    std::vector<double> charge_h;
+    // Verlet-list path
+    KParams kpl;                // grid of edge >= rc + skin
+    bool list_ok = false;       // the list geometry admits >= 3 cells per axis
+    bool list_valid = false;    // lists match the current sorted order
+    int pair_mode = 1;          // 0 = per-tick cell tiles, 1 = Verlet list
+    int logT = 2;               // lanes per atom = 1 << logT
+    double skin = 0.0;
+    int64_t list_life = 10;     // ticks between list builds (adapted)
+    int64_t list_age = 0;
+    uint32_t nbr_cap = 0;
+    uint32_t* nbr = nullptr;
+    uint32_t* nlen = nullptr;
+    double4* xref = nullptr;
+    uint32_t* flags = nullptr;  // [0] skin violation, [1] list overflow
+    unsigned long long* dispmax = nullptr;
+    int64_t list_builds = 0;
+    // snapshot for replays (allocated on first list-mode step)
+    double4* snap_pos = nullptr;
+    double* snap_v[3] = {nullptr, nullptr, nullptr};
+    double* snap_a[3] = {nullptr, nullptr, nullptr};
+    uint8_t* snap_type = nullptr;
+    uint32_t* snap_id = nullptr;
     // event timing of gs_step
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -1130,8 +1398,7 @@ int check_state(gs_universe* u) {
 
 // Cell-list rebuild: key + histogram (optionally fused with kick/drift/wrap),
 // scan, scatter, per-cell id ordering + gather into the other buffer.
-int rebuild(gs_universe* u, bool kick_drift) {
-    const KParams& P = u->kp;
+int rebuild(gs_universe* u, bool kick_drift, const KParams& P) {
     const int n = u->n;
     if (n == 0) return GS_OK;
     const int c = u->cur, o = 1 - c;
@@ -1183,6 +1450,149 @@ int launch_pair(gs_universe* u) {
     return GS_OK;
 }
 
+
+// Geometry of the list grid for a skin (cells >= rc + skin, >= 3 per axis).
+bool list_geometry(gs_universe* u, double skin) {
+    KParams q = u->kp;
+    const double rl = u->kp.rc + skin;
+    for (int a = 0; a < 3; ++a) {
+        double nc = std::floor(q.box[a] / rl);
+        if (!(nc >= 3.0)) return false;
+        if (nc > 1024.0) nc = 1024.0;
+        q.dims[a] = (uint32_t)nc;
+        q.cell[a] = q.box[a] / (double)q.dims[a];
+    }
+    q.ncell = q.dims[0] * q.dims[1] * q.dims[2];
+    if (q.ncell > u->kp.ncell) return false;  // count/cstart are sized for the force grid
+    u->kpl = q;
+    u->skin = skin;
+    return true;
+}
+
+int list_alloc(gs_universe* u, uint32_t cap) {
+    int rc;
+    const size_t n = (size_t)std::max(1, u->n);
+    if (!u->nlen) {
+        if ((rc = dalloc(&u->nlen, n)) || (rc = dalloc(&u->xref, n)) || (rc = dalloc(&u->flags, 4)) ||
+            (rc = dalloc(&u->dispmax, 1)) || (rc = dalloc(&u->snap_pos, n)) || (rc = dalloc(&u->snap_type, n)) ||
+            (rc = dalloc(&u->snap_id, n)))
+            return rc;
+        for (int a = 0; a < 3; ++a)
+            if ((rc = dalloc(&u->snap_v[a], n)) || (rc = dalloc(&u->snap_a[a], n))) return rc;
+    }
+    if (u->nbr_cap < cap) {
+        (void)hipFree(u->nbr);
+        u->nbr = nullptr;
+        u->nbr_cap = 0;
+        // rows of 8 lanes so any lanes-per-atom setting fits
+        const size_t rows = ((size_t)cap + 7) / 8 + 1;
+        if ((rc = dalloc(&u->nbr, rows * 8 * n))) return rc;
+        u->nbr_cap = cap;
+    }
+    return GS_OK;
+}
+
+int snapshot(gs_universe* u, bool restore) {
+    const int c = u->cur;
+    const size_t n = (size_t)u->n, b = n * sizeof(double);
+    auto cp = [&](void* live, void* snap, size_t bytes) {
+        return restore ? hipMemcpyAsync(live, snap, bytes, hipMemcpyDeviceToDevice, u->stream)
+                       : hipMemcpyAsync(snap, live, bytes, hipMemcpyDeviceToDevice, u->stream);
+    };
+    HIPCHK(cp(u->pos4[c], u->snap_pos, n * sizeof(double4)));
+    for (int a = 0; a < 3; ++a) {
+        HIPCHK(cp(u->v[c][a], u->snap_v[a], b));
+        HIPCHK(cp(u->a[a], u->snap_a[a], b));
+    }
+    HIPCHK(cp(u->type[c], u->snap_type, n));
+    HIPCHK(cp(u->id[c], u->snap_id, n * sizeof(uint32_t)));
+    return GS_OK;
+}
+
+int launch_pair_list(gs_universe* u) {
+    const KParams& P = u->kp;
+    const int c = u->cur;
+    const uint32_t N = (uint32_t)u->n;
+    const unsigned nb = nblocks((int64_t)N << u->logT);
+#define GS_PL(L, C, S, SG)                                                                                       \
+    k_pair_list<L, C, S, SG><<<nb, BLOCK, 0, u->stream>>>(P, N, u->pos4[c], u->type[c], u->nbr, u->nlen, u->coef, \
+                                                          u->a[0], u->a[1], u->a[2], u->vpart)
+#define GS_PL_T(C, S, SG)          \
+    switch (u->logT) {             \
+        case 0: GS_PL(0, C, S, SG); break; \
+        case 1: GS_PL(1, C, S, SG); break; \
+        case 2: GS_PL(2, C, S, SG); break; \
+        default: GS_PL(3, C, S, SG); break; \
+    }
+    if (u->single) {
+        GS_PL_T(false, false, true);
+    } else if (!u->coul) {
+        GS_PL_T(false, false, false);
+    } else if (u->shared_switch) {
+        GS_PL_T(true, true, false);
+    } else {
+        GS_PL_T(true, false, false);
+    }
+#undef GS_PL_T
+#undef GS_PL
+    u->nvpart_list = nb;
+    HIPCHK(hipGetLastError());
+    return GS_OK;
+}
+
+
+// The tick loop of gs_step: kick, drift, wrap, [cell list], forces, kick,
+// Berendsen, in build_schedule order (src/universe/mod.rs:355-369).
+int run_ticks(gs_universe* u, int64_t n_steps, bool observe, bool list) {
+    int rc;
+    const gs_params& p = u->prm;
+    const bool thermo = p.enable_thermostat != 0;
+    const double rl = u->kp.rc + u->skin;
+    for (int64_t s = 0; s < n_steps; ++s) {
+        if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 0], u->stream));
+        uint32_t nvp = u->nvpart;
+        if (list) {
+            if (!u->list_valid || u->list_age >= u->list_life) {
+                if ((rc = rebuild(u, true, u->kpl))) return rc;
+                const int c = u->cur;
+                k_copy_xref<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->n, u->pos4[c], u->xref);
+                k_list_build<<<u->kpl.ncell, PAIR_THREADS, 0, u->stream>>>(
+                    u->kpl, u->cstart, u->pos4[c], u->type[c], u->nbr, u->nlen, (uint32_t)u->n, (uint32_t)u->logT,
+                    u->nbr_cap, rl * rl, u->flags + 1);
+                u->list_valid = true;
+                u->list_age = 0;
+                u->list_builds += 1;
+            } else {
+                const int c = u->cur;
+                const double h = 0.5 * u->skin;
+                k_integrate_disp<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->kp, u->pos4[c], u->v[c][0], u->v[c][1],
+                                                                         u->v[c][2], u->a[0], u->a[1], u->a[2],
+                                                                         u->scal, u->xref, h * h, u->flags,
+                                                                         u->dispmax);
+            }
+            HIPCHK(hipGetLastError());
+            if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 1], u->stream));
+            if ((rc = launch_pair_list(u))) return rc;
+            nvp = u->nvpart_list;
+            u->list_age += 1;
+        } else {
+            if ((rc = rebuild(u, true, u->kp))) return rc;
+            u->list_valid = false;
+            if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 1], u->stream));
+            if ((rc = launch_pair<false>(u))) return rc;
+        }
+        if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 2], u->stream));
+        const int c = u->cur;
+        k_kick_ke<true><<<nblocks(u->n), BLOCK, 0, u->stream>>>(
+            u->kp, u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->type[c], u->kpart, u->counter,
+            u->vpart, nvp, u->scal, observe ? u->obs + s : nullptr, thermo ? 1 : 0, p.thermostat_temperature,
+            std::fmax(p.thermostat_tau, 1.0), p.thermal_constant);
+        if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
+    }
+    HIPCHK(hipGetLastError());
+    return GS_OK;
+}
+
 }  // namespace
 
 // =============================================================================
@@ -1213,6 +1623,9 @@ int gs_create(const gs_params* params, int64_t n_atoms, gs_universe** out) {
     u->n = (int)n_atoms;
     std::memcpy(&u->prm, params, sizeof(gs_params));
     std::vector<PairCoef> coef = derive(u);
+    // Default list skin: 6% of the cutoff (adapted lifetime keeps it valid).
+    u->list_ok = !u->small_box && list_geometry(u, 0.06 * u->kp.rc);
+    if (!u->list_ok) u->pair_mode = 0;
     (void)hipGetDevice(&u->dev);
     const size_t n = (size_t)std::max(1, u->n);
     const KParams& P = u->kp;
@@ -1233,7 +1646,7 @@ int gs_create(const gs_params* params, int64_t n_atoms, gs_universe** out) {
         if ((rc = dalloc(&u->a[a], n))) return bail(rc);
     if ((rc = dalloc(&u->key, n)) || (rc = dalloc(&u->ticket, n)) || (rc = dalloc(&u->perm, n)) ||
         (rc = dalloc(&u->count, (size_t)P.ncell)) || (rc = dalloc(&u->cstart, (size_t)P.ncell + 1)) ||
-        (rc = dalloc(&u->coef, coef.size())) || (rc = dalloc(&u->vpart, (size_t)u->nvpart)) ||
+        (rc = dalloc(&u->coef, coef.size())) || (rc = dalloc(&u->vpart, (size_t)std::max<uint32_t>(u->nvpart, nblocks((int64_t)n * 8)))) ||
         (rc = dalloc(&u->kpart, (size_t)nblocks(n))) || (rc = dalloc(&u->counter, 4)) ||
         (rc = dalloc(&u->scal, 4)) || (rc = dalloc(&u->pe, n)) || (rc = dalloc(&u->sabs, n)) ||
         (rc = dalloc(&u->stage, 9 * n)) || (rc = dalloc(&u->stage_u, n)))
@@ -1288,6 +1701,16 @@ int gs_destroy(gs_universe* u) {
     (void)hipFree(u->l_u64);
     (void)hipFree(u->l_sigma);
     for (hipEvent_t e : u->ev) (void)hipEventDestroy(e);
+    (void)hipFree(u->nbr);
+    (void)hipFree(u->nlen);
+    (void)hipFree(u->xref);
+    (void)hipFree(u->flags);
+    (void)hipFree(u->dispmax);
+    (void)hipFree(u->snap_pos);
+    for (int a = 0; a < 3; ++a) (void)hipFree(u->snap_v[a]);
+    for (int a = 0; a < 3; ++a) (void)hipFree(u->snap_a[a]);
+    (void)hipFree(u->snap_type);
+    (void)hipFree(u->snap_id);
     if (u->stream) (void)hipStreamDestroy(u->stream);
     delete u;
     return GS_OK;
@@ -1333,6 +1756,9 @@ int gs_upload_state(gs_universe* u, const double* x, const double* y, const doub
     HIPCHK(hipStreamSynchronize(u->stream));
     u->have_state = true;
     u->extras_valid = false;
+    u->list_valid = false;
+    u->species_h.assign(species, species + n);
+    u->charge_h.assign(charge, charge + n);
     std::memcpy(u->present, present, sizeof(present));
     return GS_OK;
 }
@@ -1364,7 +1790,8 @@ int gs_forces(gs_universe* u, double* potential, double* bond_energy) {
         if (bond_energy) *bond_energy = 0.0;
         return GS_OK;
     }
-    if ((rc = rebuild(u, false))) return rc;
+    if ((rc = rebuild(u, false, u->kp))) return rc;
+    u->list_valid = false;
     if ((rc = launch_pair<true>(u))) return rc;
     k_reduce_to<<<1, BLOCK, 0, u->stream>>>(u->vpart, u->nvpart, u->scal + 1);
     HIPCHK(hipGetLastError());
@@ -1508,35 +1935,60 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
         if ((rc = dalloc(&u->obs, (size_t)n_steps))) return rc;
         u->obs_cap = n_steps;
     }
-    const gs_params& p = u->prm;
-    const bool thermo = p.enable_thermostat != 0;
-    // Timing: 4 events per tick (start, after rebuild, after pair, end).
+    const bool list = u->pair_mode == 1 && u->list_ok && !u->small_box;
     const size_t nev = u->timing ? (size_t)(4 * n_steps) : 0;
     while (u->ev.size() < nev) {
         hipEvent_t e;
         HIPCHK(hipEventCreate(&e));
         u->ev.push_back(e);
     }
-    for (int64_t s = 0; s < n_steps; ++s) {
-        if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 0], u->stream));
-        if ((rc = rebuild(u, true))) return rc;
-        if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 1], u->stream));
-        if ((rc = launch_pair<false>(u))) return rc;
-        if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 2], u->stream));
-        const int c = u->cur;
-        k_kick_ke<true><<<nblocks(u->n), BLOCK, 0, u->stream>>>(
-            u->kp, u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->type[c], u->kpart, u->counter,
-            u->vpart, u->nvpart, u->scal, obs ? u->obs + s : nullptr, thermo ? 1 : 0, p.thermostat_temperature,
-            std::fmax(p.thermostat_tau, 1.0), p.thermal_constant);
-        if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
+    const int cur0 = u->cur;
+    if (list) {
+        if ((rc = list_alloc(u, u->nbr_cap ? u->nbr_cap : 384))) return rc;
+        if ((rc = snapshot(u, false))) return rc;
     }
-    const int c = u->cur;
-    k_apply_lambda<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->n, u->v[c][0], u->v[c][1], u->v[c][2], u->scal);
-    k_set_scalar<<<1, 1, 0, u->stream>>>(u->scal, 1.0);
-    HIPCHK(hipGetLastError());
+    for (int attempt = 0;; ++attempt) {
+        if (list) {
+            HIPCHK(hipMemsetAsync(u->flags, 0, 4 * sizeof(uint32_t), u->stream));
+            HIPCHK(hipMemsetAsync(u->dispmax, 0, sizeof(unsigned long long), u->stream));
+        }
+        if ((rc = run_ticks(u, n_steps, obs != nullptr, list))) return rc;
+        const int c = u->cur;
+        k_apply_lambda<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->n, u->v[c][0], u->v[c][1], u->v[c][2], u->scal);
+        k_set_scalar<<<1, 1, 0, u->stream>>>(u->scal, 1.0);
+        HIPCHK(hipGetLastError());
+        uint32_t fl[4] = {0, 0, 0, 0};
+        unsigned long long dm = 0;
+        if (list) {
+            HIPCHK(hipMemcpyAsync(fl, u->flags, sizeof(fl), hipMemcpyDeviceToHost, u->stream));
+            HIPCHK(hipMemcpyAsync(&dm, u->dispmax, sizeof(dm), hipMemcpyDeviceToHost, u->stream));
+        }
+        HIPCHK(hipStreamSynchronize(u->stream));
+        if (list && (fl[0] || fl[1])) {
+            // A list outlived its skin or overflowed: replay the call from the
+            // snapshot with a shorter lifetime / a larger list.
+            if (attempt > 40) return fail(GS_E_OVERFLOW, "neighbour list could not be made valid");
+            u->cur = cur0;
+            if ((rc = snapshot(u, true))) return rc;
+            u->list_valid = false;
+            if (fl[0]) u->list_life = std::max<int64_t>(1, u->list_life / 2);
+            if (fl[1]) {
+                const uint32_t cap = u->nbr_cap * 2;
+                if (cap > (1u << 16)) return fail(GS_E_OVERFLOW, "neighbour list exceeds 65536 entries per atom");
+                if ((rc = list_alloc(u, cap))) return rc;
+            }
+            continue;
+        }
+        if (list) {
+            double d2;
+            std::memcpy(&d2, &dm, sizeof(d2));
+            const double half = 0.5 * u->skin;
+            if (std::sqrt(d2) < 0.5 * half) u->list_life = std::min<int64_t>(u->list_life + 1 + u->list_life / 4, 500);
+        }
+        break;
+    }
     if (obs)
-        HIPCHK(hipMemcpyAsync(obs, u->obs, sizeof(gs_observables) * (size_t)n_steps, hipMemcpyDeviceToHost, u->stream));
-    HIPCHK(hipStreamSynchronize(u->stream));
+        HIPCHK(hipMemcpy(obs, u->obs, sizeof(gs_observables) * (size_t)n_steps, hipMemcpyDeviceToHost));
     u->extras_valid = false;
     if (u->timing) {
         for (int64_t s = 0; s < n_steps; ++s) {
@@ -1550,6 +2002,92 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
             u->tm.step_ms += (double)a + b + c2;
         }
         u->tm.steps += n_steps;
+    }
+    return GS_OK;
+}
+
+int gs_get_tuning(gs_universe* u, gs_tuning* out) {
+    if (!u || !out) return fail(GS_E_INVALID, "NULL argument");
+    out->pair_mode = (u->pair_mode == 1 && u->list_ok && !u->small_box) ? 1 : 0;
+    out->lanes_per_atom = 1 << u->logT;
+    out->skin = u->skin;
+    out->list_life = u->list_life;
+    out->list_builds = u->list_builds;
+    out->ms_per_tick = 0.0;
+    return GS_OK;
+}
+
+int gs_set_tuning(gs_universe* u, const gs_tuning* t) {
+    if (!u || !t) return fail(GS_E_INVALID, "NULL argument");
+    if (t->pair_mode != 0 && t->pair_mode != 1) return fail(GS_E_INVALID, "pair_mode must be 0 or 1");
+    if (t->pair_mode == 1) {
+        int logT;
+        switch (t->lanes_per_atom) {
+            case 1: logT = 0; break;
+            case 2: logT = 1; break;
+            case 4: logT = 2; break;
+            case 8: logT = 3; break;
+            default: return fail(GS_E_INVALID, "lanes_per_atom must be 1, 2, 4 or 8");
+        }
+        if (!(t->skin >= 0.0) || !std::isfinite(t->skin)) return fail(GS_E_INVALID, "skin must be >= 0");
+        if (u->small_box || !list_geometry(u, t->skin))
+            return fail(GS_E_INVALID, "list mode needs >= 3 cells of edge rc + skin on every axis");
+        u->list_ok = true;
+        u->logT = logT;
+        if (t->list_life > 0) u->list_life = t->list_life;
+    }
+    u->pair_mode = t->pair_mode;
+    u->list_valid = false;
+    return GS_OK;
+}
+
+int gs_autotune(gs_universe* u, int32_t trial_steps, gs_tuning* best) {
+    int rc = check_state(u);
+    if (rc) return rc;
+    if (trial_steps < 4) return fail(GS_E_INVALID, "trial_steps must be >= 4");
+    const int n = u->n;
+    // Host copy of the state; every trial starts from it.
+    std::vector<double> st(9 * (size_t)std::max(n, 1));
+    double* p[9];
+    for (int k = 0; k < 9; ++k) p[k] = st.data() + (size_t)k * n;
+    if ((rc = gs_download_state(u, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8]))) return rc;
+    const std::vector<uint8_t> sp = u->species_h;
+    const std::vector<double> q = u->charge_h;
+    std::vector<gs_tuning> cand;
+    cand.push_back(gs_tuning{0, 1, 0.0, 0, 0, 0.0});
+    if (!u->small_box)
+        for (double f : {0.03, 0.06, 0.10})
+            for (int T : {1, 2, 4, 8}) cand.push_back(gs_tuning{1, T, f * u->kp.rc, 0, 0, 0.0});
+    const bool was_timing = u->timing;
+    u->timing = false;
+    gs_tuning win{0, 1, 0.0, 0, 0, 1e300};
+    for (gs_tuning c : cand) {
+        if (gs_set_tuning(u, &c) != GS_OK) continue;
+        u->list_life = 10;
+        if ((rc = gs_step(u, trial_steps / 2, nullptr))) return rc;  // warm up, adapt the lifetime
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, u->stream));
+        if ((rc = gs_step(u, trial_steps, nullptr))) return rc;
+        HIPCHK(hipEventRecord(e1, u->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c.ms_per_tick = ms / trial_steps;
+        c.list_life = u->list_life;
+        if (c.ms_per_tick < win.ms_per_tick) win = c;
+        if ((rc = gs_upload_state(u, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], sp.data(), q.data())))
+            return rc;
+    }
+    u->timing = was_timing;
+    if ((rc = gs_set_tuning(u, &win))) return rc;
+    u->list_builds = 0;
+    if (best) {
+        *best = win;
+        best->list_builds = 0;
     }
     return GS_OK;
 }

@@ -126,6 +126,23 @@ class GpuUniverse:
         return np.ctypeslib.as_array(ctypes.cast(buf, ctypes.POINTER(ctypes.c_double)),
                                      shape=(max(1, n_steps), 6))[:n_steps].copy()
 
+    def tuning(self) -> dict:
+        t = _lib.Tuning()
+        _lib.check(self.lib.gs_get_tuning(self.h, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}
+
+    def set_tuning(self, pair_mode: int, lanes_per_atom: int = 4, skin: Optional[float] = None,
+                   list_life: int = 0) -> None:
+        """pair_mode 0 = per-tick cell tiles, 1 = Verlet list (lanes_per_atom, skin)."""
+        cur = self.tuning()
+        t = _lib.Tuning(pair_mode, lanes_per_atom, cur["skin"] if skin is None else skin, list_life, 0, 0.0)
+        _lib.check(self.lib.gs_set_tuning(self.h, ctypes.byref(t)))
+
+    def autotune(self, trial_steps: int = 40) -> dict:
+        t = _lib.Tuning()
+        _lib.check(self.lib.gs_autotune(self.h, trial_steps, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}
+
     def timing(self, enable: Optional[bool] = None) -> dict:
         """Per-phase event timing of step(); enable=True resets the counters."""
         if enable is not None:

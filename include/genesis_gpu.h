@@ -107,6 +107,21 @@ typedef struct gs_timing {
     int64_t pairs_in_range; /* ordered pairs within the cutoff, last tick    */
 } gs_timing;
 
+/* Pair-path tuning (no reference counterpart: the reference has one CPU
+ * path).  pair_mode 0 rebuilds the cell list every tick and evaluates the
+ * 27-cell LDS tiles directly; pair_mode 1 builds, from the same tiles, a
+ * Verlet list with cutoff rc + skin and reuses it while every atom stays
+ * within skin/2 of its position at the build.  Both give the reference's
+ * pair set. */
+typedef struct gs_tuning {
+    int32_t pair_mode;       /* 0 = per-tick cell tiles, 1 = Verlet list           */
+    int32_t lanes_per_atom;  /* list mode: 1, 2, 4 or 8 lanes share an atom's list */
+    double skin;             /* list mode: list cutoff = rc + skin                 */
+    int64_t list_life;       /* list mode: ticks between builds (adapted)          */
+    int64_t list_builds;     /* list builds so far (read only)                     */
+    double ms_per_tick;      /* gs_autotune: time of the chosen setting            */
+} gs_tuning;
+
 typedef struct gs_universe gs_universe;
 
 /* ---- lifecycle ------------------------------------------------------------ */
@@ -165,6 +180,15 @@ int gs_kinetic(gs_universe* u, double* kinetic);
  * kick, [thermostat].  Fused and resident: nothing crosses to the host but the
  * observables.  obs may be NULL, else it receives n_steps records. */
 int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs);
+
+int gs_get_tuning(gs_universe* u, gs_tuning* out);
+/* Validates and applies; GS_E_INVALID if the setting is impossible for this
+ * box (list mode needs >= 3 cells of edge rc + skin per axis). */
+int gs_set_tuning(gs_universe* u, const gs_tuning* t);
+/* Project autotuner: times trial_steps ticks of every candidate setting on
+ * the resident state, restores the state after each trial and keeps the
+ * fastest.  best (may be NULL) receives it. */
+int gs_autotune(gs_universe* u, int32_t trial_steps, gs_tuning* best);
 
 /* Enable (and reset) or disable per-phase event timing of gs_step. */
 int gs_timing_enable(gs_universe* u, int on);

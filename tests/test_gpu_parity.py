@@ -149,14 +149,16 @@ def test_thermostat_and_kinetic(GU):
     np.testing.assert_allclose(got.vx, ref.vx, rtol=1e-14, atol=0)
 
 
-def test_one_tick_matches_oracle(GU):
+@pytest.mark.parametrize("mode", [0, 1])
+def test_one_tick_matches_oracle(GU, mode):
     # Tick 1 starts with a = 0 (src/universe/mod.rs:272,357): kick, drift and
     # wrap are then bit-exact, forces and the second kick within tolerance.
     w = mixture(n=4096, seed=8, steps=1)
     ph, st = w.physics, w.state
-    say("one tick")
+    say(f"one tick, pair_mode {mode}")
     ref, obs_ref = O.run(ph, st, 1)
     with GU(ph, st) as u:
+        u.set_tuning(mode)
         obs = u.step(1)
         got = u.download()
     for a in ("x", "y", "z"):
@@ -171,14 +173,16 @@ def test_one_tick_matches_oracle(GU):
     assert obs[0, 4] == pytest.approx(obs_ref[0, 4], rel=1e-13)  # lambda
 
 
-def test_short_trajectory_tracks_oracle(GU):
-    # 20 ticks of the plain LJ workload: trajectories stay within round-off
-    # growth of the oracle's (chaos amplifies ~1e-16 slowly over few steps).
-    w = plain_lj(n_side=10, steps=20)
+@pytest.mark.parametrize("mode,lanes", [(0, 1), (1, 1), (1, 4), (1, 8)])
+def test_short_trajectory_tracks_oracle(GU, mode, lanes):
+    # 20 ticks: trajectories stay within round-off growth of the oracle's
+    # (chaos amplifies ~1e-16 slowly over few steps).
+    w = plain_lj(n_side=12, steps=20)
     ph, st = w.physics, w.state
-    say("20 ticks plain LJ n=1000")
+    say(f"20 ticks plain LJ n=1728 pair_mode {mode} lanes {lanes}")
     ref, obs_ref = O.run(ph, st, 20)
     with GU(ph, st) as u:
+        u.set_tuning(mode, lanes)
         obs = u.step(20)
         got = u.download()
     np.testing.assert_allclose(got.x, ref.x, rtol=0, atol=1e-9)
@@ -201,13 +205,15 @@ def test_nve_energy_and_momentum_conservation(GU):
     assert np.linalg.norm(p) < 1e-10 * (m * np.abs(got.vx)).sum()
 
 
-def test_step_is_bitwise_deterministic(GU):
+@pytest.mark.parametrize("mode", [0, 1])
+def test_step_is_bitwise_deterministic(GU, mode):
     w = mixture(n=32768, seed=1002, steps=50)
     ph, st = w.physics, w.state
-    say("determinism 2 x 50 ticks n=32768")
+    say(f"determinism 2 x 50 ticks n=32768 pair_mode {mode}")
     outs = []
     for _ in range(2):
         with GU(ph, st) as u:
+            u.set_tuning(mode)
             obs = u.step(50)
             outs.append((u.download(), obs))
     for a in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"):
@@ -277,3 +283,94 @@ def test_empty_universe(GU):
         assert u.step(3).shape == (3, 6)
         c, g, _, _ = u.radial_distribution(4.0, 10)
         assert c.sum() == 0
+
+
+@pytest.fixture(scope="module")
+def mixture_60_ticks():
+    w = mixture(n=32768, seed=1002, steps=60)
+    ref, obs_ref = O.run(w.physics, w.state, 60, nthreads=8)
+    return w, ref, obs_ref
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8])
+def test_list_mode_matches_oracle_over_a_mixture_trajectory(GU, lanes, mixture_60_ticks):
+    # 60 ticks of the 32k mixture with Coulomb and Berendsen: the list path
+    # (lists rebuilt only when an atom moved > skin/2) against the oracle,
+    # which rebuilds its grid every tick.
+    w, ref, obs_ref = mixture_60_ticks
+    ph, st = w.physics, w.state
+    say(f"list mode lanes {lanes}: 60 ticks n=32768 vs oracle")
+    with GU(ph, st) as u:
+        u.set_tuning(1, lanes)
+        obs = u.step(60)
+        got = u.download()
+        tun = u.tuning()
+    say(f"list builds {tun['list_builds']}, lifetime {tun['list_life']}")
+    assert tun["pair_mode"] == 1 and tun["list_builds"] >= 1
+    np.testing.assert_allclose(obs[:, 1], obs_ref[:, 1], rtol=1e-8)
+    np.testing.assert_allclose(obs[:, 0], obs_ref[:, 0], rtol=1e-8)
+    np.testing.assert_allclose(got.x, ref.x, rtol=0, atol=1e-8)
+
+
+def test_list_mode_forces_after_many_ticks_match_a_fresh_force_pass(GU):
+    # After 300 list-mode ticks the accelerations in the state must equal a
+    # fresh per-tick-grid force pass (oracle) on the same positions.
+    w = mixture(n=8000, seed=31, steps=300)
+    ph, st = w.physics, w.state
+    say("list mode 300 ticks then oracle force pass on the final positions")
+    with GU(ph, st) as u:
+        u.set_tuning(1, 4)
+        u.step(300)
+        got = u.download()
+        tun = u.tuning()
+    say(f"list builds {tun['list_builds']}, lifetime {tun['list_life']}")
+    ref = O.forces(ph, State(got.x, got.y, got.z, got.vx, got.vy, got.vz, st.species, st.charge))
+    m = np.asarray(ph.mass)[st.species]
+    ok, worst = force_tolerance_ok(got.ax, got.ay, got.az, ref, m, REL)
+    say(f"worst conditioned error {worst:.3e}")
+    assert ok, worst
+
+
+def test_list_skin_violation_replays_with_shorter_lifetime(GU):
+    # A hot gas with a tiny skin forces lifetime cuts and replays; the result
+    # must still match the tile path.
+    w = mixture(n=4096, seed=41, temperature=600.0, steps=40, thermostat=False)
+    ph, st = w.physics, w.state
+    say("skin violation replay")
+    with GU(ph, st) as u:
+        u.set_tuning(1, 2, skin=0.02, list_life=50)
+        obs_l = u.step(40)
+        got_l = u.download()
+    with GU(ph, st) as u:
+        u.set_tuning(0)
+        obs_t = u.step(40)
+        got_t = u.download()
+    np.testing.assert_allclose(obs_l[:, 5], obs_t[:, 5], rtol=1e-9)
+    np.testing.assert_allclose(got_l.x, got_t.x, rtol=0, atol=1e-9)
+
+
+def test_autotune_restores_state_and_picks_a_valid_setting(GU):
+    w = mixture(n=32768, seed=1002, steps=1)
+    ph, st = w.physics, w.state
+    say("autotune")
+    with GU(ph, st) as u:
+        before = u.download()
+        best = u.autotune(16)
+        after = u.download()
+        say(f"autotune picked {best}")
+        for a in ("x", "vx"):
+            assert np.array_equal(getattr(before, a), getattr(after, a))
+        assert best["pair_mode"] in (0, 1) and best["ms_per_tick"] > 0
+        obs = u.step(10)
+        assert np.isfinite(obs).all()
+
+
+def test_invalid_tuning_is_rejected(GU):
+    from genesis_amd._lib import GsError
+    w = mixture(n=512, seed=2, steps=1)  # 2 cells per axis: no list mode
+    with GU(w.physics, w.state) as u:
+        assert u.tuning()["pair_mode"] == 0
+        with pytest.raises(GsError):
+            u.set_tuning(1, 4)
+        with pytest.raises(GsError):
+            u.set_tuning(1, 3)
