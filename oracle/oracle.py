@@ -64,6 +64,13 @@ def lib():
             "or_radial_distribution": (ctypes.c_int, [ctypes.c_int64, _D, _D, _D, _U8, _D, ctypes.c_double,
                                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, _U64, _D, _D, _D]),
             "or_clusters": (ctypes.c_int, [ctypes.c_int64, _D, _D, _D, _U8, _D, _D, ctypes.c_double, _U32, _D]),
+            "or_pair_derivs": (None, [ctypes.c_double] * 5 + [ctypes.c_int, _D]),
+            "or_adj_accel": (ctypes.c_double, [P, ctypes.c_int64, _D, _D, _D, _U8, _U8, _D, _D, _D, _D]),
+            "or_adj_pass": (None, [P, ctypes.c_int64, _D, _D, _D, _U8, _U8, _D, _D, _D, _D, ctypes.c_double,
+                                   _D, _D, _D, _D]),
+            "or_adjoint": (ctypes.c_int, [P, ctypes.c_int64] + [_D] * 6 + [_U8, _U8, _D, ctypes.c_int64,
+                                          ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.c_double, _D,
+                                          _D, _D, _D]),
             "or_bond_components": (ctypes.c_int, [ctypes.c_int64, _U32, _U8, ctypes.c_int64, _U32, _U32, _I64, _U32,
                                                   ctypes.c_char_p, ctypes.c_int64]),
         }
@@ -272,3 +279,62 @@ def bond_components(entities, edges):
         out.append({"size": sz, "members": members[off:off + sz].tolist(), "formula": f})
         off += sz
     return out
+
+
+# ---- adjoint reference (oracle/adjoint.cpp) ----------------------------------
+
+PAIR_DERIV_NAMES = ("phi", "d1", "d2", "phi_s", "phi_e", "phi_q", "phi_rc", "f_s", "f_e", "f_q", "f_rc")
+
+
+def pair_derivs(d, sigma, eps, qq, rc, coul=True):
+    out = np.zeros(11)
+    lib().or_pair_derivs(d, sigma, eps, qq, rc, 1 if coul else 0, _p(out))
+    return dict(zip(PAIR_DERIV_NAMES, out.tolist()))
+
+
+def theta_of(physics):
+    return np.array(list(physics.sigma) + list(physics.epsilon_k) + list(physics.charge), dtype=np.float64)
+
+
+def adj_accel(physics, state, keep, theta):
+    n = state.n
+    p = physics.to_c()
+    ax, ay, az = np.zeros(n), np.zeros(n), np.zeros(n)
+    sp = np.ascontiguousarray(state.species, dtype=np.uint8)
+    kp = np.ascontiguousarray(keep, dtype=np.uint8)
+    th = _d(theta)
+    V = lib().or_adj_accel(ctypes.byref(p), n, _p(_d(state.x)), _p(_d(state.y)), _p(_d(state.z)),
+                           _p(sp, ctypes.c_uint8), _p(kp, ctypes.c_uint8), _p(th), _p(ax), _p(ay), _p(az))
+    return V, np.stack([ax, ay, az], 1)
+
+
+def adj_pass(physics, state, keep, theta, lam, vcoef=0.0):
+    n = state.n
+    p = physics.to_c()
+    sp = np.ascontiguousarray(state.species, dtype=np.uint8)
+    kp = np.ascontiguousarray(keep, dtype=np.uint8)
+    lx, ly, lz = (np.ascontiguousarray(lam[:, k]) for k in range(3))
+    ox, oy, oz, g = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(30)
+    lib().or_adj_pass(ctypes.byref(p), n, _p(_d(state.x)), _p(_d(state.y)), _p(_d(state.z)),
+                      _p(sp, ctypes.c_uint8), _p(kp, ctypes.c_uint8), _p(_d(theta)), _p(lx), _p(ly), _p(lz), vcoef,
+                      _p(ox), _p(oy), _p(oz), _p(g))
+    return np.stack([ox, oy, oz], 1), g
+
+
+def adjoint(physics, state, keep, theta, nsteps, win_from, gr=None, g0=None, grad=True):
+    """Loss (mean V over ticks >= win_from + smoothed-g(r) deviation) and its
+    theta gradient.  gr = (nbins, r_max, sigma_b)."""
+    n = state.n
+    p = physics.to_c()
+    sp = np.ascontiguousarray(state.species, dtype=np.uint8)
+    kp = np.ascontiguousarray(keep, dtype=np.uint8)
+    nb, rmax, sb = gr if gr else (0, 0.0, 0.0)
+    loss = ctypes.c_double()
+    gout = np.zeros(30) if grad else None
+    g_final = np.zeros(max(nb, 1))
+    g0a = None if g0 is None else _d(g0)
+    lib().or_adjoint(ctypes.byref(p), n, *[_p(_d(a)) for a in (state.x, state.y, state.z, state.vx, state.vy,
+                                                                 state.vz)],
+                     _p(sp, ctypes.c_uint8), _p(kp, ctypes.c_uint8), _p(_d(theta)), nsteps, win_from, nb, rmax, sb,
+                     _p(g0a), ctypes.byref(loss), _p(gout), _p(g_final))
+    return loss.value, gout, g_final[:nb]
