@@ -54,6 +54,11 @@ class Tuning(ctypes.Structure):
                 ("list_life", ctypes.c_int64), ("list_builds", ctypes.c_int64), ("ms_per_tick", ctypes.c_double)]
 
 
+class GrSpec(ctypes.Structure):
+    _fields_ = [("nbins", ctypes.c_int32), ("pad", ctypes.c_int32), ("r_max", ctypes.c_double),
+                ("sigma_b", ctypes.c_double)]
+
+
 _D = ctypes.POINTER(ctypes.c_double)
 _U8 = ctypes.POINTER(ctypes.c_uint8)
 _U32 = ctypes.POINTER(ctypes.c_uint32)
@@ -83,6 +88,10 @@ _SIGNATURES = {
     "gs_get_tuning": (ctypes.c_int, [_H, ctypes.POINTER(Tuning)]),
     "gs_set_tuning": (ctypes.c_int, [_H, ctypes.POINTER(Tuning)]),
     "gs_autotune": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.POINTER(Tuning)]),
+    "gs_hvp": (ctypes.c_int, [_H, _D, _U8, _D, _D, _D, ctypes.c_double, _D, _D, _D, _D]),
+    "gs_adjoint_forward": (ctypes.c_int, [_H, _D, _U8, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                          ctypes.POINTER(GrSpec), _D, _D, _D]),
+    "gs_adjoint_backward": (ctypes.c_int, [_H, _D]),
     "gs_timing_enable": (ctypes.c_int, [_H, ctypes.c_int]),
     "gs_timing_read": (ctypes.c_int, [_H, ctypes.POINTER(Timing)]),
     "gs_clusters": (ctypes.c_int, [_H, ctypes.c_double, _U32, ctypes.POINTER(ClusterStats)]),

@@ -1199,6 +1199,10 @@ __global__ __launch_bounds__(BLOCK) void k_fof_stats(int n, const uint32_t* __re
 // Host side
 // =============================================================================
 
+struct gs_universe;
+struct AdjState;
+static void adj_free(gs_universe* u);
+
 struct gs_universe {
     gs_params prm;
     KParams kp;
@@ -1267,6 +1271,21 @@ struct gs_universe {
     double* snap_a[3] = {nullptr, nullptr, nullptr};
     uint8_t* snap_type = nullptr;
     uint32_t* snap_id = nullptr;
+    // differentiable trajectory (gs_adjoint.inc)
+    AdjState* adj = nullptr;
+    uint8_t* adj_keep = nullptr;
+    AdjCoef* adj_coef = nullptr;
+    double* adj_qs = nullptr;
+    double* adj_acc = nullptr;
+    double* adj_gatom = nullptr;
+    double* adj_rows = nullptr;
+    size_t adj_rows_cap = 0;
+    double* adj_ck = nullptr;
+    double* adj_g = nullptr;
+    double* adj_l[3] = {nullptr, nullptr, nullptr};
+    double* adj_o[3] = {nullptr, nullptr, nullptr};
+    double* adj_lv[3] = {nullptr, nullptr, nullptr};
+    double* adj_x[3] = {nullptr, nullptr, nullptr};
     // event timing of gs_step
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -1711,6 +1730,7 @@ int gs_destroy(gs_universe* u) {
     for (int a = 0; a < 3; ++a) (void)hipFree(u->snap_a[a]);
     (void)hipFree(u->snap_type);
     (void)hipFree(u->snap_id);
+    adj_free(u);
     if (u->stream) (void)hipStreamDestroy(u->stream);
     delete u;
     return GS_OK;
@@ -2314,3 +2334,5 @@ int gs_clusters(gs_universe* u, double bond_factor, uint32_t* labels, gs_cluster
 }
 
 }  // extern "C"
+
+#include "gs_adjoint.inc"

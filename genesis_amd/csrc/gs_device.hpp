@@ -83,6 +83,14 @@ struct PairCoef {
     double bond_rc;    // bond_k_bind * sigma_ij
 };
 
+// Per-species-pair parameters of the differentiable model (adjoint).
+struct AdjCoef {
+    double sigma;            // sigma_ij
+    double eps;              // eps_ij = k sqrt(epsK_i epsK_j)
+    double eps_over_epsK_i;  // eps_ij / epsK_i
+    double pad;
+};
+
 // Scalar parameters every kernel reads (passed by value).
 struct KParams {
     double box[3];

@@ -151,6 +151,42 @@ class GpuUniverse:
         _lib.check(self.lib.gs_timing_read(self.h, ctypes.byref(t)))
         return {f: getattr(t, f) for f, _ in _lib.Timing._fields_}
 
+    # -- differentiable trajectory ---------------------------------------------------
+    def hvp(self, theta, keep, lam, vcoef: float = 0.0):
+        """(da/dx)^T lam - vcoef dV/dx and (da/dtheta)^T lam + vcoef dV/dtheta."""
+        th = _f64(theta)
+        kp = np.ascontiguousarray(keep, dtype=np.uint8)
+        lx, ly, lz = (_f64(lam[:, k]) for k in range(3))
+        ox, oy, oz, g = np.empty(self.n), np.empty(self.n), np.empty(self.n), np.empty(30)
+        _lib.check(self.lib.gs_hvp(self.h, _dp(th), kp.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _dp(lx),
+                                   _dp(ly), _dp(lz), vcoef, _dp(ox), _dp(oy), _dp(oz), _dp(g)))
+        return np.stack([ox, oy, oz], 1), g
+
+    def adjoint_forward(self, theta, keep, n_steps: int, ckpt_every: int, win_from: int, gr=None, g0=None):
+        """Loss of an n_steps trajectory (mean V over ticks >= win_from + smoothed
+        g(r) deviation); gr = (nbins, r_max, sigma_b).  Returns (loss, g(x_T))."""
+        th = _f64(theta)
+        kp = np.ascontiguousarray(keep, dtype=np.uint8)
+        self._adj_keep = (th, kp)
+        spec = None
+        nb = 0
+        if gr is not None:
+            nb = int(gr[0])
+            spec = _lib.GrSpec(nb, 0, float(gr[1]), float(gr[2]))
+        g0a = None if g0 is None else _f64(g0)
+        gout = np.zeros(max(nb, 1))
+        loss = ctypes.c_double()
+        _lib.check(self.lib.gs_adjoint_forward(self.h, _dp(th), kp.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                               n_steps, ckpt_every, win_from,
+                                               None if spec is None else ctypes.byref(spec), _dp(g0a),
+                                               ctypes.byref(loss), _dp(gout) if nb else None))
+        return loss.value, gout[:nb]
+
+    def adjoint_backward(self) -> np.ndarray:
+        g = np.empty(30)
+        _lib.check(self.lib.gs_adjoint_backward(self.h, _dp(g)))
+        return g
+
     # -- lenses --------------------------------------------------------------------
     def radial_distribution(self, r_max: float, nbins: int):
         counts = np.zeros(max(1, min(nbins, 1 << 16)), dtype=np.uint64)

@@ -214,6 +214,38 @@ int gs_radial_distribution_between(gs_universe* u, int32_t species_a, int32_t sp
 int gs_clusters(gs_universe* u, double bond_factor, uint32_t* labels,
                 gs_cluster_stats* stats);
 
+/* ---- differentiable trajectory (no reference counterpart) -------------------
+ * theta[30] = per-species sigma[10], epsilon_k[10], charge[10]; keep[n] (u8,
+ * slot order) multiplies each atom's charge (0 = neutralised atom).  These
+ * calls replace the handle's sigma/epsilon/charge by theta's, switch the
+ * thermostat off and use the per-tick cell-tile path (deterministic, so a
+ * recomputed segment reproduces the forward bit for bit).  LJ + Coulomb with
+ * the Coulomb cutoff equal to the LJ cutoff; gravity and bonds off. */
+typedef struct gs_gr_spec {
+    int32_t nbins;    /* Gaussian-smoothed g(r): bins on [0, r_max)  */
+    int32_t pad;
+    double r_max;
+    double sigma_b;   /* Gaussian width (same length unit)          */
+} gs_gr_spec;
+
+/* One adjoint pass at the resident positions: with u = lam / m,
+ * out = (da/dx)^T lam - vcoef dV/dx  (a Hessian-vector product) and
+ * dtheta = (da/dtheta)^T lam + vcoef dV/dtheta.  lam, out in slot order. */
+int gs_hvp(gs_universe* u, const double* theta, const uint8_t* keep,
+           const double* lx, const double* ly, const double* lz, double vcoef,
+           double* ox, double* oy, double* oz, double* dtheta);
+/* Forward op: n_steps velocity-Verlet ticks (NVE, a_0 = a(x_0)) from the
+ * resident state, checkpoints every ckpt_every ticks, loss = mean V over
+ * ticks [win_from, n_steps] + sum_k (g_k(x_T) - g0_k)^2 (gr and g0 may be
+ * NULL).  g_final (nbins, may be NULL) gets g(x_T).  Leaves the state at x_T. */
+int gs_adjoint_forward(gs_universe* u, const double* theta, const uint8_t* keep,
+                       int64_t n_steps, int64_t ckpt_every, int64_t win_from,
+                       const gs_gr_spec* gr, const double* g0,
+                       double* loss, double* g_final);
+/* Backward op: dloss/dtheta (30) of the last forward, by the discrete adjoint
+ * with recomputation from the checkpoints. */
+int gs_adjoint_backward(gs_universe* u, double* dloss_dtheta);
+
 #ifdef __cplusplus
 }
 #endif
