@@ -52,6 +52,7 @@ def test_gpu_matches_fixture(path):
         if "gr_counts" in d:
             c, g, _, _ = u.radial_distribution(float(d["gr_rmax"]), len(d["gr_counts"]))
             lab, cs = u.clusters(1.5)
+        u.upload(st)  # the fixture trajectory starts from a = 0, like tick 1 of the reference
         obs = u.step(int(d["nsteps"]))
         fin = u.download()
     m = np.asarray(ph.mass)[st.species]
