@@ -48,6 +48,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--pair-mode", type=int, default=-1, help="-1 autotune, 0 cell tiles, 1 Verlet list")
     ap.add_argument("--lanes", type=int, default=4)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-replica and adjoint legs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -139,11 +141,77 @@ def main() -> int:
         "list_builds_in_timed_region": u.tuning()["list_builds"] - builds0,
     }
     u.close()
+    if not args.no_extras:
+        out["batched"] = batched_leg()
+        out["adjoint"] = adjoint_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
     return 0
+
+
+def batched_leg(ticks: int = 200, segment: int = 100) -> dict:
+    """Temperature sweep: 64 replicas of log-uniform sizes in [512, 16384]
+    (20 K .. 120 K), stepped (a) as one batch (one launch per phase, hipGraph
+    segments of 100 ticks, autotuned cell size) and (b) one universe after
+    another through gs_step (the per-replica-launch plain path)."""
+    from genesis_amd.batch import GpuBatch
+    from genesis_amd.inputs import replica_sweep
+    from genesis_amd.universe import GpuUniverse
+    ws = replica_sweep()
+    total = sum(w.state.n for w in ws)
+    with GpuBatch([w.physics for w in ws], [w.state for w in ws]) as b:
+        tune = b.autotune(20)
+        b.step(segment, graph_segment=segment)
+        t0 = time.perf_counter()
+        b.step(ticks, graph_segment=segment)
+        tb = time.perf_counter() - t0
+    us = [GpuUniverse(w.physics, w.state) for w in ws]
+    for u in us:
+        u.step(20, observe=False)
+    t0 = time.perf_counter()
+    for u in us:
+        u.step(ticks, observe=False)
+    tp = time.perf_counter() - t0
+    for u in us:
+        u.close()
+    vb, vp = total * ticks / tb, total * ticks / tp
+    log(f"batched sweep: {vb:.4e} atom-steps/s vs per-replica {vp:.4e} ({vb / vp:.2f}x), {total} atoms")
+    return {"workload": f"temperature sweep, 64 replicas, {total} atoms, sizes log-uniform in [512, 16384]",
+            "batched_atom_steps_per_s": vb, "per_replica_atom_steps_per_s": vp, "speedup": vb / vp,
+            "ticks": ticks, "graph_segment": segment, "autotuned_cell_scale": tune["cell_scale"]}
+
+
+def adjoint_leg(steps: int = 1000, ckpt: int = 50) -> dict:
+    """Adjoint workload: 8,192 atoms of the mixture, NVE, dt = 0.5 fs, 1000
+    ticks, checkpoints every 50; loss = mean V over the last 200 ticks +
+    smoothed g(r) deviation; gradient w.r.t. the 30 species parameters.
+    Reports the reverse pass per tick relative to the forward tick."""
+    import numpy as np
+    from genesis_amd.adjoint import theta_of
+    from genesis_amd.inputs import adjoint_mixture
+    from genesis_amd.universe import GpuUniverse
+    w = adjoint_mixture()
+    ph, st = w.physics, w.state
+    keep = ((np.abs(st.charge) > 0) | (np.asarray(ph.charge)[st.species] == 0)).astype(np.uint8)
+    th0 = theta_of(ph).numpy()
+    gr = (200, ph.box[0] / 4.0, 0.05)
+    with GpuUniverse(ph, st) as u:
+        _, g0 = u.adjoint_forward(th0, keep, steps, ckpt, steps - 199, gr=gr)
+        th = th0 * (1.0 + 0.005 * np.cos(np.arange(30)))
+        u.upload(st)
+        t0 = time.perf_counter()
+        loss, _ = u.adjoint_forward(th, keep, steps, ckpt, steps - 199, gr=gr, g0=g0)
+        tf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        grad = u.adjoint_backward()
+        tb = time.perf_counter() - t0
+    log(f"adjoint: forward {1e3 * tf / steps:.4f} ms/tick, reverse {1e3 * tb / steps:.4f} ms/tick "
+        f"({tb / tf:.2f}x)")
+    return {"workload": f"adjoint mixture, {st.n} atoms, {steps} ticks, checkpoint every {ckpt}",
+            "forward_ms_per_tick": 1e3 * tf / steps, "reverse_ms_per_tick": 1e3 * tb / steps,
+            "reverse_over_forward": tb / tf, "loss": loss, "grad_norm": float(np.linalg.norm(grad))}
 
 
 def copy_probe() -> int:

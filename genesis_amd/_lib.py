@@ -92,6 +92,13 @@ _SIGNATURES = {
     "gs_adjoint_forward": (ctypes.c_int, [_H, _D, _U8, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                           ctypes.POINTER(GrSpec), _D, _D, _D]),
     "gs_adjoint_backward": (ctypes.c_int, [_H, _D]),
+    "gs_batch_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(GsParams), ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.POINTER(_H)]),
+    "gs_batch_destroy": (ctypes.c_int, [_H]),
+    "gs_batch_upload": (ctypes.c_int, [_H, ctypes.c_int32] + [_D] * 9 + [_U8, _D]),
+    "gs_batch_download": (ctypes.c_int, [_H, ctypes.c_int32] + [_D] * 9),
+    "gs_batch_step": (ctypes.c_int, [_H, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(Observables)]),
+    "gs_batch_autotune": (ctypes.c_int, [_H, ctypes.c_int32, _D, _D]),
     "gs_timing_enable": (ctypes.c_int, [_H, ctypes.c_int]),
     "gs_timing_read": (ctypes.c_int, [_H, ctypes.POINTER(Timing)]),
     "gs_clusters": (ctypes.c_int, [_H, ctypes.c_double, _U32, ctypes.POINTER(ClusterStats)]),

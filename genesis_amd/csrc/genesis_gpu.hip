@@ -288,14 +288,23 @@ __device__ __forceinline__ void pair_law(const KParams& P, const PairCoef& pc, d
 // own pairs, so the ordered merge of :330-341 disappears).  Requires >= 3
 // cells on every axis so the 27 stencil cells are distinct and each pair is
 // seen once per end; smaller boxes use k_pair_brute.
-template <bool COUL, bool SHARED, bool EXTRAS, bool SINGLE>
-__global__ __launch_bounds__(PAIR_THREADS) void k_pair(KParams P, const uint32_t* __restrict__ cstart,
+//
+// BATCH: many independent universes share the launch.  Their cell grids are
+// concatenated into one global cell index space (replica r owns cells
+// [cell_off[r], cell_off[r+1])); a workgroup reads its replica's parameters
+// from Ptab and skips cells of replicas marked small (those use the
+// brute-force batch kernel).
+template <bool COUL, bool SHARED, bool EXTRAS, bool SINGLE, bool BATCH = false>
+__global__ __launch_bounds__(PAIR_THREADS) void k_pair(KParams Pk, const uint32_t* __restrict__ cstart,
                                                        const double4* __restrict__ pos4,
                                                        const uint8_t* __restrict__ type,
                                                        const PairCoef* __restrict__ coef, double* __restrict__ ax,
                                                        double* __restrict__ ay, double* __restrict__ az,
                                                        double* __restrict__ pe_atom, double* __restrict__ s_abs,
-                                                       double* __restrict__ vpart) {
+                                                       double* __restrict__ vpart,
+                                                       const KParams* __restrict__ Ptab = nullptr,
+                                                       const uint16_t* __restrict__ cell_rep = nullptr,
+                                                       const uint32_t* __restrict__ cell_off = nullptr) {
     __shared__ double4 tile[TILE_CAP];
     __shared__ uint8_t ttype[TILE_CAP];
     __shared__ PairCoef sc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];
@@ -305,11 +314,15 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_pair(KParams P, const uint32_t
     __shared__ uint16_t queue[PAIR_WAVES][QCAP];
     __shared__ double wv[PAIR_WAVES];
 
-    const uint32_t c = blockIdx.x;
+    const uint32_t cg = blockIdx.x;  // global cell index
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t h0 = cstart[c], nh = cstart[c + 1] - h0;
-    if (nh == 0) {
-        if (tid == 0) vpart[c] = 0.0;
+    const uint32_t rep = BATCH ? cell_rep[cg] : 0u;
+    const KParams& P = BATCH ? Ptab[rep] : Pk;
+    const uint32_t cbase = BATCH ? cell_off[rep] : 0u;
+    const uint32_t c = cg - cbase;  // cell index within the universe
+    const uint32_t h0 = cstart[cg], nh = cstart[cg + 1] - h0;
+    if (nh == 0 || (BATCH && (P.flags & F_SMALL))) {
+        if (tid == 0) vpart[cg] = 0.0;
         return;
     }
     if (SINGLE) {
@@ -327,7 +340,7 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_pair(KParams P, const uint32_t
         if (ix < 0) { ix += nx; sx = -P.box[0]; } else if (ix >= (int)nx) { ix -= nx; sx = P.box[0]; }
         if (iy < 0) { iy += ny; sy = -P.box[1]; } else if (iy >= (int)ny) { iy -= ny; sy = P.box[1]; }
         if (iz < 0) { iz += nz; sz = -P.box[2]; } else if (iz >= (int)nz) { iz -= nz; sz = P.box[2]; }
-        const uint32_t nc = (uint32_t)ix + nx * ((uint32_t)iy + ny * (uint32_t)iz);
+        const uint32_t nc = cbase + (uint32_t)ix + nx * ((uint32_t)iy + ny * (uint32_t)iz);
         const uint32_t b = cstart[nc];
         seg_base[tid] = b;
         seg_pref[tid + 1] = cstart[nc + 1] - b;
@@ -458,7 +471,7 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_pair(KParams P, const uint32_t
     vlane = wave_sum(vlane);
     if (lane == 0) wv[w] = vlane;
     __syncthreads();
-    if (tid == 0) vpart[c] = 0.5 * ((wv[0] + wv[1]) + (wv[2] + wv[3]));
+    if (tid == 0) vpart[cg] = 0.5 * ((wv[0] + wv[1]) + (wv[2] + wv[3]));
 }
 
 // Small-box pair kernel (any axis with < 3 cells): every ordered pair, the
@@ -2336,3 +2349,4 @@ int gs_clusters(gs_universe* u, double bond_factor, uint32_t* labels, gs_cluster
 }  // extern "C"
 
 #include "gs_adjoint.inc"
+#include "gs_batch.inc"

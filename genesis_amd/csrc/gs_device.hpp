@@ -116,6 +116,7 @@ enum : int32_t {
     F_WRAP = 8,
     F_SHARED_SWITCH = 16,  // Coulomb switch identical to the LJ one
     F_REF_MULT = 32,       // reference multiplicity on 2-cell axes
+    F_SMALL = 64,          // < 3 cells on some axis (brute-force pairs)
 };
 
 }  // namespace gs

@@ -19,8 +19,8 @@ from typing import List, Optional
 
 import numpy as np
 
-from .params import (DEFAULT_CHARGE, DEFAULT_MASS, DEFAULT_SIGMA, NSPECIES, SPECIES_INDEX, Physics,
-                     fs_to_time)
+from .params import (DEFAULT_CHARGE, DEFAULT_MASS, DEFAULT_SIGMA, GRID_MINIMAGE, NSPECIES, SPECIES_INDEX,
+                     Physics, fs_to_time)
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 _M1 = np.uint64(0xBF58476D1CE4E5B9)
@@ -242,13 +242,17 @@ def mixture(seed: int = 1002, n: int = 32768, temperature: float = 60.0, steps: 
 
 def replica_sweep(seed: int = 3000, n_rep: int = 64, lo: int = 512, hi: int = 16384, steps: int = 2000):
     """Batched sweep: replica r has N_r = round(exp(U[ln lo, ln hi])) atoms of
-    the 10-element mix at the mixture density and T_r = 20 + 100 r/(n_rep-1) K."""
+    the 10-element mix at the mixture density and T_r = 20 + 100 r/(n_rep-1) K
+    (Berendsen tau = 100 fs, dt = 0.5 fs).  Replicas too small for 3 cells per
+    axis count every pair once (min-image semantics), not with the
+    reference grid's 2-cell multiplicity."""
     u = Stream(seed, 99).uniform(n_rep)
     sizes = np.rint(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(int)
     out = []
     for r in range(n_rep):
         t = 20.0 + 100.0 * r / max(n_rep - 1, 1)
         w = mixture(seed=seed + r, n=int(sizes[r]), temperature=t, steps=steps)
+        w.physics.grid_semantics = GRID_MINIMAGE
         out.append(w)
     return out
 

@@ -246,6 +246,35 @@ int gs_adjoint_forward(gs_universe* u, const double* theta, const uint8_t* keep,
  * with recomputation from the checkpoints. */
 int gs_adjoint_backward(gs_universe* u, double* dloss_dtheta);
 
+/* ---- batched replicas (no reference counterpart; replaces the sequential
+ * Universe-per-temperature loop of examples/temperature_sweep.rs:70-113) -----
+ * R independent universes of uneven sizes stepped together: one launch per
+ * phase for all replicas.  params[r] and n_atoms[r] describe replica r; all
+ * replicas share the element table and the enabled laws, and may differ in
+ * box, dt and thermostat settings.  Replicas with < 3 cells per axis use a
+ * brute-force pair kernel (grid_semantics as in gs_params). */
+typedef struct gs_batch gs_batch;
+int gs_batch_create(int32_t n_replicas, const gs_params* params, const int64_t* n_atoms, gs_batch** out);
+int gs_batch_destroy(gs_batch* b);
+/* State of replica r (slot order within the replica); every replica must be
+ * uploaded once before gs_batch_step. */
+int gs_batch_upload(gs_batch* b, int32_t replica,
+                    const double* x, const double* y, const double* z,
+                    const double* vx, const double* vy, const double* vz,
+                    const double* ax, const double* ay, const double* az,
+                    const uint8_t* species, const double* charge);
+int gs_batch_download(gs_batch* b, int32_t replica,
+                      double* x, double* y, double* z,
+                      double* vx, double* vy, double* vz,
+                      double* ax, double* ay, double* az);
+/* n_steps ticks of every replica; graph_segment > 0 captures that many ticks
+ * (rounded up to even) in a hipGraph once and replays it.  obs (may be NULL)
+ * receives n_steps * n_replicas records, tick-major. */
+int gs_batch_step(gs_batch* b, int64_t n_steps, int32_t graph_segment, gs_observables* obs);
+/* Project autotuner of the batched path (cell edge = s * rc, s in
+ * {1, 1.15, 1.3}); keeps the fastest.  Outputs may be NULL. */
+int gs_batch_autotune(gs_batch* b, int32_t trial_steps, double* best_scale, double* best_ms_per_tick);
+
 #ifdef __cplusplus
 }
 #endif
