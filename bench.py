@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# Algorithmic work, per atom-step (definitions in DESIGN.md):
+# Algorithmic work per atom-step (compulsory HBM bytes of one tick, and the pair law's flops):
 STEP_BYTES = 161.0          # x, v, a read+write (144) + species (1) + cell key (8) + charge (8)
 PAIR_BYTES = 57.0           # pair kernel: x, y, z, q, species read (33) + a written (24)
 FLOP_PER_PAIR = 82.0        # LJ + quintic switch + accumulate (70) + Coulomb (12), per ordered pair
@@ -47,7 +47,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--pair-mode", type=int, default=-1, help="-1 autotune, 0 cell tiles, 1 Verlet list")
-    ap.add_argument("--lanes", type=int, default=4)
+    ap.add_argument("--lanes", type=int, default=8)
+    ap.add_argument("--skin", type=float, default=None, help="list skin (length units) with --pair-mode 1")
+    ap.add_argument("--sort-every", type=int, default=0, help="list mode: ticks between full re-sorts")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-replica and adjoint legs")
     ap.add_argument("--copy-probe", action="store_true",
@@ -56,8 +58,8 @@ def main() -> int:
     if args.copy_probe:
         return copy_probe()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only: the path shards by spatial domain "
-                                   "(see DESIGN.md) but no multi-GPU figure is produced; run with --gpus 1"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU only: the path would shard by spatial "
+                                   "domain with halo exchange, but no multi-GPU figure is produced; run with --gpus 1"}))
         return 2
 
     from genesis_amd.inputs import mixture
@@ -75,7 +77,7 @@ def main() -> int:
         tuning = u.autotune(40)
         log(f"autotuned: {tuning}")
     else:
-        u.set_tuning(args.pair_mode, args.lanes)
+        u.set_tuning(args.pair_mode, args.lanes, skin=args.skin, sort_every=args.sort_every)
         tuning = u.tuning()
 
     def run(k, observe=False):
@@ -88,17 +90,28 @@ def main() -> int:
     if args.warmup:
         run(args.warmup)
     builds0 = u.tuning()["list_builds"]
-    u.timing(enable=True)
+    # timed region: two HIP events per tick on the universe's stream bracket
+    # the pair kernel (its average duration feeds the roofline)
+    u.timing(enable="pair")
     t0 = time.perf_counter()
     run(args.steps)
     t1 = time.perf_counter()
     tm = u.timing()
+    builds = u.tuning()["list_builds"] - builds0
     elapsed = t1 - t0
     value = n * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
-    log(f"{value:.4e} atom-steps/s, {ms_per_step:.4f} ms/step; kernel phases per step: "
-        f"rebuild {tm['rebuild_ms'] / tm['steps']:.4f} ms, pair {tm['pair_ms'] / tm['steps']:.4f} ms, "
-        f"kick {tm['kick_ms'] / tm['steps']:.4f} ms")
+    # phase breakdown from a separate, untimed pass with four events per tick
+    u.timing(enable="phases")
+    run(min(args.steps, 500))
+    ph_tm = u.timing()
+    u.timing(enable=False)
+    phases = {k: ph_tm[k] / max(ph_tm["steps"], 1) for k in ("rebuild_ms", "pair_ms", "kick_ms")}
+    log(f"{value:.4e} atom-steps/s, {ms_per_step:.4f} ms/step; pair kernel {tm['pair_ms'] / tm['steps']:.4f} ms; "
+        f"phases per step: rebuild {phases['rebuild_ms']:.4f} ms, pair {phases['pair_ms']:.4f} ms, "
+        f"kick {phases['kick_ms']:.4f} ms; list builds {builds}")
+    kernel = f"k_pair_list<{tuning['lanes_per_atom']} lanes> (pair forces + second kick + Berendsen)" \
+        if tuning["pair_mode"] == 1 else "k_pair"
 
     # pair count of the final state (ordered pairs within rc)
     fin = u.download()
@@ -127,18 +140,18 @@ def main() -> int:
                    "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "pipe": "fp64 VALU (pair forces are not a dense contraction; the MI355X "
                                               "FP64 vector peak equals its FP64 matrix peak)",
-                     "kernel": "k_pair", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "kernel": kernel, "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": tflops / FP64_PEAK_TFLOPS, "traffic": None,
                      "per_launch": {"ordered_pairs": npairs, "flop_per_pair": FLOP_PER_PAIR,
                                     "avg_ms": pair_ms}},
-        "roofline_hbm": {"bound": "hbm", "kernel": "k_pair", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline_hbm": {"bound": "hbm", "kernel": kernel, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS, "bytes_per_atom": PAIR_BYTES},
         "step_roofline": {"hbm_frac": value * STEP_BYTES / (HBM_PEAK_GBS * 1e9),
                           "fp64_frac": value * (FLOP_PER_PAIR * npairs / n) / (FP64_PEAK_TFLOPS * 1e12),
                           "bytes_per_atom_step": STEP_BYTES, "flop_per_atom_step": FLOP_PER_PAIR * npairs / n},
-        "phases_ms_per_step": {k: tm[k] / max(tm["steps"], 1) for k in ("rebuild_ms", "pair_ms", "kick_ms")},
-        "tuning": {k: tuning[k] for k in ("pair_mode", "lanes_per_atom", "skin", "ms_per_tick")},
-        "list_builds_in_timed_region": u.tuning()["list_builds"] - builds0,
+        "phases_ms_per_step": phases,
+        "tuning": {k: tuning[k] for k in ("pair_mode", "lanes_per_atom", "skin", "sort_every", "ms_per_tick")},
+        "list_builds_in_timed_region": builds,
     }
     u.close()
     if not args.no_extras:

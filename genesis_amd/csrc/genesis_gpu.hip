@@ -566,14 +566,54 @@ __global__ __launch_bounds__(BLOCK) void k_pair_brute(KParams P, const double4* 
 // Deterministic block sum: fixed per-thread strides then a fixed tree.
 __device__ double block_sum_fixed(const double* __restrict__ a, uint32_t n, double* red) {
     const int tid = threadIdx.x;
-    double s = 0.0;
-    for (uint32_t k = tid; k < n; k += BLOCK) s += a[k];
+    // four independent strided accumulators keep four loads in flight
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    uint32_t k = tid;
+    for (; k + 3 * BLOCK < n; k += 4 * BLOCK) {
+        s0 += a[k];
+        s1 += a[k + BLOCK];
+        s2 += a[k + 2 * BLOCK];
+        s3 += a[k + 3 * BLOCK];
+    }
+    for (; k < n; k += BLOCK) s0 += a[k];
+    double s = (s0 + s1) + (s2 + s3);
     s = wave_sum(s);
     if ((tid & 63) == 0) red[tid >> 6] = s;
     __syncthreads();
     double t = (red[0] + red[1]) + (red[2] + red[3]);
     __syncthreads();
     return t;
+}
+
+// Berendsen lambda from the kinetic energy after the second kick
+// (src/systems/thermostat.rs:47-71, lambda = sqrt(max(0, 1 + (T0/T - 1)/tau)),
+// tau in ticks) and the tick's observables (src/systems/aggregate.rs:86-121,
+// kinetic energy after the rescale).  The caller applies lambda at the start
+// of the next tick or with k_apply_lambda at the end of a call.
+struct ThermoArgs {
+    int on;
+    double target, tau, kconst;
+};
+
+__device__ void close_tick(double K, double V, int n, const ThermoArgs& th, double* lam_out, gs_observables* obs) {
+    double lam = 1.0;
+    const double T = (2.0 / 3.0) * (K / (double)n) / th.kconst;
+    if (th.on && th.target > 0.0 && n > 0) {
+        if (isfinite(T) && T > 0.0) {
+            const double l2 = 1.0 + (1.0 / th.tau) * (th.target / T - 1.0);
+            lam = sqrt(fmax(l2, 0.0));
+        }
+    }
+    *lam_out = lam;
+    if (obs) {
+        const double Ka = lam * lam * K;
+        obs->kinetic = Ka;
+        obs->potential = V;
+        obs->bond_energy = 0.0;
+        obs->temperature = th.kconst > 0.0 ? (2.0 / 3.0) * (Ka / (double)n) / th.kconst : 0.0;
+        obs->lambda = lam;
+        obs->energy_total = Ka + V;
+    }
 }
 
 // Second half kick (src/systems/integrate.rs:38-44) fused with the kinetic
@@ -622,24 +662,7 @@ __global__ __launch_bounds__(BLOCK) void k_kick_ke(KParams P, double* __restrict
     const double K = block_sum_fixed(kpart, gridDim.x, red);
     const double V = vpart ? block_sum_fixed(vpart, nvpart, red) : 0.0;
     if (tid == 0) {
-        double lam = 1.0;
-        const double T = (2.0 / 3.0) * (K / (double)P.n) / kconst;
-        if (thermostat && target > 0.0 && P.n > 0) {
-            if (isfinite(T) && T > 0.0) {
-                const double l2 = 1.0 + (1.0 / tau) * (target / T - 1.0);
-                lam = sqrt(fmax(l2, 0.0));
-            }
-        }
-        *lam_out = lam;
-        if (obs) {
-            const double Ka = lam * lam * K;
-            obs->kinetic = Ka;
-            obs->potential = V;
-            obs->bond_energy = 0.0;
-            obs->temperature = kconst > 0.0 ? (2.0 / 3.0) * (Ka / (double)P.n) / kconst : 0.0;
-            obs->lambda = lam;
-            obs->energy_total = Ka + V;
-        }
+        close_tick(K, V, P.n, ThermoArgs{thermostat, target, tau, kconst}, lam_out, obs);
         *counter = 0u;
     }
 }
@@ -733,14 +756,26 @@ __global__ __launch_bounds__(BLOCK) void k_cells_entity(KParams P, const uint32_
 
 // ---- Verlet-list path ---------------------------------------------------------
 //
-// The cell list is rebuilt on a grid of edge >= rc + skin and, from the same
-// LDS-tiled 27-cell stencil as k_pair, every atom gets the list of atoms
-// within rc + skin.  The lists are reused while no atom has moved more than
-// skin/2 from its position at the build (checked every tick), so every pair
-// within rc is in some list: the pair set, and hence the forces, are the
-// reference's; only the evaluation order differs.
+// Every atom carries the list of atoms within rc + skin, built from the
+// 27-cell stencil of a grid of edge >= rc + skin.  A list stays valid while
+// no atom has moved more than skin/2 from its position at the build, so
+// every pair within rc is in some list: the pair set, and hence the forces,
+// are the reference's; only the evaluation order differs.
+//
+// The device decides, tick by tick, whether the lists must be rebuilt: the
+// integrate kernel measures every atom's displacement from its build
+// position and raises flags[0]; its last workgroup then scans the cell
+// histogram, and the scatter and build kernels that follow do their work
+// only when flags[0] is set (otherwise they exit at once).  The pair kernel
+// of the tick clears the flag.  No host round trip decides a rebuild.
+//
+// flags: [0] rebuild this tick, [1] a list overflowed its capacity,
+//        [2] a stencil held more candidates than LDS_IDX_CAP,
+//        [3] rebuilds done by the device, [4] [5] last-workgroup counters.
 
 constexpr uint32_t NBR_IDX_MASK = 0x0FFFFFFFu;  // entry = (species << 28) | sorted index
+constexpr int LDS_IDX_CAP = 2048;               // stencil candidates held (and ordered) in LDS
+constexpr int MAX_LOGT = 4;                     // up to 16 lanes per atom
 
 // Entry k of atom i with T lanes per atom: row k/T of an [rows][N][T] array,
 // so the T lanes of consecutive atoms read consecutive words.
@@ -748,14 +783,158 @@ __device__ __forceinline__ size_t nbr_slot(uint32_t k, uint32_t i, uint32_t N, u
     return (((size_t)(k >> logT) * N + i) << logT) + (k & ((1u << logT) - 1u));
 }
 
-__global__ __launch_bounds__(PAIR_THREADS) void k_list_build(KParams PL, const uint32_t* __restrict__ cstart,
-                                                             const double4* __restrict__ pos4,
-                                                             const uint8_t* __restrict__ type,
-                                                             uint32_t* __restrict__ nbr, uint32_t* __restrict__ nlen,
-                                                             uint32_t N, uint32_t logT, uint32_t cap, double cut2,
-                                                             uint32_t* __restrict__ overflow) {
+// Last-workgroup exclusive scan of count[0..ncell) into start[0..ncell]
+// (when `write`), and reset of count to zero.  BLOCK threads, 16 cells per
+// thread per pass.
+__device__ void block_scan_reset(uint32_t* __restrict__ count, uint32_t* __restrict__ start, uint32_t ncell,
+                                 bool write, uint32_t* wsum, uint32_t* carry) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) *carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < ncell; base += BLOCK * 16u) {
+        uint32_t v[16];
+        uint32_t s = 0;
+        const uint32_t i0 = base + (uint32_t)tid * 16u;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const uint32_t idx = i0 + e;
+            v[e] = idx < ncell ? count[idx] : 0u;
+            if (idx < ncell) count[idx] = 0u;
+            s += v[e];
+        }
+        if (!write) continue;
+        uint32_t incl = s;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t t = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += t;
+        }
+        if (lane == 63) wsum[w] = incl;
+        __syncthreads();
+        uint32_t before = *carry;
+        for (int q = 0; q < w; ++q) before += wsum[q];
+        uint32_t excl = before + incl - s;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const uint32_t idx = i0 + e;
+            if (idx < ncell) start[idx] = excl;
+            excl += v[e];
+        }
+        __syncthreads();
+        if (tid == BLOCK - 1) *carry = excl;
+        __syncthreads();
+    }
+    if (write && tid == 0) start[ncell] = *carry;
+}
+
+// List-mode tick start: thermostat scale, half kick, drift, wrap (the same
+// expressions as k_integrate_key), the displacement from the build position
+// (flags[0] when beyond skin/2), the list-grid cell key with a
+// wave-aggregated histogram (one atomic per distinct cell in a wave; the
+// tickets give each atom its slot in its cell), and in the last workgroup
+// the scan of the histogram when a rebuild is due.
+__global__ __launch_bounds__(BLOCK) void k_list_integrate(
+    KParams P, KParams PL, double4* __restrict__ pos4, double* __restrict__ vx, double* __restrict__ vy,
+    double* __restrict__ vz, const double* __restrict__ ax, const double* __restrict__ ay,
+    const double* __restrict__ az, const double* __restrict__ lam_ptr, const double4* __restrict__ xref, double thr2,
+    uint32_t* __restrict__ key, uint32_t* __restrict__ ticket, uint32_t* __restrict__ count,
+    uint32_t* __restrict__ cstart, uint32_t* __restrict__ flags) {
+    __shared__ uint32_t wsum[BLOCK / WAVE];
+    __shared__ uint32_t carry;
+    __shared__ bool last;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int k = blockIdx.x * BLOCK + tid;
+    const bool valid = k < P.n;
+    bool far = false;
+    uint32_t kk = 0;
+    if (valid) {
+        double4 p = pos4[k];
+        const double lam = *lam_ptr;
+        double a = vx[k] * lam, b = vy[k] * lam, c = vz[k] * lam;
+        a = a + ax[k] * P.half_dt;
+        b = b + ay[k] * P.half_dt;
+        c = c + az[k] * P.half_dt;
+        vx[k] = a;
+        vy[k] = b;
+        vz[k] = c;
+        p.x = p.x + a * P.dt;
+        p.y = p.y + b * P.dt;
+        p.z = p.z + c * P.dt;
+        if (P.flags & F_WRAP) {
+            p.x = rem_euclid(p.x, 2.0 * P.half[0]) - P.half[0];
+            p.y = rem_euclid(p.y, 2.0 * P.half[1]) - P.half[1];
+            p.z = rem_euclid(p.z, 2.0 * P.half[2]) - P.half[2];
+        }
+        pos4[k] = p;
+        const double4 r = xref[k];
+        double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+        dx = fma(-P.box[0], rint(dx / P.box[0]), dx);
+        dy = fma(-P.box[1], rint(dy / P.box[1]), dy);
+        dz = fma(-P.box[2], rint(dz / P.box[2]), dz);
+        const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        far = !(d2 <= thr2);  // NaN counts as far
+        kk = cell_key(PL, p.x, p.y, p.z);
+        key[k] = kk;
+    }
+    if (__ballot(far) != 0 && lane == 0) atomicOr(&flags[0], 1u);
+    uint64_t todo = __ballot(valid);
+    while (todo) {
+        const int leader = __builtin_ctzll(todo);
+        const uint32_t lk = __shfl(kk, leader, WAVE);
+        const bool mine = valid && kk == lk;
+        const uint64_t m = __ballot(mine);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&count[lk], (uint32_t)__popcll(m));
+        base = __shfl(base, leader, WAVE);
+        if (mine) ticket[k] = base + mask_rank(m);
+        todo &= ~m;
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) last = atomicAdd(&flags[4], 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const bool need = __atomic_load_n(&flags[0], __ATOMIC_RELAXED) != 0u;
+    block_scan_reset(count, cstart, PL.ncell, need, wsum, &carry);
+    if (tid == 0) {
+        flags[4] = 0u;
+        if (need) flags[3] += 1u;
+    }
+}
+
+// Cell-ordered atom indices of a rebuild (slot = cell start + ticket).
+__global__ __launch_bounds__(BLOCK) void k_list_scatter(int n, const uint32_t* __restrict__ key,
+                                                        const uint32_t* __restrict__ ticket,
+                                                        const uint32_t* __restrict__ start,
+                                                        uint32_t* __restrict__ cidx,
+                                                        const uint32_t* __restrict__ flags) {
+    if (flags[0] == 0u) return;
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= n) return;
+    cidx[start[key[k]] + ticket[k]] = (uint32_t)k;
+}
+
+__global__ void k_flag_set(uint32_t* flags, int idx, uint32_t v) { flags[idx] = v; }
+
+// One workgroup per home cell: the 27 stencil segments of cell-ordered
+// indices are staged in LDS and each segment is ordered by atom index (rank
+// by counting), so the lists do not depend on the order in which atomics
+// handed out the tickets; the candidates' positions, shifted by the image
+// offset of their segment, are then staged in LDS chunks and every wave
+// compacts, for one home atom at a time, the candidates within rc + skin
+// with a ballot.  The home atoms' positions are recorded as the build
+// positions of the displacement test.
+__global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
+    KParams PL, const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cidx,
+    const double4* __restrict__ pos4, const uint8_t* __restrict__ type, uint32_t* __restrict__ nbr,
+    uint32_t* __restrict__ nlen, double4* __restrict__ xref, uint32_t N, uint32_t logT, uint32_t cap, double cut2,
+    uint32_t* __restrict__ flags) {
+    if (flags[0] == 0u) return;
     __shared__ double4 tile[TILE_CAP];
     __shared__ uint32_t tidx[TILE_CAP];
+    __shared__ uint32_t cand[LDS_IDX_CAP];
+    __shared__ uint32_t ord[LDS_IDX_CAP];
     __shared__ uint32_t seg_base[27];
     __shared__ uint32_t seg_pref[28];
     __shared__ double seg_sh[27][3];
@@ -786,6 +965,28 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(KParams PL, const u
     }
     __syncthreads();
     const uint32_t ncand = seg_pref[27];
+    if (ncand > (uint32_t)LDS_IDX_CAP) {
+        if (tid == 0) atomicOr(&flags[2], 1u);
+        return;
+    }
+    for (uint32_t t = tid; t < ncand; t += PAIR_THREADS) {
+        int sg = 0;
+        while (seg_pref[sg + 1] <= t) ++sg;
+        cand[t] = cidx[seg_base[sg] + (t - seg_pref[sg])];
+    }
+    __syncthreads();
+    for (uint32_t t = tid; t < ncand; t += PAIR_THREADS) {
+        int sg = 0;
+        while (seg_pref[sg + 1] <= t) ++sg;
+        const uint32_t v = cand[t], lo = seg_pref[sg], hi = seg_pref[sg + 1];
+        uint32_t r = 0;
+        for (uint32_t f = lo; f < hi; ++f) r += cand[f] < v ? 1u : 0u;
+        ord[lo + r] = v;
+    }
+    for (uint32_t i = tid; i < nh; i += PAIR_THREADS) {
+        const uint32_t gi = cidx[h0 + i];
+        xref[gi] = pos4[gi];
+    }
     const int nchunks = (int)((ncand + TILE_CAP - 1) / TILE_CAP);
     for (int ch = 0; ch < nchunks; ++ch) {
         const uint32_t c0 = (uint32_t)ch * TILE_CAP;
@@ -795,7 +996,7 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(KParams PL, const u
             const uint32_t idx = c0 + t;
             int sg = 0;
             while (seg_pref[sg + 1] <= idx) ++sg;
-            const uint32_t j = seg_base[sg] + (idx - seg_pref[sg]);
+            const uint32_t j = ord[idx];
             double4 pj = pos4[j];
             pj.x += seg_sh[sg][0];
             pj.y += seg_sh[sg][1];
@@ -805,7 +1006,7 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(KParams PL, const u
         }
         __syncthreads();
         for (uint32_t i = w; i < nh; i += PAIR_WAVES) {
-            const uint32_t gi = h0 + i;
+            const uint32_t gi = cidx[h0 + i];
             const double4 pi = pos4[gi];
             uint32_t cnt = ch == 0 ? 0u : nlen[gi];
             for (uint32_t base = 0; base < len; base += 64) {
@@ -828,7 +1029,7 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(KParams PL, const u
             }
             if (lane == 0) {
                 if (ch == nchunks - 1 && cnt > cap) {
-                    atomicOr(overflow, 1u);
+                    atomicOr(&flags[1], 1u);
                     cnt = cap;
                 }
                 nlen[gi] = cnt;
@@ -837,10 +1038,15 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(KParams PL, const u
     }
 }
 
-// Per-atom list traversal: T lanes per atom split its list (entries s, s+T,
-// ...), sum with shuffles.  The minimum image is taken per pair
-// (dx - L rint(dx / L)); pairs of the skin shell beyond rc get exactly zero
-// from the clamped switch, coincident pairs are masked.
+// Per-atom list traversal fused with the second half of the tick: T lanes
+// per atom split its list (entries s, s+T, ...) and sum with shuffles; the
+// minimum image is taken per pair (dx - L rint(dx / L)); pairs of the skin
+// shell beyond rc get exactly zero from the clamped switch, coincident pairs
+// are masked.  The lane that owns the atom then stores a = F/m, applies the
+// second half kick (src/systems/integrate.rs:38-44) and contributes the
+// kinetic energy; the last workgroup reduces the per-block kinetic and
+// potential partials in a fixed order, computes the Berendsen lambda and the
+// observables (close_tick) and clears the rebuild flag for the next tick.
 template <int LOGT, bool COUL, bool SHARED, bool SINGLE>
 __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, const double4* __restrict__ pos4,
                                                      const uint8_t* __restrict__ type,
@@ -848,10 +1054,17 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
                                                      const uint32_t* __restrict__ nlen,
                                                      const PairCoef* __restrict__ coef, double* __restrict__ ax,
                                                      double* __restrict__ ay, double* __restrict__ az,
-                                                     double* __restrict__ vpart) {
+                                                     double* __restrict__ vx, double* __restrict__ vy,
+                                                     double* __restrict__ vz, double* __restrict__ vpart,
+                                                     double* __restrict__ kpart, uint32_t* __restrict__ flags,
+                                                     double* __restrict__ lam_out, gs_observables* __restrict__ obs,
+                                                     ThermoArgs th) {
     constexpr int T = 1 << LOGT;
     __shared__ PairCoef sc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];
     __shared__ double wv[BLOCK / WAVE];
+    __shared__ double wk[BLOCK / WAVE];
+    __shared__ double red[BLOCK / WAVE];
+    __shared__ bool last;
     const int tid = threadIdx.x;
     if (SINGLE) {
         if (tid == 0) sc[0] = coef[0];
@@ -903,74 +1116,45 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
         fy += __shfl_xor(fy, off, WAVE);
         fz += __shfl_xor(fz, off, WAVE);
     }
+    double ke = 0.0;
     if (i < N && s == 0) {
         const double im = P.inv_mass[ti];
-        ax[i] = fx * im;
-        ay[i] = fy * im;
-        az[i] = fz * im;
+        const double a0 = fx * im, a1 = fy * im, a2 = fz * im;
+        ax[i] = a0;
+        ay[i] = a1;
+        az[i] = a2;
+        double u0 = vx[i], u1 = vy[i], u2 = vz[i];
+        u0 = u0 + a0 * P.half_dt;
+        u1 = u1 + a1 * P.half_dt;
+        u2 = u2 + a2 * P.half_dt;
+        vx[i] = u0;
+        vy[i] = u1;
+        vz[i] = u2;
+        ke = 0.5 * P.mass[ti] * (u0 * u0 + u1 * u1 + u2 * u2);
     }
     vl = wave_sum(vl);
-    if ((tid & 63) == 0) wv[tid >> 6] = vl;
-    __syncthreads();
-    if (tid == 0) vpart[blockIdx.x] = 0.5 * ((wv[0] + wv[1]) + (wv[2] + wv[3]));
-}
-
-// List-mode tick start: lambda, kick, drift, wrap (as k_integrate_key) plus
-// the displacement from the list-build position: any atom beyond skin/2
-// flags the tick (the host then replays from its snapshot with a shorter
-// list lifetime); the largest displacement steers the lifetime.
-__global__ __launch_bounds__(BLOCK) void k_integrate_disp(KParams P, double4* __restrict__ pos4,
-                                                          double* __restrict__ vx, double* __restrict__ vy,
-                                                          double* __restrict__ vz, const double* __restrict__ ax,
-                                                          const double* __restrict__ ay,
-                                                          const double* __restrict__ az,
-                                                          const double* __restrict__ lam_ptr,
-                                                          const double4* __restrict__ xref, double thr2,
-                                                          uint32_t* __restrict__ violation,
-                                                          unsigned long long* __restrict__ dispmax) {
-    __shared__ double red[BLOCK / WAVE];
-    const int k = blockIdx.x * BLOCK + threadIdx.x;
-    double d2 = 0.0;
-    if (k < P.n) {
-        double4 p = pos4[k];
-        const double lam = *lam_ptr;
-        double a = vx[k] * lam, b = vy[k] * lam, c = vz[k] * lam;
-        a = a + ax[k] * P.half_dt;
-        b = b + ay[k] * P.half_dt;
-        c = c + az[k] * P.half_dt;
-        vx[k] = a;
-        vy[k] = b;
-        vz[k] = c;
-        p.x = p.x + a * P.dt;
-        p.y = p.y + b * P.dt;
-        p.z = p.z + c * P.dt;
-        if (P.flags & F_WRAP) {
-            p.x = rem_euclid(p.x, 2.0 * P.half[0]) - P.half[0];
-            p.y = rem_euclid(p.y, 2.0 * P.half[1]) - P.half[1];
-            p.z = rem_euclid(p.z, 2.0 * P.half[2]) - P.half[2];
-        }
-        pos4[k] = p;
-        const double4 r = xref[k];
-        double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
-        dx = fma(-P.box[0], rint(dx / P.box[0]), dx);
-        dy = fma(-P.box[1], rint(dy / P.box[1]), dy);
-        dz = fma(-P.box[2], rint(dz / P.box[2]), dz);
-        d2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        if (!(d2 <= thr2)) atomicOr(violation, 1u);  // also catches NaN
+    ke = wave_sum(ke);
+    if ((tid & 63) == 0) {
+        wv[tid >> 6] = vl;
+        wk[tid >> 6] = ke;
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off, WAVE));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = d2;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-        if (m > 0.0 && m == m) atomicMax(dispmax, (unsigned long long)__double_as_longlong(m));
+    if (tid == 0) {
+        vpart[blockIdx.x] = 0.5 * ((wv[0] + wv[1]) + (wv[2] + wv[3]));
+        kpart[blockIdx.x] = (wk[0] + wk[1]) + (wk[2] + wk[3]);
+        __threadfence();
+        last = atomicAdd(&flags[5], 1u) == gridDim.x - 1;
     }
-}
-
-__global__ __launch_bounds__(BLOCK) void k_copy_xref(int n, const double4* __restrict__ p, double4* __restrict__ r) {
-    const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (k < n) r[k] = p[k];
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const double K = block_sum_fixed(kpart, gridDim.x, red);
+    const double V = block_sum_fixed(vpart, gridDim.x, red);
+    if (tid == 0) {
+        close_tick(K, V, (int)N, th, lam_out, obs);
+        flags[5] = 0u;
+        flags[0] = 0u;
+    }
 }
 
 // ---- observation lenses --------------------------------------------------------
@@ -1239,7 +1423,6 @@ struct gs_universe {
     PairCoef* coef = nullptr;
     double* vpart = nullptr;
     uint32_t nvpart = 0;
-    uint32_t nvpart_list = 0;
     double* kpart = nullptr;
     uint32_t* counter = nullptr;
     double* scal = nullptr;  // [0] lambda, [1] V, [2] K, [3] spare
@@ -1269,14 +1452,13 @@ struct gs_universe {
     int pair_mode = 1;          // 0 = per-tick cell tiles, 1 = Verlet list
     int logT = 2;               // lanes per atom = 1 << logT
     double skin = 0.0;
-    int64_t list_life = 10;     // ticks between list builds (adapted)
-    int64_t list_age = 0;
+    int64_t sort_every = 100;   // ticks between full re-sorts of the state in list mode
+    int64_t list_age = 0;       // ticks since the last full re-sort
     uint32_t nbr_cap = 0;
     uint32_t* nbr = nullptr;
     uint32_t* nlen = nullptr;
     double4* xref = nullptr;
-    uint32_t* flags = nullptr;  // [0] skin violation, [1] list overflow
-    unsigned long long* dispmax = nullptr;
+    uint32_t* flags = nullptr;  // device list flags and counters (k_list_integrate)
     int64_t list_builds = 0;
     // snapshot for replays (allocated on first list-mode step)
     double4* snap_pos = nullptr;
@@ -1300,7 +1482,7 @@ struct gs_universe {
     double* adj_lv[3] = {nullptr, nullptr, nullptr};
     double* adj_x[3] = {nullptr, nullptr, nullptr};
     // event timing of gs_step
-    bool timing = false;
+    int timing = 0;  // 0 off, 1 pair kernel only, 2 every phase
     std::vector<hipEvent_t> ev;
     gs_timing tm = {0, 0.0, 0.0, 0.0, 0.0, 0};
 };
@@ -1505,20 +1687,21 @@ int list_alloc(gs_universe* u, uint32_t cap) {
     int rc;
     const size_t n = (size_t)std::max(1, u->n);
     if (!u->nlen) {
-        if ((rc = dalloc(&u->nlen, n)) || (rc = dalloc(&u->xref, n)) || (rc = dalloc(&u->flags, 4)) ||
-            (rc = dalloc(&u->dispmax, 1)) || (rc = dalloc(&u->snap_pos, n)) || (rc = dalloc(&u->snap_type, n)) ||
+        if ((rc = dalloc(&u->nlen, n)) || (rc = dalloc(&u->xref, n)) || (rc = dalloc(&u->flags, 8)) ||
+            (rc = dalloc(&u->snap_pos, n)) || (rc = dalloc(&u->snap_type, n)) ||
             (rc = dalloc(&u->snap_id, n)))
             return rc;
         for (int a = 0; a < 3; ++a)
             if ((rc = dalloc(&u->snap_v[a], n)) || (rc = dalloc(&u->snap_a[a], n))) return rc;
+        HIPCHK(hipMemsetAsync(u->flags, 0, 8 * sizeof(uint32_t), u->stream));
     }
     if (u->nbr_cap < cap) {
         (void)hipFree(u->nbr);
         u->nbr = nullptr;
         u->nbr_cap = 0;
-        // rows of 8 lanes so any lanes-per-atom setting fits
-        const size_t rows = ((size_t)cap + 7) / 8 + 1;
-        if ((rc = dalloc(&u->nbr, rows * 8 * n))) return rc;
+        // rows of 16 lanes so any lanes-per-atom setting fits
+        const size_t rows = ((size_t)cap + 15) / 16 + 1;
+        if ((rc = dalloc(&u->nbr, rows * 16 * n))) return rc;
         u->nbr_cap = cap;
     }
     return GS_OK;
@@ -1541,20 +1724,27 @@ int snapshot(gs_universe* u, bool restore) {
     return GS_OK;
 }
 
-int launch_pair_list(gs_universe* u) {
+// List pair pass of one tick, fused with the second kick, the kinetic
+// reduction and the Berendsen lambda; obs (device) receives the observables.
+int launch_pair_list(gs_universe* u, gs_observables* obs) {
     const KParams& P = u->kp;
+    const gs_params& p = u->prm;
     const int c = u->cur;
     const uint32_t N = (uint32_t)u->n;
     const unsigned nb = nblocks((int64_t)N << u->logT);
+    const ThermoArgs th{p.enable_thermostat != 0 ? 1 : 0, p.thermostat_temperature, std::fmax(p.thermostat_tau, 1.0),
+                        p.thermal_constant};
 #define GS_PL(L, C, S, SG)                                                                                       \
     k_pair_list<L, C, S, SG><<<nb, BLOCK, 0, u->stream>>>(P, N, u->pos4[c], u->type[c], u->nbr, u->nlen, u->coef, \
-                                                          u->a[0], u->a[1], u->a[2], u->vpart)
-#define GS_PL_T(C, S, SG)          \
-    switch (u->logT) {             \
-        case 0: GS_PL(0, C, S, SG); break; \
-        case 1: GS_PL(1, C, S, SG); break; \
-        case 2: GS_PL(2, C, S, SG); break; \
-        default: GS_PL(3, C, S, SG); break; \
+                                                          u->a[0], u->a[1], u->a[2], u->v[c][0], u->v[c][1],     \
+                                                          u->v[c][2], u->vpart, u->kpart, u->flags, u->scal, obs, th)
+#define GS_PL_T(C, S, SG)                   \
+    switch (u->logT) {                      \
+        case 0: GS_PL(0, C, S, SG); break;  \
+        case 1: GS_PL(1, C, S, SG); break;  \
+        case 2: GS_PL(2, C, S, SG); break;  \
+        case 3: GS_PL(3, C, S, SG); break;  \
+        default: GS_PL(4, C, S, SG); break; \
     }
     if (u->single) {
         GS_PL_T(false, false, true);
@@ -1567,11 +1757,44 @@ int launch_pair_list(gs_universe* u) {
     }
 #undef GS_PL_T
 #undef GS_PL
-    u->nvpart_list = nb;
     HIPCHK(hipGetLastError());
     return GS_OK;
 }
 
+// Start of a list-mode tick.  A full re-sort (every sort_every ticks, or
+// when the lists do not match the resident order) runs the per-tick rebuild
+// on the list grid, which permutes the state into cell order, and forces a
+// list build over the identity order.  Otherwise the device integrates,
+// tests the displacements and rebuilds the lists only when an atom moved
+// beyond skin/2.
+int list_tick_start(gs_universe* u) {
+    const uint32_t N = (uint32_t)u->n;
+    const double rl = u->kp.rc + u->skin;
+    const KParams& PL = u->kpl;
+    int rc;
+    if (!u->list_valid || u->list_age >= u->sort_every) {
+        if ((rc = rebuild(u, true, PL))) return rc;
+        k_iota<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->n, u->perm);
+        k_flag_set<<<1, 1, 0, u->stream>>>(u->flags, 0, 1u);
+        u->list_valid = true;
+        u->list_age = 0;
+        u->list_builds += 1;
+    } else {
+        const int c = u->cur;
+        const double h = 0.5 * u->skin;
+        k_list_integrate<<<nblocks(u->n), BLOCK, 0, u->stream>>>(
+            u->kp, PL, u->pos4[c], u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->scal, u->xref,
+            h * h, u->key, u->ticket, u->count, u->cstart, u->flags);
+        k_list_scatter<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->n, u->key, u->ticket, u->cstart, u->perm, u->flags);
+    }
+    const int c = u->cur;
+    k_list_build<<<PL.ncell, PAIR_THREADS, 0, u->stream>>>(PL, u->cstart, u->perm, u->pos4[c], u->type[c], u->nbr,
+                                                           u->nlen, u->xref, N, (uint32_t)u->logT, u->nbr_cap,
+                                                           rl * rl, u->flags);
+    u->list_age += 1;
+    HIPCHK(hipGetLastError());
+    return GS_OK;
+}
 
 // The tick loop of gs_step: kick, drift, wrap, [cell list], forces, kick,
 // Berendsen, in build_schedule order (src/universe/mod.rs:355-369).
@@ -1579,47 +1802,27 @@ int run_ticks(gs_universe* u, int64_t n_steps, bool observe, bool list) {
     int rc;
     const gs_params& p = u->prm;
     const bool thermo = p.enable_thermostat != 0;
-    const double rl = u->kp.rc + u->skin;
     for (int64_t s = 0; s < n_steps; ++s) {
-        if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 0], u->stream));
-        uint32_t nvp = u->nvpart;
+        if (u->timing > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 0], u->stream));
         if (list) {
-            if (!u->list_valid || u->list_age >= u->list_life) {
-                if ((rc = rebuild(u, true, u->kpl))) return rc;
-                const int c = u->cur;
-                k_copy_xref<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->n, u->pos4[c], u->xref);
-                k_list_build<<<u->kpl.ncell, PAIR_THREADS, 0, u->stream>>>(
-                    u->kpl, u->cstart, u->pos4[c], u->type[c], u->nbr, u->nlen, (uint32_t)u->n, (uint32_t)u->logT,
-                    u->nbr_cap, rl * rl, u->flags + 1);
-                u->list_valid = true;
-                u->list_age = 0;
-                u->list_builds += 1;
-            } else {
-                const int c = u->cur;
-                const double h = 0.5 * u->skin;
-                k_integrate_disp<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->kp, u->pos4[c], u->v[c][0], u->v[c][1],
-                                                                         u->v[c][2], u->a[0], u->a[1], u->a[2],
-                                                                         u->scal, u->xref, h * h, u->flags,
-                                                                         u->dispmax);
-            }
-            HIPCHK(hipGetLastError());
+            if ((rc = list_tick_start(u))) return rc;
             if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 1], u->stream));
-            if ((rc = launch_pair_list(u))) return rc;
-            nvp = u->nvpart_list;
-            u->list_age += 1;
-        } else {
-            if ((rc = rebuild(u, true, u->kp))) return rc;
-            u->list_valid = false;
-            if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 1], u->stream));
-            if ((rc = launch_pair<false>(u))) return rc;
+            if ((rc = launch_pair_list(u, observe ? u->obs + s : nullptr))) return rc;
+            if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 2], u->stream));
+            if (u->timing > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
+            continue;
         }
+        if ((rc = rebuild(u, true, u->kp))) return rc;
+        u->list_valid = false;
+        if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 1], u->stream));
+        if ((rc = launch_pair<false>(u))) return rc;
         if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 2], u->stream));
         const int c = u->cur;
         k_kick_ke<true><<<nblocks(u->n), BLOCK, 0, u->stream>>>(
             u->kp, u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->type[c], u->kpart, u->counter,
-            u->vpart, nvp, u->scal, observe ? u->obs + s : nullptr, thermo ? 1 : 0, p.thermostat_temperature,
+            u->vpart, u->nvpart, u->scal, observe ? u->obs + s : nullptr, thermo ? 1 : 0, p.thermostat_temperature,
             std::fmax(p.thermostat_tau, 1.0), p.thermal_constant);
-        if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
+        if (u->timing > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
     }
     HIPCHK(hipGetLastError());
     return GS_OK;
@@ -1678,8 +1881,8 @@ int gs_create(const gs_params* params, int64_t n_atoms, gs_universe** out) {
         if ((rc = dalloc(&u->a[a], n))) return bail(rc);
     if ((rc = dalloc(&u->key, n)) || (rc = dalloc(&u->ticket, n)) || (rc = dalloc(&u->perm, n)) ||
         (rc = dalloc(&u->count, (size_t)P.ncell)) || (rc = dalloc(&u->cstart, (size_t)P.ncell + 1)) ||
-        (rc = dalloc(&u->coef, coef.size())) || (rc = dalloc(&u->vpart, (size_t)std::max<uint32_t>(u->nvpart, nblocks((int64_t)n * 8)))) ||
-        (rc = dalloc(&u->kpart, (size_t)nblocks(n))) || (rc = dalloc(&u->counter, 4)) ||
+        (rc = dalloc(&u->coef, coef.size())) || (rc = dalloc(&u->vpart, (size_t)std::max<uint32_t>(u->nvpart, nblocks((int64_t)n << MAX_LOGT)))) ||
+        (rc = dalloc(&u->kpart, (size_t)nblocks((int64_t)n << MAX_LOGT))) || (rc = dalloc(&u->counter, 4)) ||
         (rc = dalloc(&u->scal, 4)) || (rc = dalloc(&u->pe, n)) || (rc = dalloc(&u->sabs, n)) ||
         (rc = dalloc(&u->stage, 9 * n)) || (rc = dalloc(&u->stage_u, n)))
         return bail(rc);
@@ -1737,7 +1940,6 @@ int gs_destroy(gs_universe* u) {
     (void)hipFree(u->nlen);
     (void)hipFree(u->xref);
     (void)hipFree(u->flags);
-    (void)hipFree(u->dispmax);
     (void)hipFree(u->snap_pos);
     for (int a = 0; a < 3; ++a) (void)hipFree(u->snap_v[a]);
     for (int a = 0; a < 3; ++a) (void)hipFree(u->snap_a[a]);
@@ -1968,7 +2170,7 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
         if ((rc = dalloc(&u->obs, (size_t)n_steps))) return rc;
         u->obs_cap = n_steps;
     }
-    const bool list = u->pair_mode == 1 && u->list_ok && !u->small_box;
+    bool list = u->pair_mode == 1 && u->list_ok && !u->small_box;
     const size_t nev = u->timing ? (size_t)(4 * n_steps) : 0;
     while (u->ev.size() < nev) {
         hipEvent_t e;
@@ -1981,43 +2183,34 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
         if ((rc = snapshot(u, false))) return rc;
     }
     for (int attempt = 0;; ++attempt) {
-        if (list) {
-            HIPCHK(hipMemsetAsync(u->flags, 0, 4 * sizeof(uint32_t), u->stream));
-            HIPCHK(hipMemsetAsync(u->dispmax, 0, sizeof(unsigned long long), u->stream));
-        }
+        if (list) HIPCHK(hipMemsetAsync(u->flags, 0, 8 * sizeof(uint32_t), u->stream));
         if ((rc = run_ticks(u, n_steps, obs != nullptr, list))) return rc;
         const int c = u->cur;
         k_apply_lambda<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->n, u->v[c][0], u->v[c][1], u->v[c][2], u->scal);
         k_set_scalar<<<1, 1, 0, u->stream>>>(u->scal, 1.0);
         HIPCHK(hipGetLastError());
-        uint32_t fl[4] = {0, 0, 0, 0};
-        unsigned long long dm = 0;
-        if (list) {
-            HIPCHK(hipMemcpyAsync(fl, u->flags, sizeof(fl), hipMemcpyDeviceToHost, u->stream));
-            HIPCHK(hipMemcpyAsync(&dm, u->dispmax, sizeof(dm), hipMemcpyDeviceToHost, u->stream));
-        }
+        uint32_t fl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (list) HIPCHK(hipMemcpyAsync(fl, u->flags, sizeof(fl), hipMemcpyDeviceToHost, u->stream));
         HIPCHK(hipStreamSynchronize(u->stream));
-        if (list && (fl[0] || fl[1])) {
-            // A list outlived its skin or overflowed: replay the call from the
-            // snapshot with a shorter lifetime / a larger list.
-            if (attempt > 40) return fail(GS_E_OVERFLOW, "neighbour list could not be made valid");
+        if (list && (fl[1] || fl[2])) {
+            // A list overflowed its capacity, or a stencil held more candidates
+            // than the build stages in LDS: replay the call from the snapshot
+            // with larger lists, or on the per-tick cell tiles.
+            if (attempt > 8) return fail(GS_E_OVERFLOW, "neighbour list could not be made valid");
             u->cur = cur0;
             if ((rc = snapshot(u, true))) return rc;
             u->list_valid = false;
-            if (fl[0]) u->list_life = std::max<int64_t>(1, u->list_life / 2);
-            if (fl[1]) {
+            if (fl[2]) {
+                u->pair_mode = 0;
+                list = false;
+            } else {
                 const uint32_t cap = u->nbr_cap * 2;
                 if (cap > (1u << 16)) return fail(GS_E_OVERFLOW, "neighbour list exceeds 65536 entries per atom");
                 if ((rc = list_alloc(u, cap))) return rc;
             }
             continue;
         }
-        if (list) {
-            double d2;
-            std::memcpy(&d2, &dm, sizeof(d2));
-            const double half = 0.5 * u->skin;
-            if (std::sqrt(d2) < 0.5 * half) u->list_life = std::min<int64_t>(u->list_life + 1 + u->list_life / 4, 500);
-        }
+        if (list) u->list_builds += fl[3];
         break;
     }
     if (obs)
@@ -2026,9 +2219,11 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
     if (u->timing) {
         for (int64_t s = 0; s < n_steps; ++s) {
             float a = 0.f, b = 0.f, c2 = 0.f;
-            HIPCHK(hipEventElapsedTime(&a, u->ev[4 * s + 0], u->ev[4 * s + 1]));
             HIPCHK(hipEventElapsedTime(&b, u->ev[4 * s + 1], u->ev[4 * s + 2]));
-            HIPCHK(hipEventElapsedTime(&c2, u->ev[4 * s + 2], u->ev[4 * s + 3]));
+            if (u->timing > 1) {
+                HIPCHK(hipEventElapsedTime(&a, u->ev[4 * s + 0], u->ev[4 * s + 1]));
+                HIPCHK(hipEventElapsedTime(&c2, u->ev[4 * s + 2], u->ev[4 * s + 3]));
+            }
             u->tm.rebuild_ms += a;
             u->tm.pair_ms += b;
             u->tm.kick_ms += c2;
@@ -2044,7 +2239,7 @@ int gs_get_tuning(gs_universe* u, gs_tuning* out) {
     out->pair_mode = (u->pair_mode == 1 && u->list_ok && !u->small_box) ? 1 : 0;
     out->lanes_per_atom = 1 << u->logT;
     out->skin = u->skin;
-    out->list_life = u->list_life;
+    out->sort_every = u->sort_every;
     out->list_builds = u->list_builds;
     out->ms_per_tick = 0.0;
     return GS_OK;
@@ -2060,14 +2255,15 @@ int gs_set_tuning(gs_universe* u, const gs_tuning* t) {
             case 2: logT = 1; break;
             case 4: logT = 2; break;
             case 8: logT = 3; break;
-            default: return fail(GS_E_INVALID, "lanes_per_atom must be 1, 2, 4 or 8");
+            case 16: logT = 4; break;
+            default: return fail(GS_E_INVALID, "lanes_per_atom must be 1, 2, 4, 8 or 16");
         }
         if (!(t->skin >= 0.0) || !std::isfinite(t->skin)) return fail(GS_E_INVALID, "skin must be >= 0");
         if (u->small_box || !list_geometry(u, t->skin))
             return fail(GS_E_INVALID, "list mode needs >= 3 cells of edge rc + skin on every axis");
         u->list_ok = true;
         u->logT = logT;
-        if (t->list_life > 0) u->list_life = t->list_life;
+        if (t->sort_every > 0) u->sort_every = t->sort_every;
     }
     u->pair_mode = t->pair_mode;
     u->list_valid = false;
@@ -2089,15 +2285,14 @@ int gs_autotune(gs_universe* u, int32_t trial_steps, gs_tuning* best) {
     std::vector<gs_tuning> cand;
     cand.push_back(gs_tuning{0, 1, 0.0, 0, 0, 0.0});
     if (!u->small_box)
-        for (double f : {0.03, 0.06, 0.10})
-            for (int T : {1, 2, 4, 8}) cand.push_back(gs_tuning{1, T, f * u->kp.rc, 0, 0, 0.0});
-    const bool was_timing = u->timing;
-    u->timing = false;
+        for (double f : {0.04, 0.08, 0.12, 0.18})
+            for (int T : {2, 4, 8, 16}) cand.push_back(gs_tuning{1, T, f * u->kp.rc, 0, 0, 0.0});
+    const int was_timing = u->timing;
+    u->timing = 0;
     gs_tuning win{0, 1, 0.0, 0, 0, 1e300};
     for (gs_tuning c : cand) {
         if (gs_set_tuning(u, &c) != GS_OK) continue;
-        u->list_life = 10;
-        if ((rc = gs_step(u, trial_steps / 2, nullptr))) return rc;  // warm up, adapt the lifetime
+        if ((rc = gs_step(u, trial_steps / 2, nullptr))) return rc;  // warm up: first lists, list capacity
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
@@ -2110,8 +2305,9 @@ int gs_autotune(gs_universe* u, int32_t trial_steps, gs_tuning* best) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         c.ms_per_tick = ms / trial_steps;
-        c.list_life = u->list_life;
-        if (c.ms_per_tick < win.ms_per_tick) win = c;
+        c.sort_every = u->sort_every;
+        // a list setting that fell back to the cell tiles is not a candidate
+        if (u->pair_mode == c.pair_mode && c.ms_per_tick < win.ms_per_tick) win = c;
         if ((rc = gs_upload_state(u, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], sp.data(), q.data())))
             return rc;
     }
@@ -2127,7 +2323,8 @@ int gs_autotune(gs_universe* u, int32_t trial_steps, gs_tuning* best) {
 
 int gs_timing_enable(gs_universe* u, int on) {
     if (!u) return fail(GS_E_INVALID, "universe handle is NULL");
-    u->timing = on != 0;
+    if (on < 0 || on > 2) return fail(GS_E_INVALID, "timing mode must be 0, 1 or 2");
+    u->timing = on;
     u->tm = gs_timing{0, 0.0, 0.0, 0.0, 0.0, 0};
     return GS_OK;
 }

@@ -132,10 +132,11 @@ class GpuUniverse:
         return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}
 
     def set_tuning(self, pair_mode: int, lanes_per_atom: int = 4, skin: Optional[float] = None,
-                   list_life: int = 0) -> None:
-        """pair_mode 0 = per-tick cell tiles, 1 = Verlet list (lanes_per_atom, skin)."""
+                   sort_every: int = 0) -> None:
+        """pair_mode 0 = per-tick cell tiles, 1 = Verlet list (lanes_per_atom, skin, and
+        sort_every = ticks between full re-sorts of the resident state; 0 keeps it)."""
         cur = self.tuning()
-        t = _lib.Tuning(pair_mode, lanes_per_atom, cur["skin"] if skin is None else skin, list_life, 0, 0.0)
+        t = _lib.Tuning(pair_mode, lanes_per_atom, cur["skin"] if skin is None else skin, sort_every, 0, 0.0)
         _lib.check(self.lib.gs_set_tuning(self.h, ctypes.byref(t)))
 
     def autotune(self, trial_steps: int = 40) -> dict:
@@ -143,10 +144,13 @@ class GpuUniverse:
         _lib.check(self.lib.gs_autotune(self.h, trial_steps, ctypes.byref(t)))
         return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}
 
-    def timing(self, enable: Optional[bool] = None) -> dict:
-        """Per-phase event timing of step(); enable=True resets the counters."""
+    def timing(self, enable=None) -> dict:
+        """Event timing of step(): enable=True or "phases" times every phase (four
+        events per tick), "pair" only the pair kernel (two), False stops; enabling
+        resets the counters."""
         if enable is not None:
-            _lib.check(self.lib.gs_timing_enable(self.h, 1 if enable else 0))
+            mode = {False: 0, True: 2, "phases": 2, "pair": 1}[enable]
+            _lib.check(self.lib.gs_timing_enable(self.h, mode))
         t = _lib.Timing()
         _lib.check(self.lib.gs_timing_read(self.h, ctypes.byref(t)))
         return {f: getattr(t, f) for f, _ in _lib.Timing._fields_}

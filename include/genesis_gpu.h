@@ -111,15 +111,18 @@ typedef struct gs_timing {
  * path).  pair_mode 0 rebuilds the cell list every tick and evaluates the
  * 27-cell LDS tiles directly; pair_mode 1 builds, from the same tiles, a
  * Verlet list with cutoff rc + skin and reuses it while every atom stays
- * within skin/2 of its position at the build.  Both give the reference's
+ * within skin/2 of its position at the build.  The device tests the
+ * displacements every tick and rebuilds the lists itself when needed; the
+ * host only schedules a full re-sort of the resident state every
+ * sort_every ticks (memory locality).  Both modes give the reference's
  * pair set. */
 typedef struct gs_tuning {
-    int32_t pair_mode;       /* 0 = per-tick cell tiles, 1 = Verlet list           */
-    int32_t lanes_per_atom;  /* list mode: 1, 2, 4 or 8 lanes share an atom's list */
-    double skin;             /* list mode: list cutoff = rc + skin                 */
-    int64_t list_life;       /* list mode: ticks between builds (adapted)          */
-    int64_t list_builds;     /* list builds so far (read only)                     */
-    double ms_per_tick;      /* gs_autotune: time of the chosen setting            */
+    int32_t pair_mode;       /* 0 = per-tick cell tiles, 1 = Verlet list             */
+    int32_t lanes_per_atom;  /* list mode: 1, 2, 4, 8 or 16 lanes share a list       */
+    double skin;             /* list mode: list cutoff = rc + skin                   */
+    int64_t sort_every;      /* list mode: ticks between full re-sorts (0 = keep)    */
+    int64_t list_builds;     /* list builds so far (read only)                       */
+    double ms_per_tick;      /* gs_autotune: time of the chosen setting              */
 } gs_tuning;
 
 typedef struct gs_universe gs_universe;
@@ -190,7 +193,9 @@ int gs_set_tuning(gs_universe* u, const gs_tuning* t);
  * fastest.  best (may be NULL) receives it. */
 int gs_autotune(gs_universe* u, int32_t trial_steps, gs_tuning* best);
 
-/* Enable (and reset) or disable per-phase event timing of gs_step. */
+/* Event timing of gs_step, reset on every call: on = 0 off, 1 the pair
+ * kernel only (two events per tick), 2 every phase (four events per tick;
+ * rebuild_ms and kick_ms stay 0 in mode 1). */
 int gs_timing_enable(gs_universe* u, int on);
 int gs_timing_read(gs_universe* u, gs_timing* out);
 

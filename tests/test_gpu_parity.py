@@ -305,7 +305,7 @@ def test_list_mode_matches_oracle_over_a_mixture_trajectory(GU, lanes, mixture_6
         obs = u.step(60)
         got = u.download()
         tun = u.tuning()
-    say(f"list builds {tun['list_builds']}, lifetime {tun['list_life']}")
+    say(f"list builds {tun['list_builds']}, sort every {tun['sort_every']}")
     assert tun["pair_mode"] == 1 and tun["list_builds"] >= 1
     np.testing.assert_allclose(obs[:, 1], obs_ref[:, 1], rtol=1e-8)
     np.testing.assert_allclose(obs[:, 0], obs_ref[:, 0], rtol=1e-8)
@@ -323,7 +323,7 @@ def test_list_mode_forces_after_many_ticks_match_a_fresh_force_pass(GU):
         u.step(300)
         got = u.download()
         tun = u.tuning()
-    say(f"list builds {tun['list_builds']}, lifetime {tun['list_life']}")
+    say(f"list builds {tun['list_builds']}, sort every {tun['sort_every']}")
     ref = O.forces(ph, State(got.x, got.y, got.z, got.vx, got.vy, got.vz, st.species, st.charge))
     m = np.asarray(ph.mass)[st.species]
     ok, worst = force_tolerance_ok(got.ax, got.ay, got.az, ref, m, REL)
@@ -331,22 +331,47 @@ def test_list_mode_forces_after_many_ticks_match_a_fresh_force_pass(GU):
     assert ok, worst
 
 
-def test_list_skin_violation_replays_with_shorter_lifetime(GU):
-    # A hot gas with a tiny skin forces lifetime cuts and replays; the result
-    # must still match the tile path.
+def test_list_rebuilds_on_device_when_atoms_pass_half_skin(GU):
+    # A hot gas with a tiny skin: the device must rebuild the lists on many
+    # ticks (no host round trip decides it), and the trajectory must match
+    # the per-tick cell tiles.  Two calls, and a short re-sort period, cover
+    # the lists surviving across calls and the forced re-sorts.
     w = mixture(n=4096, seed=41, temperature=600.0, steps=40, thermostat=False)
     ph, st = w.physics, w.state
-    say("skin violation replay")
+    say("device-driven list rebuilds, hot gas, skin 0.02")
     with GU(ph, st) as u:
-        u.set_tuning(1, 2, skin=0.02, list_life=50)
-        obs_l = u.step(40)
+        u.set_tuning(1, 2, skin=0.02, sort_every=15)
+        obs_l = np.concatenate([u.step(25), u.step(15)])
         got_l = u.download()
+        tun = u.tuning()
     with GU(ph, st) as u:
         u.set_tuning(0)
         obs_t = u.step(40)
         got_t = u.download()
+    say(f"list builds {tun['list_builds']} in 40 ticks")
+    assert tun["pair_mode"] == 1 and tun["list_builds"] >= 10
     np.testing.assert_allclose(obs_l[:, 5], obs_t[:, 5], rtol=1e-9)
     np.testing.assert_allclose(got_l.x, got_t.x, rtol=0, atol=1e-9)
+
+
+def test_list_mode_falls_back_to_tiles_when_a_stencil_is_too_dense(GU):
+    # A skin so large that a 27-cell stencil holds more candidates than the
+    # list build stages in LDS: the call is replayed on the cell tiles and
+    # the tuning reports pair_mode 0.
+    w = mixture(n=32768, seed=43, steps=10)
+    ph, st = w.physics, w.state
+    say("dense stencil fallback")
+    with GU(ph, st) as u:
+        u.set_tuning(1, 4, skin=0.6 * ph.force_rc())
+        obs_l = u.step(10)
+        got_l = u.download()
+        tun = u.tuning()
+    with GU(ph, st) as u:
+        u.set_tuning(0)
+        obs_t = u.step(10)
+        got_t = u.download()
+    assert tun["pair_mode"] == 0
+    assert np.array_equal(obs_l, obs_t) and np.array_equal(got_l.x, got_t.x)
 
 
 def test_autotune_restores_state_and_picks_a_valid_setting(GU):
