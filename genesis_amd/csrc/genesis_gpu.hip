@@ -866,11 +866,15 @@ __global__ __launch_bounds__(BLOCK) void k_list_integrate(
             p.z = rem_euclid(p.z, 2.0 * P.half[2]) - P.half[2];
         }
         pos4[k] = p;
+        // The wrap (rem_euclid(v, 2h) - h) translates the whole universe
+        // rigidly by h on every axis each tick, so the displacement since
+        // the build is taken modulo h: that removes the accumulated
+        // translation exactly and keeps any motion below h/2.
         const double4 r = xref[k];
         double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
-        dx = fma(-P.box[0], rint(dx / P.box[0]), dx);
-        dy = fma(-P.box[1], rint(dy / P.box[1]), dy);
-        dz = fma(-P.box[2], rint(dz / P.box[2]), dz);
+        dx = fma(-P.half[0], rint(dx / P.half[0]), dx);
+        dy = fma(-P.half[1], rint(dy / P.half[1]), dy);
+        dz = fma(-P.half[2], rint(dz / P.half[2]), dz);
         const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
         far = !(d2 <= thr2);  // NaN counts as far
         kk = cell_key(PL, p.x, p.y, p.z);
