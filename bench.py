@@ -90,9 +90,10 @@ def main() -> int:
     if args.warmup:
         run(args.warmup)
     builds0 = u.tuning()["list_builds"]
-    # timed region: two HIP events per tick on the universe's stream bracket
-    # the pair kernel (its average duration feeds the roofline)
-    u.timing(enable="pair")
+    # timed region: on every 10th tick two HIP events on the universe's
+    # stream bracket the pair kernel (its average duration feeds the
+    # roofline); sampling keeps the event records' cost out of the rate
+    u.timing(enable="pair", every=10)
     t0 = time.perf_counter()
     run(args.steps)
     t1 = time.perf_counter()

@@ -764,14 +764,18 @@ __global__ __launch_bounds__(BLOCK) void k_cells_entity(KParams P, const uint32_
 //
 // The device decides, tick by tick, whether the lists must be rebuilt: the
 // integrate kernel measures every atom's displacement from its build
-// position and raises flags[0]; its last workgroup then scans the cell
-// histogram, and the scatter and build kernels that follow do their work
-// only when flags[0] is set (otherwise they exit at once).  The pair kernel
-// of the tick clears the flag.  No host round trip decides a rebuild.
+// position and raises flags[0]; the bin and build kernels that follow do
+// their work only when flags[0] is set (otherwise they exit at once), and
+// the pair kernel of the tick clears the flag.  No host round trip decides
+// a rebuild.  A rebuild bins the atoms into fixed-capacity cell buckets
+// (one atomic per distinct cell in a wave, no scan), and the build orders
+// each bucket by atom index in LDS, so the lists do not depend on the order
+// in which the atomics handed out the slots.
 //
 // flags: [0] rebuild this tick, [1] a list overflowed its capacity,
-//        [2] a stencil held more candidates than LDS_IDX_CAP,
-//        [3] rebuilds done by the device, [4] [5] last-workgroup counters.
+//        [2] a stencil held more candidates than LDS_IDX_CAP, [3] rebuilds
+//        done by the device, [5] last-workgroup counter of the pair kernel,
+//        [6] a cell bucket overflowed its capacity.
 
 constexpr uint32_t NBR_IDX_MASK = 0x0FFFFFFFu;  // entry = (species << 28) | sorted index
 constexpr int LDS_IDX_CAP = 2048;               // stencil candidates held (and ordered) in LDS
@@ -783,71 +787,20 @@ __device__ __forceinline__ size_t nbr_slot(uint32_t k, uint32_t i, uint32_t N, u
     return (((size_t)(k >> logT) * N + i) << logT) + (k & ((1u << logT) - 1u));
 }
 
-// Last-workgroup exclusive scan of count[0..ncell) into start[0..ncell]
-// (when `write`), and reset of count to zero.  BLOCK threads, 16 cells per
-// thread per pass.
-__device__ void block_scan_reset(uint32_t* __restrict__ count, uint32_t* __restrict__ start, uint32_t ncell,
-                                 bool write, uint32_t* wsum, uint32_t* carry) {
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    if (tid == 0) *carry = 0;
-    __syncthreads();
-    for (uint32_t base = 0; base < ncell; base += BLOCK * 16u) {
-        uint32_t v[16];
-        uint32_t s = 0;
-        const uint32_t i0 = base + (uint32_t)tid * 16u;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const uint32_t idx = i0 + e;
-            v[e] = idx < ncell ? count[idx] : 0u;
-            if (idx < ncell) count[idx] = 0u;
-            s += v[e];
-        }
-        if (!write) continue;
-        uint32_t incl = s;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t t = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += t;
-        }
-        if (lane == 63) wsum[w] = incl;
-        __syncthreads();
-        uint32_t before = *carry;
-        for (int q = 0; q < w; ++q) before += wsum[q];
-        uint32_t excl = before + incl - s;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const uint32_t idx = i0 + e;
-            if (idx < ncell) start[idx] = excl;
-            excl += v[e];
-        }
-        __syncthreads();
-        if (tid == BLOCK - 1) *carry = excl;
-        __syncthreads();
-    }
-    if (write && tid == 0) start[ncell] = *carry;
-}
-
 // List-mode tick start: thermostat scale, half kick, drift, wrap (the same
-// expressions as k_integrate_key), the displacement from the build position
-// (flags[0] when beyond skin/2), the list-grid cell key with a
-// wave-aggregated histogram (one atomic per distinct cell in a wave; the
-// tickets give each atom its slot in its cell), and in the last workgroup
-// the scan of the histogram when a rebuild is due.
+// expressions as k_integrate_key) and the displacement from the build
+// position (flags[0] when beyond skin/2).  Also clears the bucket counts a
+// rebuild of this tick would fill.
 __global__ __launch_bounds__(BLOCK) void k_list_integrate(
-    KParams P, KParams PL, double4* __restrict__ pos4, double* __restrict__ vx, double* __restrict__ vy,
-    double* __restrict__ vz, const double* __restrict__ ax, const double* __restrict__ ay,
-    const double* __restrict__ az, const double* __restrict__ lam_ptr, const double4* __restrict__ xref, double thr2,
-    uint32_t* __restrict__ key, uint32_t* __restrict__ ticket, uint32_t* __restrict__ count,
-    uint32_t* __restrict__ cstart, uint32_t* __restrict__ flags) {
-    __shared__ uint32_t wsum[BLOCK / WAVE];
-    __shared__ uint32_t carry;
-    __shared__ bool last;
+    KParams P, double4* __restrict__ pos4, double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+    const double* __restrict__ ax, const double* __restrict__ ay, const double* __restrict__ az,
+    const double* __restrict__ lam_ptr, const double4* __restrict__ xref, double thr2,
+    uint32_t* __restrict__ bcount, uint32_t nbucket, uint32_t* __restrict__ flags) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int k = blockIdx.x * BLOCK + tid;
-    const bool valid = k < P.n;
+    for (uint32_t c = (uint32_t)k; c < nbucket; c += gridDim.x * BLOCK) bcount[c] = 0u;
     bool far = false;
-    uint32_t kk = 0;
-    if (valid) {
+    if (k < P.n) {
         double4 p = pos4[k];
         const double lam = *lam_ptr;
         double a = vx[k] * lam, b = vy[k] * lam, c = vz[k] * lam;
@@ -877,10 +830,26 @@ __global__ __launch_bounds__(BLOCK) void k_list_integrate(
         dz = fma(-P.half[2], rint(dz / P.half[2]), dz);
         const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
         far = !(d2 <= thr2);  // NaN counts as far
-        kk = cell_key(PL, p.x, p.y, p.z);
-        key[k] = kk;
     }
     if (__ballot(far) != 0 && lane == 0) atomicOr(&flags[0], 1u);
+}
+
+// Rebuild binning (only when flags[0] is set): list-grid cell of every
+// atom and its slot in the cell's bucket, one atomic per distinct cell in a
+// wave.  A bucket that overflows its capacity raises flags[6].
+__global__ __launch_bounds__(BLOCK) void k_list_bin(KParams PL, const double4* __restrict__ pos4,
+                                                    uint32_t* __restrict__ bcount, uint32_t* __restrict__ bucket,
+                                                    uint32_t bcap, uint32_t* __restrict__ flags) {
+    if (flags[0] == 0u) return;
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k == 0) flags[3] += 1u;
+    const bool valid = k < PL.n;
+    uint32_t kk = 0;
+    if (valid) {
+        const double4 p = pos4[k];
+        kk = cell_key(PL, p.x, p.y, p.z);
+    }
     uint64_t todo = __ballot(valid);
     while (todo) {
         const int leader = __builtin_ctzll(todo);
@@ -888,64 +857,66 @@ __global__ __launch_bounds__(BLOCK) void k_list_integrate(
         const bool mine = valid && kk == lk;
         const uint64_t m = __ballot(mine);
         uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&count[lk], (uint32_t)__popcll(m));
+        if (lane == leader) base = atomicAdd(&bcount[lk], (uint32_t)__popcll(m));
         base = __shfl(base, leader, WAVE);
-        if (mine) ticket[k] = base + mask_rank(m);
+        if (mine) {
+            const uint32_t slot = base + mask_rank(m);
+            if (slot < bcap)
+                bucket[(size_t)lk * bcap + slot] = (uint32_t)k;
+            else
+                atomicOr(&flags[6], 1u);
+        }
         todo &= ~m;
-    }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) last = atomicAdd(&flags[4], 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    const bool need = __atomic_load_n(&flags[0], __ATOMIC_RELAXED) != 0u;
-    block_scan_reset(count, cstart, PL.ncell, need, wsum, &carry);
-    if (tid == 0) {
-        flags[4] = 0u;
-        if (need) flags[3] += 1u;
     }
 }
 
-// Cell-ordered atom indices of a rebuild (slot = cell start + ticket).
-__global__ __launch_bounds__(BLOCK) void k_list_scatter(int n, const uint32_t* __restrict__ key,
-                                                        const uint32_t* __restrict__ ticket,
-                                                        const uint32_t* __restrict__ start,
-                                                        uint32_t* __restrict__ cidx,
-                                                        const uint32_t* __restrict__ flags) {
-    if (flags[0] == 0u) return;
-    const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= n) return;
-    cidx[start[key[k]] + ticket[k]] = (uint32_t)k;
+// Buckets of a freshly cell-sorted state: bucket c holds cstart[c] ..
+// cstart[c+1]-1 in order.
+__global__ __launch_bounds__(BLOCK) void k_list_fill_buckets(uint32_t ncell, const uint32_t* __restrict__ cstart,
+                                                             uint32_t* __restrict__ bcount,
+                                                             uint32_t* __restrict__ bucket, uint32_t bcap,
+                                                             uint32_t* __restrict__ flags) {
+    const uint32_t c = blockIdx.x * (BLOCK / WAVE) + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (c >= ncell) return;
+    const uint32_t s = cstart[c], m = cstart[c + 1] - s;
+    if (lane == 0) {
+        bcount[c] = m;
+        if (m > bcap) atomicOr(&flags[6], 1u);
+    }
+    for (uint32_t e = lane; e < m && e < bcap; e += 64) bucket[(size_t)c * bcap + e] = s + e;
 }
 
 __global__ void k_flag_set(uint32_t* flags, int idx, uint32_t v) { flags[idx] = v; }
 
-// One workgroup per home cell: the 27 stencil segments of cell-ordered
-// indices are staged in LDS and each segment is ordered by atom index (rank
-// by counting), so the lists do not depend on the order in which atomics
-// handed out the tickets; the candidates' positions, shifted by the image
-// offset of their segment, are then staged in LDS chunks and every wave
-// compacts, for one home atom at a time, the candidates within rc + skin
-// with a ballot.  The home atoms' positions are recorded as the build
-// positions of the displacement test.
+// One workgroup per home cell (only when flags[0] is set): the 27 stencil
+// buckets are staged in LDS and each is ordered by atom index (rank by
+// counting); the candidates' positions, shifted by the image offset of
+// their bucket, are then staged in LDS chunks and every wave compacts, for
+// one home atom at a time, the candidates within rc + skin with a ballot.
+// The home atoms' positions are recorded as the build positions of the
+// displacement test.
 __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
-    KParams PL, const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cidx,
+    KParams PL, const uint32_t* __restrict__ bcount, const uint32_t* __restrict__ bucket, uint32_t bcap,
     const double4* __restrict__ pos4, const uint8_t* __restrict__ type, uint32_t* __restrict__ nbr,
     uint32_t* __restrict__ nlen, double4* __restrict__ xref, uint32_t N, uint32_t logT, uint32_t cap, double cut2,
     uint32_t* __restrict__ flags) {
     if (flags[0] == 0u) return;
+    // the candidates' indices are staged, unordered, in the memory of the
+    // position tile, which is filled only after they have been ordered
     __shared__ double4 tile[TILE_CAP];
     __shared__ uint32_t tidx[TILE_CAP];
-    __shared__ uint32_t cand[LDS_IDX_CAP];
     __shared__ uint32_t ord[LDS_IDX_CAP];
     __shared__ uint32_t seg_base[27];
     __shared__ uint32_t seg_pref[28];
     __shared__ double seg_sh[27][3];
+    static_assert(LDS_IDX_CAP * sizeof(uint32_t) <= TILE_CAP * sizeof(double4), "index staging fits the tile");
+    uint32_t* cand = reinterpret_cast<uint32_t*>(tile);
     const uint32_t c = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t h0 = cstart[c], nh = cstart[c + 1] - h0;
+    const uint32_t nh = min(bcount[c], bcap);
     if (nh == 0) return;
+    const uint32_t* home = bucket + (size_t)c * bcap;
     if (tid < 27) {
         const uint32_t nx = PL.dims[0], ny = PL.dims[1], nz = PL.dims[2];
         const int cx = (int)(c % nx), cy = (int)((c / nx) % ny), cz = (int)(c / (nx * ny));
@@ -955,9 +926,8 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
         if (iy < 0) { iy += ny; sy = -PL.box[1]; } else if (iy >= (int)ny) { iy -= ny; sy = PL.box[1]; }
         if (iz < 0) { iz += nz; sz = -PL.box[2]; } else if (iz >= (int)nz) { iz -= nz; sz = PL.box[2]; }
         const uint32_t nc = (uint32_t)ix + nx * ((uint32_t)iy + ny * (uint32_t)iz);
-        const uint32_t b = cstart[nc];
-        seg_base[tid] = b;
-        seg_pref[tid + 1] = cstart[nc + 1] - b;
+        seg_base[tid] = nc * bcap;
+        seg_pref[tid + 1] = min(bcount[nc], bcap);
         seg_sh[tid][0] = sx;
         seg_sh[tid][1] = sy;
         seg_sh[tid][2] = sz;
@@ -976,7 +946,7 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
     for (uint32_t t = tid; t < ncand; t += PAIR_THREADS) {
         int sg = 0;
         while (seg_pref[sg + 1] <= t) ++sg;
-        cand[t] = cidx[seg_base[sg] + (t - seg_pref[sg])];
+        cand[t] = bucket[seg_base[sg] + (t - seg_pref[sg])];
     }
     __syncthreads();
     for (uint32_t t = tid; t < ncand; t += PAIR_THREADS) {
@@ -988,7 +958,7 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
         ord[lo + r] = v;
     }
     for (uint32_t i = tid; i < nh; i += PAIR_THREADS) {
-        const uint32_t gi = cidx[h0 + i];
+        const uint32_t gi = home[i];
         xref[gi] = pos4[gi];
     }
     const int nchunks = (int)((ncand + TILE_CAP - 1) / TILE_CAP);
@@ -1010,7 +980,7 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
         }
         __syncthreads();
         for (uint32_t i = w; i < nh; i += PAIR_WAVES) {
-            const uint32_t gi = cidx[h0 + i];
+            const uint32_t gi = home[i];
             const double4 pi = pos4[gi];
             uint32_t cnt = ch == 0 ? 0u : nlen[gi];
             for (uint32_t base = 0; base < len; base += 64) {
@@ -1090,11 +1060,19 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
         const double iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
         const uint32_t len = nlen[i];
         const PairCoef* row = SINGLE ? sc : sc + ti * GS_NSPECIES;
+        // software pipeline: the entry two iterations ahead and the
+        // position one iteration ahead are in flight during each pair
+        uint32_t e_cur = s < len ? nbr[nbr_slot(s, i, N, LOGT)] : 0u;
+        uint32_t e_nxt = s + T < len ? nbr[nbr_slot(s + T, i, N, LOGT)] : 0u;
+        double4 pj_cur = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (s < len) pj_cur = pos4[e_cur & NBR_IDX_MASK];
         for (uint32_t k = s; k < len; k += T) {
-            const uint32_t e = nbr[nbr_slot(k, i, N, LOGT)];
-            const uint32_t j = e & NBR_IDX_MASK;
+            const uint32_t e = e_cur;
+            const double4 pj = pj_cur;
+            e_cur = e_nxt;
+            if (k + 2 * T < len) e_nxt = nbr[nbr_slot(k + 2 * T, i, N, LOGT)];
+            if (k + T < len) pj_cur = pos4[e_cur & NBR_IDX_MASK];
             const int tj = (int)(e >> 28);
-            const double4 pj = pos4[j];
             double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             dx = fma(-Lx, rint(dx * iLx), dx);
             dy = fma(-Ly, rint(dy * iLy), dy);
@@ -1463,6 +1441,9 @@ struct gs_universe {
     uint32_t* nlen = nullptr;
     double4* xref = nullptr;
     uint32_t* flags = nullptr;  // device list flags and counters (k_list_integrate)
+    uint32_t* bcount = nullptr;  // list-grid cell buckets of a rebuild
+    uint32_t* bucket = nullptr;
+    uint32_t bcap = 0;
     int64_t list_builds = 0;
     // snapshot for replays (allocated on first list-mode step)
     double4* snap_pos = nullptr;
@@ -1487,6 +1468,7 @@ struct gs_universe {
     double* adj_x[3] = {nullptr, nullptr, nullptr};
     // event timing of gs_step
     int timing = 0;  // 0 off, 1 pair kernel only, 2 every phase
+    int timing_every = 1;  // time every k-th tick of a call
     std::vector<hipEvent_t> ev;
     gs_timing tm = {0, 0.0, 0.0, 0.0, 0.0, 0};
 };
@@ -1687,7 +1669,7 @@ bool list_geometry(gs_universe* u, double skin) {
     return true;
 }
 
-int list_alloc(gs_universe* u, uint32_t cap) {
+int list_alloc(gs_universe* u, uint32_t cap, uint32_t bcap_min = 0) {
     int rc;
     const size_t n = (size_t)std::max(1, u->n);
     if (!u->nlen) {
@@ -1698,6 +1680,18 @@ int list_alloc(gs_universe* u, uint32_t cap) {
         for (int a = 0; a < 3; ++a)
             if ((rc = dalloc(&u->snap_v[a], n)) || (rc = dalloc(&u->snap_a[a], n))) return rc;
         HIPCHK(hipMemsetAsync(u->flags, 0, 8 * sizeof(uint32_t), u->stream));
+    }
+    // bucket capacity: twice the mean list-cell occupancy plus 32, doubled
+    // on overflow; the force grid bounds the list grid's cell count
+    const uint32_t want = std::max<uint32_t>(
+        bcap_min, (uint32_t)(2.0 * (double)u->n / (double)std::max<uint32_t>(1, u->kpl.ncell)) + 32u);
+    if (u->bcap < want) {
+        (void)hipFree(u->bucket);
+        u->bucket = nullptr;
+        u->bcap = 0;
+        if (!u->bcount && (rc = dalloc(&u->bcount, (size_t)u->kp.ncell))) return rc;
+        if ((rc = dalloc(&u->bucket, (size_t)u->kp.ncell * want))) return rc;
+        u->bcap = want;
     }
     if (u->nbr_cap < cap) {
         (void)hipFree(u->nbr);
@@ -1778,7 +1772,8 @@ int list_tick_start(gs_universe* u) {
     int rc;
     if (!u->list_valid || u->list_age >= u->sort_every) {
         if ((rc = rebuild(u, true, PL))) return rc;
-        k_iota<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->n, u->perm);
+        k_list_fill_buckets<<<nblocks(PL.ncell, BLOCK / WAVE), BLOCK, 0, u->stream>>>(PL.ncell, u->cstart, u->bcount,
+                                                                                      u->bucket, u->bcap, u->flags);
         k_flag_set<<<1, 1, 0, u->stream>>>(u->flags, 0, 1u);
         u->list_valid = true;
         u->list_age = 0;
@@ -1786,15 +1781,15 @@ int list_tick_start(gs_universe* u) {
     } else {
         const int c = u->cur;
         const double h = 0.5 * u->skin;
-        k_list_integrate<<<nblocks(u->n), BLOCK, 0, u->stream>>>(
-            u->kp, PL, u->pos4[c], u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->scal, u->xref,
-            h * h, u->key, u->ticket, u->count, u->cstart, u->flags);
-        k_list_scatter<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->n, u->key, u->ticket, u->cstart, u->perm, u->flags);
+        k_list_integrate<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->kp, u->pos4[c], u->v[c][0], u->v[c][1],
+                                                                 u->v[c][2], u->a[0], u->a[1], u->a[2], u->scal,
+                                                                 u->xref, h * h, u->bcount, PL.ncell, u->flags);
+        k_list_bin<<<nblocks(u->n), BLOCK, 0, u->stream>>>(PL, u->pos4[c], u->bcount, u->bucket, u->bcap, u->flags);
     }
     const int c = u->cur;
-    k_list_build<<<PL.ncell, PAIR_THREADS, 0, u->stream>>>(PL, u->cstart, u->perm, u->pos4[c], u->type[c], u->nbr,
-                                                           u->nlen, u->xref, N, (uint32_t)u->logT, u->nbr_cap,
-                                                           rl * rl, u->flags);
+    k_list_build<<<PL.ncell, PAIR_THREADS, 0, u->stream>>>(PL, u->bcount, u->bucket, u->bcap, u->pos4[c],
+                                                           u->type[c], u->nbr, u->nlen, u->xref, N,
+                                                           (uint32_t)u->logT, u->nbr_cap, rl * rl, u->flags);
     u->list_age += 1;
     HIPCHK(hipGetLastError());
     return GS_OK;
@@ -1807,26 +1802,27 @@ int run_ticks(gs_universe* u, int64_t n_steps, bool observe, bool list) {
     const gs_params& p = u->prm;
     const bool thermo = p.enable_thermostat != 0;
     for (int64_t s = 0; s < n_steps; ++s) {
-        if (u->timing > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 0], u->stream));
+        const int tmode = (s % u->timing_every) == 0 ? u->timing : 0;
+        if (tmode > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 0], u->stream));
         if (list) {
             if ((rc = list_tick_start(u))) return rc;
-            if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 1], u->stream));
+            if (tmode) HIPCHK(hipEventRecord(u->ev[4 * s + 1], u->stream));
             if ((rc = launch_pair_list(u, observe ? u->obs + s : nullptr))) return rc;
-            if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 2], u->stream));
-            if (u->timing > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
+            if (tmode) HIPCHK(hipEventRecord(u->ev[4 * s + 2], u->stream));
+            if (tmode > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
             continue;
         }
         if ((rc = rebuild(u, true, u->kp))) return rc;
         u->list_valid = false;
-        if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 1], u->stream));
+        if (tmode) HIPCHK(hipEventRecord(u->ev[4 * s + 1], u->stream));
         if ((rc = launch_pair<false>(u))) return rc;
-        if (u->timing) HIPCHK(hipEventRecord(u->ev[4 * s + 2], u->stream));
+        if (tmode) HIPCHK(hipEventRecord(u->ev[4 * s + 2], u->stream));
         const int c = u->cur;
         k_kick_ke<true><<<nblocks(u->n), BLOCK, 0, u->stream>>>(
             u->kp, u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->type[c], u->kpart, u->counter,
             u->vpart, u->nvpart, u->scal, observe ? u->obs + s : nullptr, thermo ? 1 : 0, p.thermostat_temperature,
             std::fmax(p.thermostat_tau, 1.0), p.thermal_constant);
-        if (u->timing > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
+        if (tmode > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
     }
     HIPCHK(hipGetLastError());
     return GS_OK;
@@ -1944,6 +1940,8 @@ int gs_destroy(gs_universe* u) {
     (void)hipFree(u->nlen);
     (void)hipFree(u->xref);
     (void)hipFree(u->flags);
+    (void)hipFree(u->bcount);
+    (void)hipFree(u->bucket);
     (void)hipFree(u->snap_pos);
     for (int a = 0; a < 3; ++a) (void)hipFree(u->snap_v[a]);
     for (int a = 0; a < 3; ++a) (void)hipFree(u->snap_a[a]);
@@ -2196,10 +2194,11 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
         uint32_t fl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (list) HIPCHK(hipMemcpyAsync(fl, u->flags, sizeof(fl), hipMemcpyDeviceToHost, u->stream));
         HIPCHK(hipStreamSynchronize(u->stream));
-        if (list && (fl[1] || fl[2])) {
-            // A list overflowed its capacity, or a stencil held more candidates
-            // than the build stages in LDS: replay the call from the snapshot
-            // with larger lists, or on the per-tick cell tiles.
+        if (list && (fl[1] || fl[2] || fl[6])) {
+            // A list or a cell bucket overflowed its capacity, or a stencil
+            // held more candidates than the build stages in LDS: replay the
+            // call from the snapshot with larger lists or buckets, or on the
+            // per-tick cell tiles.
             if (attempt > 8) return fail(GS_E_OVERFLOW, "neighbour list could not be made valid");
             u->cur = cur0;
             if ((rc = snapshot(u, true))) return rc;
@@ -2208,9 +2207,9 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
                 u->pair_mode = 0;
                 list = false;
             } else {
-                const uint32_t cap = u->nbr_cap * 2;
+                const uint32_t cap = fl[1] ? u->nbr_cap * 2 : u->nbr_cap;
                 if (cap > (1u << 16)) return fail(GS_E_OVERFLOW, "neighbour list exceeds 65536 entries per atom");
-                if ((rc = list_alloc(u, cap))) return rc;
+                if ((rc = list_alloc(u, cap, fl[6] ? 2 * u->bcap : 0))) return rc;
             }
             continue;
         }
@@ -2221,8 +2220,10 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
         HIPCHK(hipMemcpy(obs, u->obs, sizeof(gs_observables) * (size_t)n_steps, hipMemcpyDeviceToHost));
     u->extras_valid = false;
     if (u->timing) {
-        for (int64_t s = 0; s < n_steps; ++s) {
+        int64_t timed = 0;
+        for (int64_t s = 0; s < n_steps; s += u->timing_every) {
             float a = 0.f, b = 0.f, c2 = 0.f;
+            ++timed;
             HIPCHK(hipEventElapsedTime(&b, u->ev[4 * s + 1], u->ev[4 * s + 2]));
             if (u->timing > 1) {
                 HIPCHK(hipEventElapsedTime(&a, u->ev[4 * s + 0], u->ev[4 * s + 1]));
@@ -2233,7 +2234,7 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
             u->tm.kick_ms += c2;
             u->tm.step_ms += (double)a + b + c2;
         }
-        u->tm.steps += n_steps;
+        u->tm.steps += timed;
     }
     return GS_OK;
 }
@@ -2325,10 +2326,12 @@ int gs_autotune(gs_universe* u, int32_t trial_steps, gs_tuning* best) {
     return GS_OK;
 }
 
-int gs_timing_enable(gs_universe* u, int on) {
+int gs_timing_enable(gs_universe* u, int on, int every) {
     if (!u) return fail(GS_E_INVALID, "universe handle is NULL");
     if (on < 0 || on > 2) return fail(GS_E_INVALID, "timing mode must be 0, 1 or 2");
+    if (every < 1) return fail(GS_E_INVALID, "timing stride must be >= 1");
     u->timing = on;
+    u->timing_every = every;
     u->tm = gs_timing{0, 0.0, 0.0, 0.0, 0.0, 0};
     return GS_OK;
 }

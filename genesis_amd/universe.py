@@ -144,13 +144,13 @@ class GpuUniverse:
         _lib.check(self.lib.gs_autotune(self.h, trial_steps, ctypes.byref(t)))
         return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}
 
-    def timing(self, enable=None) -> dict:
+    def timing(self, enable=None, every: int = 1) -> dict:
         """Event timing of step(): enable=True or "phases" times every phase (four
-        events per tick), "pair" only the pair kernel (two), False stops; enabling
-        resets the counters."""
+        events per timed tick), "pair" only the pair kernel (two), False stops;
+        every k-th tick of each step() call is timed.  Enabling resets the counters."""
         if enable is not None:
             mode = {False: 0, True: 2, "phases": 2, "pair": 1}[enable]
-            _lib.check(self.lib.gs_timing_enable(self.h, mode))
+            _lib.check(self.lib.gs_timing_enable(self.h, mode, every))
         t = _lib.Timing()
         _lib.check(self.lib.gs_timing_read(self.h, ctypes.byref(t)))
         return {f: getattr(t, f) for f, _ in _lib.Timing._fields_}

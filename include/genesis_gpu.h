@@ -194,9 +194,12 @@ int gs_set_tuning(gs_universe* u, const gs_tuning* t);
 int gs_autotune(gs_universe* u, int32_t trial_steps, gs_tuning* best);
 
 /* Event timing of gs_step, reset on every call: on = 0 off, 1 the pair
- * kernel only (two events per tick), 2 every phase (four events per tick;
- * rebuild_ms and kick_ms stay 0 in mode 1). */
-int gs_timing_enable(gs_universe* u, int on);
+ * kernel only (two events per timed tick), 2 every phase (four events per
+ * timed tick; rebuild_ms and kick_ms stay 0 in mode 1).  Every `every`-th
+ * tick of a gs_step call is timed (tick 0, every, 2 every, ...), which
+ * bounds the cost of the event records; gs_timing.steps counts the timed
+ * ticks. */
+int gs_timing_enable(gs_universe* u, int on, int every);
 int gs_timing_read(gs_universe* u, gs_timing* out);
 
 /* ---- observation lenses ---------------------------------------------------- */
