@@ -237,7 +237,7 @@ __device__ __forceinline__ void switch_q(double r, double r_on, double inv_w, do
 // Coulomb term with r = max(d, 0.5) switched at d, and gravity.
 template <bool COUL, bool SHARED>
 __device__ __forceinline__ void pair_law(const KParams& P, const PairCoef& pc, double d, double inv_d, double qq,
-                                         double mm, double& fmag, double& vv) {
+                                         double& fmag, double& vv) {
     const bool core = d < pc.rcore;
     const double r = core ? pc.rcore : d;
     const double inv_r = core ? pc.inv_rcore : inv_d;
@@ -274,7 +274,7 @@ __device__ __forceinline__ void pair_law(const KParams& P, const PairCoef& pc, d
     if (P.flags & F_GRAVITY) {
         const bool cc = d < 0.5;
         const double inv_g = cc ? 2.0 : inv_d;
-        const double vg = -(P.gconst * mm) * inv_g;
+        const double vg = -(P.gconst * pc.mm) * inv_g;
         const double fg = vg * inv_g;
         double swg, dswg;
         switch_q(d, P.g_ron, P.g_inv_w, swg, dswg);
@@ -380,7 +380,6 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_pair(KParams Pk, const uint32_
             const double4 pi = pos4[h0 + i];
             const int ti = type[h0 + i];
             const double kqi = P.ke * pi.w;
-            const double mi = P.mass[ti];
             double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, sa = 0.0;
             uint32_t qn = 0, qh = 0;
 
@@ -395,7 +394,7 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_pair(KParams Pk, const uint32_
                     const int tj = ttype[j];
                     const PairCoef& pc = SINGLE ? sc[0] : sc[ti * GS_NSPECIES + tj];
                     double fmag, vv;
-                    pair_law<COUL, SHARED>(P, pc, d, inv_d, kqi * pj.w, mi * P.mass[tj], fmag, vv);
+                    pair_law<COUL, SHARED>(P, pc, d, inv_d, kqi * pj.w, fmag, vv);
                     const double fs = fmag * inv_d;
                     fx = fma(fs, dx, fx);
                     fy = fma(fs, dy, fy);
@@ -533,8 +532,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair_brute(KParams P, const double4* 
                 const double d = r2 * inv_d;
                 const int tj = tt[jj];
                 double fmag, vv;
-                pair_law<COUL, false>(P, sc[ti * GS_NSPECIES + tj], d, inv_d, P.ke * pi.w * pj.w,
-                                      P.mass[ti] * P.mass[tj], fmag, vv);
+                pair_law<COUL, false>(P, sc[ti * GS_NSPECIES + tj], d, inv_d, P.ke * pi.w * pj.w, fmag, vv);
                 fmag *= mult;
                 vv *= mult;
                 const double fs = fmag * inv_d;
@@ -781,10 +779,14 @@ constexpr uint32_t NBR_IDX_MASK = 0x0FFFFFFFu;  // entry = (species << 28) | sor
 constexpr int LDS_IDX_CAP = 2048;               // stencil candidates held (and ordered) in LDS
 constexpr int MAX_LOGT = 4;                     // up to 16 lanes per atom
 
-// Entry k of atom i with T lanes per atom: row k/T of an [rows][N][T] array,
-// so the T lanes of consecutive atoms read consecutive words.
+// Entry k of atom i with T lanes per atom: lane s = k mod T owns entries
+// s, s+T, s+2T, ...; its m-th entry (m = k / T) sits in group m/4, slot
+// m mod 4 of an [groups][N][T][4] array, so a lane reads four entries with
+// one 16-byte load and the lanes of consecutive atoms read consecutive
+// 16-byte words.
 __device__ __forceinline__ size_t nbr_slot(uint32_t k, uint32_t i, uint32_t N, uint32_t logT) {
-    return (((size_t)(k >> logT) * N + i) << logT) + (k & ((1u << logT) - 1u));
+    const uint32_t m = k >> logT, s = k & ((1u << logT) - 1u);
+    return ((((size_t)(m >> 2) * N + i) << logT) + s) * 4u + (m & 3u);
 }
 
 // List-mode tick start: thermostat scale, half kick, drift, wrap (the same
@@ -1055,41 +1057,48 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
         const double4 pi = pos4[i];
         ti = type[i];
         const double kqi = P.ke * pi.w;
-        const double mi = P.mass[ti];
         const double Lx = P.box[0], Ly = P.box[1], Lz = P.box[2];
         const double iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
         const uint32_t len = nlen[i];
         const PairCoef* row = SINGLE ? sc : sc + ti * GS_NSPECIES;
-        // software pipeline: the entry two iterations ahead and the
-        // position one iteration ahead are in flight during each pair
-        uint32_t e_cur = s < len ? nbr[nbr_slot(s, i, N, LOGT)] : 0u;
-        uint32_t e_nxt = s + T < len ? nbr[nbr_slot(s + T, i, N, LOGT)] : 0u;
-        double4 pj_cur = make_double4(0.0, 0.0, 0.0, 0.0);
-        if (s < len) pj_cur = pos4[e_cur & NBR_IDX_MASK];
-        for (uint32_t k = s; k < len; k += T) {
-            const uint32_t e = e_cur;
-            const double4 pj = pj_cur;
-            e_cur = e_nxt;
-            if (k + 2 * T < len) e_nxt = nbr[nbr_slot(k + 2 * T, i, N, LOGT)];
-            if (k + T < len) pj_cur = pos4[e_cur & NBR_IDX_MASK];
-            const int tj = (int)(e >> 28);
-            double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            dx = fma(-Lx, rint(dx * iLx), dx);
-            dy = fma(-Ly, rint(dy * iLy), dy);
-            dz = fma(-Lz, rint(dz * iLz), dz);
-            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-            const bool ok = r2 > P.eps2;
-            const double r2s = ok ? r2 : 1.0;
-            const double inv_d = rsqrt_refined(r2s);
-            const double d = r2s * inv_d;
-            const PairCoef& pc = SINGLE ? row[0] : row[tj];
-            double fmag, vv;
-            pair_law<COUL, SHARED>(P, pc, d, inv_d, kqi * pj.w, mi * P.mass[tj], fmag, vv);
-            const double fs = ok ? fmag * inv_d : 0.0;
-            fx = fma(fs, dx, fx);
-            fy = fma(fs, dy, fy);
-            fz = fma(fs, dz, fz);
-            vl += ok ? vv : 0.0;
+        // this lane's entries: s, s+T, ... (cnt of them), four per group;
+        // a group's four position gathers are all in flight before its four
+        // pairs are evaluated
+        const uint32_t cnt = len > s ? (len - s + T - 1) >> LOGT : 0u;
+        const uint4* grp = reinterpret_cast<const uint4*>(nbr) + (((size_t)i << LOGT) + s);
+        const size_t gstride = (size_t)N << LOGT;
+        for (uint32_t m0 = 0; m0 < cnt; m0 += 4, grp += gstride) {
+            const uint4 ev = *grp;
+            const uint32_t e4[4] = {ev.x, ev.y, ev.z, ev.w};
+            double4 pj4[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t jj = m0 + q < cnt ? (e4[q] & NBR_IDX_MASK) : i;  // past the end: self, masked
+                pj4[q] = pos4[jj];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const bool live = m0 + q < cnt;
+                const int tj = live ? (int)(e4[q] >> 28) : 0;
+                const double4 pj = pj4[q];
+                double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                dx = fma(-Lx, rint(dx * iLx), dx);
+                dy = fma(-Ly, rint(dy * iLy), dy);
+                dz = fma(-Lz, rint(dz * iLz), dz);
+                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                const bool ok = live && r2 > P.eps2;
+                const double r2s = ok ? r2 : 1.0;
+                const double inv_d = rsqrt_refined(r2s);
+                const double d = r2s * inv_d;
+                const PairCoef& pc = SINGLE ? row[0] : row[tj];
+                double fmag, vv;
+                pair_law<COUL, SHARED>(P, pc, d, inv_d, kqi * pj.w, fmag, vv);
+                const double fs = ok ? fmag * inv_d : 0.0;
+                fx = fma(fs, dx, fx);
+                fy = fma(fs, dy, fy);
+                fz = fma(fs, dz, fz);
+                vl += ok ? vv : 0.0;
+            }
         }
     }
 #pragma unroll
@@ -1184,8 +1193,9 @@ __device__ __forceinline__ int axis_offsets(uint32_t dims, bool refm, int* o) {
 // g(r) pair counts (radial_distribution, src/analysis/mod.rs:103-117): every
 // unordered pair of the subsample found by the grid of edge r_max, counted
 // from its lower-index end, with the reference's min_image, length and
-// (d / dr) as usize.  Integer counts are order independent, so the u32 LDS
-// histogram with atomics is exact.
+// (d / dr) as usize.  One wave per point, its lanes strided over each
+// stencil cell's members.  Integer counts are order independent, so the
+// u32 LDS histogram with atomics is exact.
 template <bool BETWEEN>
 __global__ __launch_bounds__(BLOCK) void k_gr_pairs(int n, AuxGrid G, KParams P, const double4* __restrict__ pts,
                                                     const uint8_t* __restrict__ ptype,
@@ -1198,7 +1208,8 @@ __global__ __launch_bounds__(BLOCK) void k_gr_pairs(int n, AuxGrid G, KParams P,
     if (lds)
         for (int b = threadIdx.x; b < nbins; b += BLOCK) hist[b] = 0u;
     __syncthreads();
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * (BLOCK / WAVE) + (threadIdx.x >> 6);
     if (i < n) {
         const double4 p = pts[i];
         const int ti = BETWEEN ? ptype[i] : 0;
@@ -1215,7 +1226,7 @@ __global__ __launch_bounds__(BLOCK) void k_gr_pairs(int n, AuxGrid G, KParams P,
                     const uint32_t iy = (c[1] + G.dims[1] + oy[b]) % G.dims[1];
                     const uint32_t iz = (c[2] + G.dims[2] + oz[a]) % G.dims[2];
                     const uint32_t nc = ix + G.dims[0] * (iy + G.dims[1] * iz);
-                    for (uint32_t t = start[nc]; t < start[nc + 1]; ++t) {
+                    for (uint32_t t = start[nc] + lane; t < start[nc + 1]; t += WAVE) {
                         const uint32_t j = perm[t];
                         if ((int)j <= i) continue;
                         if (BETWEEN) {
@@ -1516,9 +1527,9 @@ std::vector<PairCoef> derive(gs_universe* u) {
             c.c24 = 24.0 * eps / sig;
             c.rcore = 0.5 * sig;
             c.inv_rcore = 1.0 / c.rcore;
+            c.mm = p.mass[i] * p.mass[j];
             c.bond_k = 72.0 * c2m13 * eps / (sig * sig);
             c.bond_req = sig * c2p16;
-            c.bond_rc = p.bond_k_bind * sig;
             max_rcore = std::fmax(max_rcore, c.rcore);
         }
     const double lj_rc = std::fmax(p.lj_cutoff_factor, 1.0) * max_pair_sigma;
@@ -1697,9 +1708,9 @@ int list_alloc(gs_universe* u, uint32_t cap, uint32_t bcap_min = 0) {
         (void)hipFree(u->nbr);
         u->nbr = nullptr;
         u->nbr_cap = 0;
-        // rows of 16 lanes so any lanes-per-atom setting fits
-        const size_t rows = ((size_t)cap + 15) / 16 + 1;
-        if ((rc = dalloc(&u->nbr, rows * 16 * n))) return rc;
+        // groups of 4 entries per lane: (cap + 4 T) entries per atom cover any
+        // lanes-per-atom setting up to 16
+        if ((rc = dalloc(&u->nbr, ((size_t)cap + (4u << MAX_LOGT)) * n))) return rc;
         u->nbr_cap = cap;
     }
     return GS_OK;
@@ -2442,11 +2453,11 @@ int radial_impl(gs_universe* u, bool between, int ta, int tb, double r_max_in, i
         const int refm = u->prm.grid_semantics == GS_GRID_REFERENCE;
         const size_t shm = nbins <= 4096 ? sizeof(uint32_t) * (size_t)nbins : 0;
         if (between)
-            k_gr_pairs<true><<<nblocks(m), BLOCK, shm, u->stream>>>((int)m, G, P, u->l_pts, u->l_type, u->l_start,
+            k_gr_pairs<true><<<nblocks(m, BLOCK / WAVE), BLOCK, shm, u->stream>>>((int)m, G, P, u->l_pts, u->l_type, u->l_start,
                                                                     u->l_perm, r_max, dr, nbins, ta, tb, refm,
                                                                     u->l_bins);
         else
-            k_gr_pairs<false><<<nblocks(m), BLOCK, shm, u->stream>>>((int)m, G, P, u->l_pts, u->l_type, u->l_start,
+            k_gr_pairs<false><<<nblocks(m, BLOCK / WAVE), BLOCK, shm, u->stream>>>((int)m, G, P, u->l_pts, u->l_type, u->l_start,
                                                                      u->l_perm, r_max, dr, nbins, ta, tb, refm,
                                                                      u->l_bins);
         HIPCHK(hipGetLastError());

@@ -78,9 +78,9 @@ struct PairCoef {
     double c24;        // 24 eps_ij / sigma_ij
     double rcore;      // LJ_NUCLEUS * sigma_ij (hardened core radius)
     double inv_rcore;  // 1 / rcore
+    double mm;         // m_i m_j (gravity)
     double bond_k;     // well_curvature(eps_ij, sigma_ij)
     double bond_req;   // equilibrium_distance(sigma_ij)
-    double bond_rc;    // bond_k_bind * sigma_ij
 };
 
 // Per-species-pair parameters of the differentiable model (adjoint).
