@@ -635,6 +635,125 @@ std::vector<std::vector<uint32_t>> load_bonds(int64_t n, const int32_t* ptr, con
     return bond_of;
 }
 
+
+// ---- BondObservationSystem: src/systems/bonds.rs, src/analysis/pairs.rs -----
+
+constexpr uint64_t DEBOUNCE = 2;  // src/systems/bonds.rs:27
+
+using IdPair = std::pair<uint32_t, uint32_t>;  // BoundPair, ids normalised a < b
+
+// PairTracker::track_tick, src/analysis/pairs.rs:64-90.  Episodes closed by
+// the debounce are not kept (only the observation reads the tracker).
+struct PairTracker {
+    uint64_t debounce = DEBOUNCE;
+    std::map<IdPair, uint64_t> open, missing;
+    void track_tick(const std::vector<IdPair>& bound) {
+        std::set<IdPair> current(bound.begin(), bound.end());
+        for (const IdPair& pr : current) {
+            missing.erase(pr);
+            open[pr] += 1;  // and_modify(+1).or_insert(1)
+        }
+        std::vector<IdPair> absent;
+        for (const auto& kv : open)
+            if (!current.count(kv.first)) absent.push_back(kv.first);
+        for (const IdPair& pr : absent) {
+            uint64_t& m = missing[pr];
+            m += 1;
+            if (m >= debounce) {
+                open.erase(pr);
+                missing.erase(pr);
+            }
+        }
+    }
+};
+
+// BondObservationSystem state and its BondObservation summary
+// (src/systems/bonds.rs:29-40, src/stats bond fields).
+struct BondObserver {
+    PairTracker tracker;
+    std::map<IdPair, uint64_t> active_since;
+    std::vector<std::vector<uint32_t>> bonds;  // Bonds component, per slot
+    uint64_t bonds_formed = 0;
+    double lifetime_sum_ticks = 0.0;
+    uint64_t tick = 0, bonded_pairs = 0, bonded_entities = 0;
+    double mean_coordination = 0.0;
+    std::vector<uint64_t> matrix = std::vector<uint64_t>(GS_NSPECIES * GS_NSPECIES, 0);
+};
+
+// bind_cutoff, src/analysis/pairs.rs:115-117 (max over all ten elements).
+double bind_cutoff(const gs_params& p) {
+    double m = 0.0;
+    for (int t = 0; t < GS_NSPECIES; ++t) m = fmax_r(m, p.sigma[t]);
+    return p.bond_k_bind * m;
+}
+
+// bound_pairs_with_grid, src/analysis/pairs.rs:132-174: grid candidates
+// within the bind cutoff (check_pair: d2 < cutoff^2, d > EPS), kept when
+// |min_image(a - b)| < k_bind * sigma_ab.  Slot index = entity id.
+void bound_pairs(const gs_params& p, const std::vector<V3>& pos, const uint8_t* types, std::vector<IdPair>& out) {
+    out.clear();
+    std::vector<Pair> cand;
+    enumerate_pairs(p, pos, bind_cutoff(p), cand);
+    V3 size = v3(p.box[0], p.box[1], p.box[2]);
+    for (const Pair& pr : cand) {
+        double d = std::sqrt(len2(min_image(pos[pr.a] - pos[pr.b], size)));
+        double thr = p.bond_k_bind * (0.5 * (p.sigma[types[pr.a]] + p.sigma[types[pr.b]]));
+        if (d < thr) out.emplace_back(std::min(pr.a, pr.b), std::max(pr.a, pr.b));
+    }
+}
+
+// Persistence threshold of a species pair, src/systems/bonds.rs:92-97:
+// ceil(min_periods * vib_period(eps_ab, sigma_ab, mu_ab) / dt) as u64.
+uint64_t persistence_ticks(const gs_params& p, int a, int b) {
+    double mu = p.mass[a] * p.mass[b] / (p.mass[a] + p.mass[b]);
+    double eps = p.thermal_constant * std::sqrt(p.epsilon_k[a] * p.epsilon_k[b]);
+    double sig = 0.5 * (p.sigma[a] + p.sigma[b]);
+    return f64_as_u64(std::ceil(fmax_r(p.bond_min_periods, 0.1) * vib_period(eps, sig, mu) / p.dt));
+}
+
+// BondObservationSystem::run, src/systems/bonds.rs:73-185, at tick `now`.
+void bond_observe(const gs_params& p, const std::vector<V3>& pos, const uint8_t* types, uint64_t now,
+                  BondObserver& ob) {
+    std::vector<IdPair> bound;
+    bound_pairs(p, pos, types, bound);
+    ob.tracker.track_tick(bound);
+    std::map<uint32_t, std::vector<uint32_t>> persistent;
+    for (const auto& kv : ob.tracker.open) {
+        const IdPair& pr = kv.first;
+        if (kv.second < persistence_ticks(p, types[pr.first], types[pr.second])) continue;
+        persistent[pr.first].push_back(pr.second);
+        persistent[pr.second].push_back(pr.first);
+    }
+    ob.bonds.assign(pos.size(), std::vector<uint32_t>());
+    for (auto& kv : persistent) ob.bonds[kv.first] = kv.second;
+    uint64_t bonded_pairs = 0;
+    std::vector<uint64_t> matrix(GS_NSPECIES * GS_NSPECIES, 0);
+    std::set<IdPair> current;
+    for (auto& kv : persistent)
+        for (uint32_t m : kv.second) {
+            IdPair pr(std::min(kv.first, m), std::max(kv.first, m));
+            if (!current.insert(pr).second) continue;
+            bonded_pairs += 1;
+            int ia = types[kv.first], ib = types[m];
+            matrix[ia * GS_NSPECIES + ib] += 1;
+            matrix[ib * GS_NSPECIES + ia] += 1;
+        }
+    std::vector<IdPair> broken;
+    for (const auto& kv : ob.active_since)
+        if (!current.count(kv.first)) broken.push_back(kv.first);
+    for (const IdPair& pr : broken) {
+        ob.bonds_formed += 1;
+        ob.lifetime_sum_ticks += (double)(now - ob.active_since[pr]);
+        ob.active_since.erase(pr);
+    }
+    for (const IdPair& pr : current) ob.active_since.emplace(pr, now);
+    ob.tick = now;
+    ob.bonded_pairs = bonded_pairs;
+    ob.bonded_entities = persistent.size();
+    ob.mean_coordination = ob.bonded_entities > 0 ? 2.0 * (double)bonded_pairs / (double)ob.bonded_entities : 0.0;
+    ob.matrix = matrix;
+}
+
 }  // namespace
 
 // =============================================================================
@@ -794,22 +913,45 @@ double or_thermostat(const gs_params* p, int64_t n, double* vx, double* vy, doub
 
 // Universe::tick x nsteps with the schedule of build_schedule
 // (src/universe/mod.rs:141-159,355-392): kick, drift, [wrap], forces, kick,
-// [thermostat], stats.  obs: 6 doubles per step (K, V, Vbond, T, lambda, E),
-// may be NULL.  State arrays are updated in place.
-int or_run(const gs_params* p, int64_t n, double* x, double* y, double* z, double* vx, double* vy, double* vz,
-           double* ax, double* ay, double* az, const uint8_t* species, const double* charge, int64_t nsteps,
-           int nthreads, double* obs) {
+// [thermostat], [bond observation], stats.  obs: 6 doubles per step (K, V,
+// Vbond, T, lambda, E), may be NULL.  State arrays are updated in place.
+// With bond observation on, the Bonds written at tick n feed the force pass
+// of tick n + 1 (the tracker starts empty, at tick tick0 + 1); bstats gets
+// per step: tick, bonded_pairs, bonded_entities, bonds_formed,
+// lifetime_sum_ticks, mean_coordination, then the 100 species-matrix counts
+// (106 doubles), and bond_ptr / bond_idx (n + 1 and up to bond_cap entries)
+// the final Bonds adjacency, neighbours ascending.
+int or_run_observed(const gs_params* p, int64_t n, double* x, double* y, double* z, double* vx, double* vy,
+                    double* vz, double* ax, double* ay, double* az, const uint8_t* species, const double* charge,
+                    int64_t nsteps, int nthreads, double* obs, int64_t tick0, double* bstats, int32_t* bond_ptr,
+                    int32_t* bond_idx, int64_t bond_cap) {
     std::vector<V3> pos = load_pos(n, x, y, z), vel = load_pos(n, vx, vy, vz), acc = load_pos(n, ax, ay, az);
     ForceOut out;
+    BondObserver ob;
+    ob.bonds.assign((size_t)n, std::vector<uint32_t>());
+    const bool observe_bonds = p->enable_bond_observation != 0;
     for (int64_t s = 0; s < nsteps; ++s) {
         half_kick(vel, acc, p->dt);
         drift(pos, vel, p->dt);
         if (p->enable_boundaries) wrap(pos, p->box);
-        force_pass(*p, pos, species, charge, nullptr, nthreads, false, out);
+        force_pass(*p, pos, species, charge, p->enable_bond_interaction ? &ob.bonds : nullptr, nthreads, false, out);
         acc = out.acc;
         half_kick(vel, acc, p->dt);
         double lambda = 1.0;
         if (p->enable_thermostat) lambda = thermostat(*p, vel, species);
+        if (observe_bonds) {
+            bond_observe(*p, pos, species, (uint64_t)(tick0 + s + 1), ob);
+            if (bstats) {
+                double* b = bstats + 106 * s;
+                b[0] = (double)ob.tick;
+                b[1] = (double)ob.bonded_pairs;
+                b[2] = (double)ob.bonded_entities;
+                b[3] = (double)ob.bonds_formed;
+                b[4] = ob.lifetime_sum_ticks;
+                b[5] = ob.mean_coordination;
+                for (int k = 0; k < GS_NSPECIES * GS_NSPECIES; ++k) b[6 + k] = (double)ob.matrix[k];
+            }
+        }
         if (obs) {
             double k = kinetic(vel, species, p->mass);
             double t = p->thermal_constant > 0.0 ? (2.0 / 3.0) * (k / (double)n) / p->thermal_constant : 0.0;
@@ -827,7 +969,94 @@ int or_run(const gs_params* p, int64_t n, double* x, double* y, double* z, doubl
         vx[i] = vel[i].x; vy[i] = vel[i].y; vz[i] = vel[i].z;
         ax[i] = acc[i].x; ay[i] = acc[i].y; az[i] = acc[i].z;
     }
+    if (bond_ptr) {
+        int64_t off = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            bond_ptr[i] = (int32_t)off;
+            std::vector<uint32_t> nb = ob.bonds[(size_t)i];
+            std::sort(nb.begin(), nb.end());
+            for (uint32_t m : nb) {
+                if (off >= bond_cap) return -1;
+                bond_idx[off++] = (int32_t)m;
+            }
+        }
+        bond_ptr[n] = (int32_t)off;
+    }
     return 0;
+}
+
+int or_run(const gs_params* p, int64_t n, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+           double* ax, double* ay, double* az, const uint8_t* species, const double* charge, int64_t nsteps,
+           int nthreads, double* obs) {
+    return or_run_observed(p, n, x, y, z, vx, vy, vz, ax, ay, az, species, charge, nsteps, nthreads, obs, 0, nullptr,
+                           nullptr, nullptr, 0);
+}
+
+// BondObservationSystem::run alone, driven by explicit position frames (the
+// way its own test drives it, src/systems/bonds.rs:263-343): frame f is
+// observed at tick tick0 + f + 1.  bstats as in or_run_observed.
+int or_observe_frames(const gs_params* p, int64_t n, int64_t nframes, const double* x, const double* y,
+                      const double* z, const uint8_t* species, int64_t tick0, double* bstats) {
+    BondObserver ob;
+    for (int64_t f = 0; f < nframes; ++f) {
+        std::vector<V3> pos = load_pos(n, x + f * n, y + f * n, z + f * n);
+        bond_observe(*p, pos, species, (uint64_t)(tick0 + f + 1), ob);
+        double* b = bstats + 106 * f;
+        b[0] = (double)ob.tick;
+        b[1] = (double)ob.bonded_pairs;
+        b[2] = (double)ob.bonded_entities;
+        b[3] = (double)ob.bonds_formed;
+        b[4] = ob.lifetime_sum_ticks;
+        b[5] = ob.mean_coordination;
+        for (int k = 0; k < GS_NSPECIES * GS_NSPECIES; ++k) b[6 + k] = (double)ob.matrix[k];
+    }
+    return 0;
+}
+
+// bound_pairs_with_grid on one state: the normalised (a < b) pairs, sorted.
+int64_t or_bound_pairs(const gs_params* p, int64_t n, const double* x, const double* y, const double* z,
+                       const uint8_t* species, uint32_t* pa, uint32_t* pb, int64_t cap) {
+    std::vector<V3> pos = load_pos(n, x, y, z);
+    std::vector<IdPair> b;
+    bound_pairs(*p, pos, species, b);
+    std::set<IdPair> s(b.begin(), b.end());
+    int64_t k = 0;
+    for (const IdPair& pr : s) {
+        if (k < cap) {
+            pa[k] = pr.first;
+            pb[k] = pr.second;
+        }
+        ++k;
+    }
+    return k;
+}
+
+// persistence threshold (ticks) of a species pair.
+uint64_t or_persistence_ticks(const gs_params* p, int a, int b) { return persistence_ticks(*p, a, b); }
+
+// PairTracker fed with explicit per-tick pair lists (the tracker's own
+// unit-test interface): ticks concatenated in pa/pb with per-tick counts;
+// returns the open count after the last tick and writes open (pair, ticks).
+int64_t or_track(int64_t nticks, const int64_t* counts, const uint32_t* pa, const uint32_t* pb, uint64_t debounce,
+                 uint32_t* open_a, uint32_t* open_b, uint64_t* open_ticks, int64_t cap) {
+    PairTracker t;
+    t.debounce = debounce < 1 ? 1 : debounce;
+    int64_t off = 0;
+    for (int64_t k = 0; k < nticks; ++k) {
+        std::vector<IdPair> b;
+        for (int64_t e = 0; e < counts[k]; ++e, ++off) b.emplace_back(pa[off], pb[off]);
+        t.track_tick(b);
+    }
+    int64_t m = 0;
+    for (const auto& kv : t.open) {
+        if (m < cap) {
+            open_a[m] = kv.first.first;
+            open_b[m] = kv.first.second;
+            open_ticks[m] = kv.second;
+        }
+        ++m;
+    }
+    return m;
 }
 
 // radial_distribution / radial_distribution_between.  species may be NULL
@@ -965,9 +1194,12 @@ int or_bond_components(int64_t ne, const uint32_t* ent_id, const uint8_t* ent_sp
         o.formula = f;
         out.push_back(o);
     }
-    std::stable_sort(out.begin(), out.end(), [](const Out& a, const Out& b) {
+    // size desc, then formula (src/analysis/mod.rs:411); components equal on
+    // both keep the reference's hash order there, here the smallest member
+    std::sort(out.begin(), out.end(), [](const Out& a, const Out& b) {
         if (a.members.size() != b.members.size()) return a.members.size() > b.members.size();
-        return a.formula < b.formula;
+        if (a.formula != b.formula) return a.formula < b.formula;
+        return a.members.front() < b.members.front();
     });
     int64_t off = 0;
     for (size_t k = 0; k < out.size(); ++k) {

@@ -73,6 +73,14 @@ def lib():
                                           _D, _D, _D]),
             "or_bond_components": (ctypes.c_int, [ctypes.c_int64, _U32, _U8, ctypes.c_int64, _U32, _U32, _I64, _U32,
                                                   ctypes.c_char_p, ctypes.c_int64]),
+            "or_run_observed": (ctypes.c_int, [P, ctypes.c_int64] + [_D] * 9 + [_U8, _D, ctypes.c_int64, ctypes.c_int,
+                                               _D, ctypes.c_int64, _D, _I32, _I32, ctypes.c_int64]),
+            "or_observe_frames": (ctypes.c_int, [P, ctypes.c_int64, ctypes.c_int64, _D, _D, _D, _U8, ctypes.c_int64,
+                                                 _D]),
+            "or_bound_pairs": (ctypes.c_int64, [P, ctypes.c_int64, _D, _D, _D, _U8, _U32, _U32, ctypes.c_int64]),
+            "or_persistence_ticks": (ctypes.c_uint64, [P, ctypes.c_int, ctypes.c_int]),
+            "or_track": (ctypes.c_int64, [ctypes.c_int64, _I64, _U32, _U32, ctypes.c_uint64, _U32, _U32, _U64,
+                                          ctypes.c_int64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -206,6 +214,84 @@ def run(physics, state, nsteps, nthreads=1, observe=True):
                  _p(obs))
     s.x, s.y, s.z, s.vx, s.vy, s.vz, s.ax, s.ay, s.az = arrs
     return s, (obs[:nsteps] if observe else None)
+
+
+BOND_STAT_NAMES = ("tick", "bonded_pairs", "bonded_entities", "bonds_formed", "lifetime_sum_ticks",
+                   "mean_coordination")
+
+
+def run_observed(physics, state, nsteps, nthreads=1, tick0=0, bond_cap=None):
+    """run() with the bond observation of every tick (physics.enable_bond_observation):
+    returns (state, obs, bstats, (ptr, idx)) where bstats is (nsteps, 106): the six
+    BOND_STAT_NAMES then the 10x10 species matrix, and (ptr, idx) the final Bonds CSR."""
+    s = state.copy()
+    n = s.n
+    if s.ax is None:
+        s.ax, s.ay, s.az = np.zeros(n), np.zeros(n), np.zeros(n)
+    arrs = [_d(a) for a in (s.x, s.y, s.z, s.vx, s.vy, s.vz, s.ax, s.ay, s.az)]
+    sp = np.ascontiguousarray(s.species, dtype=np.uint8)
+    q = _d(s.charge)
+    obs = np.zeros((max(nsteps, 1), 6))
+    bst = np.zeros((max(nsteps, 1), 106))
+    cap = 32 * max(n, 1) if bond_cap is None else bond_cap
+    ptr = np.zeros(n + 1, dtype=np.int32)
+    idx = np.zeros(max(cap, 1), dtype=np.int32)
+    p = physics.to_c()
+    rc = lib().or_run_observed(ctypes.byref(p), n, *[_p(a) for a in arrs], _p(sp, ctypes.c_uint8), _p(q), nsteps,
+                               nthreads, _p(obs), tick0, _p(bst), _p(ptr, ctypes.c_int32), _p(idx, ctypes.c_int32),
+                               cap)
+    if rc != 0:
+        raise RuntimeError("bond adjacency exceeds bond_cap")
+    s.x, s.y, s.z, s.vx, s.vy, s.vz, s.ax, s.ay, s.az = arrs
+    return s, obs[:nsteps], bst[:nsteps], (ptr, idx[:ptr[n]])
+
+
+def observe_frames(physics, frames, species, tick0=0):
+    """BondObservationSystem alone over position frames [(x, y, z), ...]; returns (F, 106) stats."""
+    nf = len(frames)
+    n = len(frames[0][0]) if nf else 0
+    X = np.ascontiguousarray(np.concatenate([f[0] for f in frames]) if nf else np.zeros(1))
+    Y = np.ascontiguousarray(np.concatenate([f[1] for f in frames]) if nf else np.zeros(1))
+    Z = np.ascontiguousarray(np.concatenate([f[2] for f in frames]) if nf else np.zeros(1))
+    sp = np.ascontiguousarray(species, dtype=np.uint8)
+    bst = np.zeros((max(nf, 1), 106))
+    p = physics.to_c()
+    lib().or_observe_frames(ctypes.byref(p), n, nf, _p(X), _p(Y), _p(Z), _p(sp, ctypes.c_uint8), tick0, _p(bst))
+    return bst[:nf]
+
+
+def bound_pairs(physics, x, y, z, species):
+    """bound_pairs_with_grid on one state: sorted (a, b) pairs, a < b."""
+    x, y, z = _d(x), _d(y), _d(z)
+    sp = np.ascontiguousarray(species, dtype=np.uint8)
+    n = len(x)
+    p = physics.to_c()
+    cap = 64 * max(n, 1)
+    pa = np.zeros(cap, dtype=np.uint32)
+    pb = np.zeros(cap, dtype=np.uint32)
+    k = lib().or_bound_pairs(ctypes.byref(p), n, _p(x), _p(y), _p(z), _p(sp, ctypes.c_uint8),
+                             _p(pa, ctypes.c_uint32), _p(pb, ctypes.c_uint32), cap)
+    return np.stack([pa[:k], pb[:k]], 1)
+
+
+def persistence_ticks(physics, a, b):
+    p = physics.to_c()
+    return int(lib().or_persistence_ticks(ctypes.byref(p), a, b))
+
+
+def track(ticks, debounce=2):
+    """PairTracker fed with per-tick lists of (a, b) pairs; returns {(a, b): open ticks}."""
+    counts = np.array([len(t) for t in ticks], dtype=np.int64)
+    flat = [p for t in ticks for p in t]
+    pa = np.array([p[0] for p in flat] or [0], dtype=np.uint32)
+    pb = np.array([p[1] for p in flat] or [0], dtype=np.uint32)
+    cap = max(len(flat), 1)
+    oa = np.zeros(cap, dtype=np.uint32)
+    ob = np.zeros(cap, dtype=np.uint32)
+    ot = np.zeros(cap, dtype=np.uint64)
+    m = lib().or_track(len(ticks), _p(counts, ctypes.c_int64), _p(pa, ctypes.c_uint32), _p(pb, ctypes.c_uint32),
+                       debounce, _p(oa, ctypes.c_uint32), _p(ob, ctypes.c_uint32), _p(ot, ctypes.c_uint64), cap)
+    return {(int(oa[k]), int(ob[k])): int(ot[k]) for k in range(m)}
 
 
 def half_kick(state, dt):
