@@ -54,6 +54,12 @@ class Tuning(ctypes.Structure):
                 ("sort_every", ctypes.c_int64), ("list_builds", ctypes.c_int64), ("ms_per_tick", ctypes.c_double)]
 
 
+class BondStats(ctypes.Structure):
+    _fields_ = [("tick", ctypes.c_int64), ("bonded_pairs", ctypes.c_int64), ("bonded_entities", ctypes.c_int64),
+                ("mean_coordination", ctypes.c_double), ("bonds_formed", ctypes.c_int64),
+                ("lifetime_sum_ticks", ctypes.c_double), ("species_matrix", ctypes.c_int64 * 100)]
+
+
 class GrSpec(ctypes.Structure):
     _fields_ = [("nbins", ctypes.c_int32), ("pad", ctypes.c_int32), ("r_max", ctypes.c_double),
                 ("sigma_b", ctypes.c_double)]
@@ -102,6 +108,13 @@ _SIGNATURES = {
     "gs_timing_enable": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int]),
     "gs_timing_read": (ctypes.c_int, [_H, ctypes.POINTER(Timing)]),
     "gs_clusters": (ctypes.c_int, [_H, ctypes.c_double, _U32, ctypes.POINTER(ClusterStats)]),
+    "gs_bond_observe": (ctypes.c_int, [_H, ctypes.c_int64]),
+    "gs_bond_stats_read": (ctypes.c_int, [_H, ctypes.POINTER(BondStats)]),
+    "gs_download_bonds": (ctypes.c_int, [_H, _I32, _I32, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
+    "gs_upload_bonds": (ctypes.c_int, [_H, _I32, _I32]),
+    "gs_bound_pairs": (ctypes.c_int, [_H, _U32, _U32, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
+    "gs_bond_components": (ctypes.c_int, [_H, _U32, ctypes.POINTER(ctypes.c_int64)]),
+    "gs_bond_reset": (ctypes.c_int, [_H]),
 }
 
 _lib = None

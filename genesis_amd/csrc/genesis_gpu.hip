@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -593,7 +594,8 @@ struct ThermoArgs {
     double target, tau, kconst;
 };
 
-__device__ void close_tick(double K, double V, int n, const ThermoArgs& th, double* lam_out, gs_observables* obs) {
+__device__ void close_tick(double K, double V, double Vb, int n, const ThermoArgs& th, double* lam_out,
+                           gs_observables* obs) {
     double lam = 1.0;
     const double T = (2.0 / 3.0) * (K / (double)n) / th.kconst;
     if (th.on && th.target > 0.0 && n > 0) {
@@ -607,10 +609,10 @@ __device__ void close_tick(double K, double V, int n, const ThermoArgs& th, doub
         const double Ka = lam * lam * K;
         obs->kinetic = Ka;
         obs->potential = V;
-        obs->bond_energy = 0.0;
+        obs->bond_energy = Vb;
         obs->temperature = th.kconst > 0.0 ? (2.0 / 3.0) * (Ka / (double)n) / th.kconst : 0.0;
         obs->lambda = lam;
-        obs->energy_total = Ka + V;
+        obs->energy_total = Ka + V + Vb;
     }
 }
 
@@ -625,7 +627,8 @@ __global__ __launch_bounds__(BLOCK) void k_kick_ke(KParams P, double* __restrict
                                                    const double* __restrict__ ay, const double* __restrict__ az,
                                                    const uint8_t* __restrict__ type, double* __restrict__ kpart,
                                                    uint32_t* __restrict__ counter, const double* __restrict__ vpart,
-                                                   uint32_t nvpart, double* __restrict__ lam_out,
+                                                   uint32_t nvpart, const double* __restrict__ vbpart, uint32_t nvb,
+                                                   double* __restrict__ lam_out,
                                                    gs_observables* __restrict__ obs, int thermostat, double target,
                                                    double tau, double kconst) {
     __shared__ double red[BLOCK / WAVE];
@@ -659,8 +662,9 @@ __global__ __launch_bounds__(BLOCK) void k_kick_ke(KParams P, double* __restrict
     __threadfence();
     const double K = block_sum_fixed(kpart, gridDim.x, red);
     const double V = vpart ? block_sum_fixed(vpart, nvpart, red) : 0.0;
+    const double Vb = vbpart ? block_sum_fixed(vbpart, nvb, red) : 0.0;
     if (tid == 0) {
-        close_tick(K, V, P.n, ThermoArgs{thermostat, target, tau, kconst}, lam_out, obs);
+        close_tick(K, V, Vb, P.n, ThermoArgs{thermostat, target, tau, kconst}, lam_out, obs);
         *counter = 0u;
     }
 }
@@ -1034,7 +1038,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
                                                      double* __restrict__ vz, double* __restrict__ vpart,
                                                      double* __restrict__ kpart, uint32_t* __restrict__ flags,
                                                      double* __restrict__ lam_out, gs_observables* __restrict__ obs,
-                                                     ThermoArgs th) {
+                                                     ThermoArgs th, const double* __restrict__ abond,
+                                                     const double* __restrict__ vbpart, uint32_t nvb) {
     constexpr int T = 1 << LOGT;
     __shared__ PairCoef sc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];
     __shared__ double wv[BLOCK / WAVE];
@@ -1110,7 +1115,12 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     double ke = 0.0;
     if (i < N && s == 0) {
         const double im = P.inv_mass[ti];
-        const double a0 = fx * im, a1 = fy * im, a2 = fz * im;
+        double a0 = fx * im, a1 = fy * im, a2 = fz * im;
+        if (abond) {
+            a0 += abond[i];
+            a1 += abond[N + i];
+            a2 += abond[2 * (size_t)N + i];
+        }
         ax[i] = a0;
         ay[i] = a1;
         az[i] = a2;
@@ -1141,8 +1151,9 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     __threadfence();
     const double K = block_sum_fixed(kpart, gridDim.x, red);
     const double V = block_sum_fixed(vpart, gridDim.x, red);
+    const double Vb = vbpart ? block_sum_fixed(vbpart, nvb, red) : 0.0;
     if (tid == 0) {
-        close_tick(K, V, (int)N, th, lam_out, obs);
+        close_tick(K, V, Vb, (int)N, th, lam_out, obs);
         flags[5] = 0u;
         flags[0] = 0u;
     }
@@ -1391,7 +1402,9 @@ __global__ __launch_bounds__(BLOCK) void k_fof_stats(int n, const uint32_t* __re
 
 struct gs_universe;
 struct AdjState;
+struct BondState;
 static void adj_free(gs_universe* u);
+static void bond_free(gs_universe* u);
 
 struct gs_universe {
     gs_params prm;
@@ -1462,6 +1475,13 @@ struct gs_universe {
     double* snap_a[3] = {nullptr, nullptr, nullptr};
     uint8_t* snap_type = nullptr;
     uint32_t* snap_id = nullptr;
+    // observed bonds (gs_bonds.inc)
+    BondState* bonds = nullptr;
+    double* l_thr = nullptr;      // bound thresholds k_bind sigma_ab
+    int64_t tick = 0;             // ctx.time.tick of the last tick
+    double* ab_live = nullptr;    // this tick's bond accelerations (list path)
+    double* vb_live = nullptr;    // this tick's V_bond partials
+    uint32_t nvb_live = 0;
     // differentiable trajectory (gs_adjoint.inc)
     AdjState* adj = nullptr;
     uint8_t* adj_keep = nullptr;
@@ -1485,6 +1505,11 @@ struct gs_universe {
 };
 
 namespace {
+
+// observed bonds (gs_bonds.inc)
+int bond_force_pass(gs_universe* u, bool accumulate);
+int bond_observe_pass(gs_universe* u, uint64_t now);
+int bond_refresh_species(gs_universe* u, const uint8_t* species);
 
 int validate_params(const gs_params* p, int64_t n) {
     if (!p) return fail(GS_E_INVALID, "params is NULL");
@@ -1746,7 +1771,8 @@ int launch_pair_list(gs_universe* u, gs_observables* obs) {
 #define GS_PL(L, C, S, SG)                                                                                       \
     k_pair_list<L, C, S, SG><<<nb, BLOCK, 0, u->stream>>>(P, N, u->pos4[c], u->type[c], u->nbr, u->nlen, u->coef, \
                                                           u->a[0], u->a[1], u->a[2], u->v[c][0], u->v[c][1],     \
-                                                          u->v[c][2], u->vpart, u->kpart, u->flags, u->scal, obs, th)
+                                                          u->v[c][2], u->vpart, u->kpart, u->flags, u->scal, obs, th, \
+                                                          u->ab_live, u->vb_live, u->nvb_live)
 #define GS_PL_T(C, S, SG)                   \
     switch (u->logT) {                      \
         case 0: GS_PL(0, C, S, SG); break;  \
@@ -1817,23 +1843,29 @@ int run_ticks(gs_universe* u, int64_t n_steps, bool observe, bool list) {
         if (tmode > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 0], u->stream));
         if (list) {
             if ((rc = list_tick_start(u))) return rc;
+            if ((rc = bond_force_pass(u, false))) return rc;
             if (tmode) HIPCHK(hipEventRecord(u->ev[4 * s + 1], u->stream));
             if ((rc = launch_pair_list(u, observe ? u->obs + s : nullptr))) return rc;
             if (tmode) HIPCHK(hipEventRecord(u->ev[4 * s + 2], u->stream));
             if (tmode > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
-            continue;
+        } else {
+            if ((rc = rebuild(u, true, u->kp))) return rc;
+            u->list_valid = false;
+            if (tmode) HIPCHK(hipEventRecord(u->ev[4 * s + 1], u->stream));
+            if ((rc = launch_pair<false>(u))) return rc;
+            if ((rc = bond_force_pass(u, true))) return rc;
+            if (tmode) HIPCHK(hipEventRecord(u->ev[4 * s + 2], u->stream));
+            const int c = u->cur;
+            k_kick_ke<true><<<nblocks(u->n), BLOCK, 0, u->stream>>>(
+                u->kp, u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->type[c], u->kpart,
+                u->counter, u->vpart, u->nvpart, u->vb_live, u->nvb_live, u->scal, observe ? u->obs + s : nullptr,
+                thermo ? 1 : 0, p.thermostat_temperature, std::fmax(p.thermostat_tau, 1.0), p.thermal_constant);
+            if (tmode > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
         }
-        if ((rc = rebuild(u, true, u->kp))) return rc;
-        u->list_valid = false;
-        if (tmode) HIPCHK(hipEventRecord(u->ev[4 * s + 1], u->stream));
-        if ((rc = launch_pair<false>(u))) return rc;
-        if (tmode) HIPCHK(hipEventRecord(u->ev[4 * s + 2], u->stream));
-        const int c = u->cur;
-        k_kick_ke<true><<<nblocks(u->n), BLOCK, 0, u->stream>>>(
-            u->kp, u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->type[c], u->kpart, u->counter,
-            u->vpart, u->nvpart, u->scal, observe ? u->obs + s : nullptr, thermo ? 1 : 0, p.thermostat_temperature,
-            std::fmax(p.thermostat_tau, 1.0), p.thermal_constant);
-        if (tmode > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
+        // BondObservationSystem runs after the thermostat (build_schedule,
+        // src/universe/mod.rs:383-388); its Bonds feed the next tick.
+        u->tick += 1;
+        if (p.enable_bond_observation && (rc = bond_observe_pass(u, (uint64_t)u->tick))) return rc;
     }
     HIPCHK(hipGetLastError());
     return GS_OK;
@@ -1912,6 +1944,7 @@ int gs_create(const gs_params* params, int64_t n_atoms, gs_universe** out) {
 int gs_destroy(gs_universe* u) {
     if (!u) return GS_OK;
     if (u->stream) (void)hipStreamSynchronize(u->stream);
+    bond_free(u);
     for (int b = 0; b < 2; ++b) {
         (void)hipFree(u->pos4[b]);
         for (int a = 0; a < 3; ++a) (void)hipFree(u->v[b][a]);
@@ -2008,7 +2041,7 @@ int gs_upload_state(gs_universe* u, const double* x, const double* y, const doub
     u->species_h.assign(species, species + n);
     u->charge_h.assign(charge, charge + n);
     std::memcpy(u->present, present, sizeof(present));
-    return GS_OK;
+    return bond_refresh_species(u, species);
 }
 
 int gs_download_state(gs_universe* u, double* x, double* y, double* z, double* vx, double* vy, double* vz, double* ax,
@@ -2041,13 +2074,16 @@ int gs_forces(gs_universe* u, double* potential, double* bond_energy) {
     if ((rc = rebuild(u, false, u->kp))) return rc;
     u->list_valid = false;
     if ((rc = launch_pair<true>(u))) return rc;
+    if ((rc = bond_force_pass(u, true))) return rc;
     k_reduce_to<<<1, BLOCK, 0, u->stream>>>(u->vpart, u->nvpart, u->scal + 1);
+    if (u->vb_live) k_reduce_to<<<1, BLOCK, 0, u->stream>>>(u->vb_live, u->nvb_live, u->scal + 3);
     HIPCHK(hipGetLastError());
-    double v = 0.0;
+    double v = 0.0, vb = 0.0;
     HIPCHK(hipMemcpyAsync(&v, u->scal + 1, sizeof(double), hipMemcpyDeviceToHost, u->stream));
+    if (u->vb_live) HIPCHK(hipMemcpyAsync(&vb, u->scal + 3, sizeof(double), hipMemcpyDeviceToHost, u->stream));
     HIPCHK(hipStreamSynchronize(u->stream));
     if (potential) *potential = v;
-    if (bond_energy) *bond_energy = 0.0;
+    if (bond_energy) *bond_energy = vb;
     u->extras_valid = true;
     return GS_OK;
 }
@@ -2123,8 +2159,8 @@ static int reduce_ke(gs_universe* u, bool thermostat, gs_observables* obs_dev) {
     const gs_params& p = u->prm;
     k_kick_ke<false><<<nblocks(u->n), BLOCK, 0, u->stream>>>(
         u->kp, u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->type[c], u->kpart, u->counter,
-        nullptr, 0, u->scal, obs_dev, thermostat ? 1 : 0, p.thermostat_temperature, std::fmax(p.thermostat_tau, 1.0),
-        p.thermal_constant);
+        nullptr, 0, nullptr, 0, u->scal, obs_dev, thermostat ? 1 : 0, p.thermostat_temperature,
+        std::fmax(p.thermostat_tau, 1.0), p.thermal_constant);
     HIPCHK(hipGetLastError());
     return GS_OK;
 }
@@ -2565,3 +2601,4 @@ int gs_clusters(gs_universe* u, double bond_factor, uint32_t* labels, gs_cluster
 
 #include "gs_adjoint.inc"
 #include "gs_batch.inc"
+#include "gs_bonds.inc"

@@ -218,3 +218,97 @@ class GpuUniverse:
         stats = {"monomers": st.monomers, "aggregates": st.aggregates, "largest": st.largest,
                  "mean_size": st.mean_size, "bound_pairs": st.bound_pairs}
         return labels, stats
+
+    # -- observed bonds ---------------------------------------------------------
+    def bond_observe(self, tick: int) -> None:
+        """BondObservationSystem::run on the resident positions as tick ``tick``."""
+        _lib.check(self.lib.gs_bond_observe(self.h, int(tick)))
+
+    def bond_stats(self) -> dict:
+        """The BondObservation summary of the last observed tick."""
+        b = _lib.BondStats()
+        _lib.check(self.lib.gs_bond_stats_read(self.h, ctypes.byref(b)))
+        return {"tick": b.tick, "bonded_pairs": b.bonded_pairs, "bonded_entities": b.bonded_entities,
+                "mean_coordination": b.mean_coordination, "bonds_formed": b.bonds_formed,
+                "lifetime_sum_ticks": b.lifetime_sum_ticks,
+                "species_matrix": np.array(b.species_matrix[:], dtype=np.int64).reshape(10, 10)}
+
+    def bonds(self):
+        """The Bonds adjacency as CSR (ptr[n+1], idx) over entity ids, neighbours ascending."""
+        ptr = np.zeros(self.n + 1, dtype=np.int32)
+        nnz = ctypes.c_int64()
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        cap = 32 * max(self.n, 1)
+        idx = np.zeros(cap, dtype=np.int32)
+        _lib.check(self.lib.gs_download_bonds(self.h, ptr.ctypes.data_as(i32), idx.ctypes.data_as(i32), cap,
+                                              ctypes.byref(nnz)))
+        return ptr, idx[:nnz.value].copy()
+
+    def upload_bonds(self, ptr, idx) -> None:
+        ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+        idx = np.ascontiguousarray(idx if len(idx) else np.zeros(1), dtype=np.int32)
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        _lib.check(self.lib.gs_upload_bonds(self.h, ptr.ctypes.data_as(i32), idx.ctypes.data_as(i32)))
+
+    def bound_pairs(self) -> np.ndarray:
+        """bound_pairs_with_grid on the resident positions: (k, 2) entity-id pairs, a < b, ascending."""
+        cnt = ctypes.c_int64()
+        u32 = ctypes.POINTER(ctypes.c_uint32)
+        _lib.check(self.lib.gs_bound_pairs(self.h, None, None, 0, ctypes.byref(cnt)))
+        k = cnt.value
+        a = np.zeros(max(k, 1), dtype=np.uint32)
+        b = np.zeros(max(k, 1), dtype=np.uint32)
+        _lib.check(self.lib.gs_bound_pairs(self.h, a.ctypes.data_as(u32), b.ctypes.data_as(u32), k,
+                                           ctypes.byref(cnt)))
+        return np.stack([a[:k], b[:k]], 1)
+
+    def bond_labels(self) -> np.ndarray:
+        """Component label per entity (smallest id of its Bonds component; 0xFFFFFFFF when unbonded)."""
+        labels = np.zeros(max(self.n, 1), dtype=np.uint32)
+        nc = ctypes.c_int64()
+        _lib.check(self.lib.gs_bond_components(self.h, labels.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                               ctypes.byref(nc)))
+        return labels[:self.n]
+
+    def bond_components(self, species) -> list:
+        """bond_components (src/analysis/mod.rs:337-412) of the Bonds graph: one dict per
+        component (size, composition, formula, members ascending, edges) ordered by size
+        descending, then formula, then smallest member (the reference leaves that last tie
+        to its hash order)."""
+        labels = self.bond_labels()
+        ptr, idx = self.bonds()
+        return components_from_labels(labels, np.asarray(species), ptr, idx)
+
+    def bond_reset(self) -> None:
+        _lib.check(self.lib.gs_bond_reset(self.h))
+
+
+def formula(composition) -> str:
+    """Stoichiometric formula (src/analysis/mod.rs:417-435): symbols by descending
+    count, then alphabetically, '-' separated, the count shown when > 1."""
+    from .params import SYMBOLS
+    parts = [(int(c), SYMBOLS[i]) for i, c in enumerate(composition) if c > 0]
+    parts.sort(key=lambda t: (-t[0], t[1]))
+    return "-".join(f"{sym}{c}" if c > 1 else sym for c, sym in parts)
+
+
+def components_from_labels(labels, species, ptr, idx) -> list:
+    """ChemicalComponent records from per-entity component labels and the Bonds CSR."""
+    no = np.uint32(0xFFFFFFFF)
+    comps = {}
+    for e in np.nonzero(labels != no)[0]:
+        comps.setdefault(int(labels[e]), []).append(int(e))
+    out = []
+    for root, members in comps.items():
+        comp = np.zeros(10, dtype=np.int64)
+        edges = []
+        for e in members:
+            comp[int(species[e])] += 1
+            for m in idx[ptr[e]:ptr[e + 1]]:
+                if int(m) > e:
+                    edges.append((e, int(m)))
+        out.append({"size": len(members), "composition": comp.tolist(), "formula": formula(comp),
+                    "members": sorted(members), "edges": sorted(edges)})
+    out.sort(key=lambda c: (-c["size"], c["formula"], c["members"][0]))
+    return out
+

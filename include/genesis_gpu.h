@@ -222,6 +222,50 @@ int gs_radial_distribution_between(gs_universe* u, int32_t species_a, int32_t sp
 int gs_clusters(gs_universe* u, double bond_factor, uint32_t* labels,
                 gs_cluster_stats* stats);
 
+/* ---- observed bonds ---------------------------------------------------------
+ * With enable_bond_observation, every gs_step tick ends with
+ * BondObservationSystem::run (src/systems/bonds.rs:73-185): the pairs with
+ * |min_image(a - b)| < bond_k_bind * sigma_ab (bound_pairs_with_grid,
+ * src/analysis/pairs.rs:132-174) feed a PairTracker with a 2-tick debounce
+ * (:40-111); an episode that reaches ceil(bond_min_periods * T_vib / dt)
+ * ticks is a persistent bond and enters the Bonds adjacency, which the
+ * force pass of the next tick uses when enable_bond_interaction is on (the
+ * harmonic term of src/systems/forces.rs:305-317).  Entity ids are the
+ * positions in the arrays given to gs_upload_state.  The tracker, the
+ * adjacency and the tick counter survive gs_upload_state (a new state is a
+ * new frame of the same world); gs_bond_reset clears them. */
+typedef struct gs_bond_stats {    /* BondObservation (src/systems/bonds.rs:140-184) */
+    int64_t tick;                 /* tick of the last observation                */
+    int64_t bonded_pairs;         /* persistent pairs now                        */
+    int64_t bonded_entities;      /* entities with at least one persistent bond  */
+    double mean_coordination;     /* 2 bonded_pairs / bonded_entities            */
+    int64_t bonds_formed;         /* persistent pairs that broke, cumulative     */
+    double lifetime_sum_ticks;    /* their summed persistence, cumulative        */
+    int64_t species_matrix[GS_NSPECIES * GS_NSPECIES]; /* [a*10+b] and [b*10+a] */
+} gs_bond_stats;
+
+/* BondObservationSystem::run on the resident positions as tick `tick`
+ * (ctx.time.tick); later gs_step ticks continue from tick + 1. */
+int gs_bond_observe(gs_universe* u, int64_t tick);
+int gs_bond_stats_read(gs_universe* u, gs_bond_stats* out);
+/* The Bonds adjacency as CSR over entity ids: ptr[n+1], idx[cap] with each
+ * entity's neighbours ascending; *nnz receives the entry count
+ * (GS_E_OVERFLOW when it exceeds cap). */
+int gs_download_bonds(gs_universe* u, int32_t* ptr, int32_t* idx, int64_t cap, int64_t* nnz);
+/* Replaces the Bonds adjacency (CSR over entity ids, made symmetric and
+ * sorted; at most 32 neighbours per entity); the tracker is untouched. */
+int gs_upload_bonds(gs_universe* u, const int32_t* ptr, const int32_t* idx);
+/* bound_pairs_with_grid on the resident positions: pairs (a < b, entity
+ * ids) ascending; *count may exceed cap (then only cap are written). */
+int gs_bound_pairs(gs_universe* u, uint32_t* a, uint32_t* b, int64_t cap, int64_t* count);
+/* Connected components of the Bonds graph (bond_components,
+ * src/analysis/mod.rs:337-412): labels[n] by entity id = smallest entity
+ * id of its component, 0xFFFFFFFF for entities without bonds. */
+int gs_bond_components(gs_universe* u, uint32_t* labels, int64_t* n_components);
+/* Clears the tracker, the Bonds adjacency, the cumulative counters and the
+ * tick counter. */
+int gs_bond_reset(gs_universe* u);
+
 /* ---- differentiable trajectory (no reference counterpart) -------------------
  * theta[30] = per-species sigma[10], epsilon_k[10], charge[10]; keep[n] (u8,
  * slot order) multiplies each atom's charge (0 = neutralised atom).  These

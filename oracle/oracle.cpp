@@ -996,8 +996,10 @@ int or_run(const gs_params* p, int64_t n, double* x, double* y, double* z, doubl
 // way its own test drives it, src/systems/bonds.rs:263-343): frame f is
 // observed at tick tick0 + f + 1.  bstats as in or_run_observed.
 int or_observe_frames(const gs_params* p, int64_t n, int64_t nframes, const double* x, const double* y,
-                      const double* z, const uint8_t* species, int64_t tick0, double* bstats) {
+                      const double* z, const uint8_t* species, int64_t tick0, double* bstats, int32_t* bond_ptr,
+                      int32_t* bond_idx, int64_t bond_cap) {
     BondObserver ob;
+    ob.bonds.assign((size_t)n, std::vector<uint32_t>());
     for (int64_t f = 0; f < nframes; ++f) {
         std::vector<V3> pos = load_pos(n, x + f * n, y + f * n, z + f * n);
         bond_observe(*p, pos, species, (uint64_t)(tick0 + f + 1), ob);
@@ -1009,6 +1011,19 @@ int or_observe_frames(const gs_params* p, int64_t n, int64_t nframes, const doub
         b[4] = ob.lifetime_sum_ticks;
         b[5] = ob.mean_coordination;
         for (int k = 0; k < GS_NSPECIES * GS_NSPECIES; ++k) b[6 + k] = (double)ob.matrix[k];
+    }
+    if (bond_ptr) {
+        int64_t off = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            bond_ptr[i] = (int32_t)off;
+            std::vector<uint32_t> nb = ob.bonds[(size_t)i];
+            std::sort(nb.begin(), nb.end());
+            for (uint32_t m : nb) {
+                if (off >= bond_cap) return -1;
+                bond_idx[off++] = (int32_t)m;
+            }
+        }
+        bond_ptr[n] = (int32_t)off;
     }
     return 0;
 }

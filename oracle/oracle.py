@@ -76,7 +76,7 @@ def lib():
             "or_run_observed": (ctypes.c_int, [P, ctypes.c_int64] + [_D] * 9 + [_U8, _D, ctypes.c_int64, ctypes.c_int,
                                                _D, ctypes.c_int64, _D, _I32, _I32, ctypes.c_int64]),
             "or_observe_frames": (ctypes.c_int, [P, ctypes.c_int64, ctypes.c_int64, _D, _D, _D, _U8, ctypes.c_int64,
-                                                 _D]),
+                                                 _D, _I32, _I32, ctypes.c_int64]),
             "or_bound_pairs": (ctypes.c_int64, [P, ctypes.c_int64, _D, _D, _D, _U8, _U32, _U32, ctypes.c_int64]),
             "or_persistence_ticks": (ctypes.c_uint64, [P, ctypes.c_int, ctypes.c_int]),
             "or_track": (ctypes.c_int64, [ctypes.c_int64, _I64, _U32, _U32, ctypes.c_uint64, _U32, _U32, _U64,
@@ -246,8 +246,9 @@ def run_observed(physics, state, nsteps, nthreads=1, tick0=0, bond_cap=None):
     return s, obs[:nsteps], bst[:nsteps], (ptr, idx[:ptr[n]])
 
 
-def observe_frames(physics, frames, species, tick0=0):
-    """BondObservationSystem alone over position frames [(x, y, z), ...]; returns (F, 106) stats."""
+def observe_frames(physics, frames, species, tick0=0, with_bonds=False):
+    """BondObservationSystem alone over position frames [(x, y, z), ...]; returns (F, 106)
+    stats (and the final Bonds CSR when with_bonds)."""
     nf = len(frames)
     n = len(frames[0][0]) if nf else 0
     X = np.ascontiguousarray(np.concatenate([f[0] for f in frames]) if nf else np.zeros(1))
@@ -255,8 +256,16 @@ def observe_frames(physics, frames, species, tick0=0):
     Z = np.ascontiguousarray(np.concatenate([f[2] for f in frames]) if nf else np.zeros(1))
     sp = np.ascontiguousarray(species, dtype=np.uint8)
     bst = np.zeros((max(nf, 1), 106))
+    cap = 32 * max(n, 1)
+    ptr = np.zeros(n + 1, dtype=np.int32)
+    idx = np.zeros(cap, dtype=np.int32)
     p = physics.to_c()
-    lib().or_observe_frames(ctypes.byref(p), n, nf, _p(X), _p(Y), _p(Z), _p(sp, ctypes.c_uint8), tick0, _p(bst))
+    rc = lib().or_observe_frames(ctypes.byref(p), n, nf, _p(X), _p(Y), _p(Z), _p(sp, ctypes.c_uint8), tick0,
+                                 _p(bst), _p(ptr, ctypes.c_int32), _p(idx, ctypes.c_int32), cap)
+    if rc != 0:
+        raise RuntimeError("bond adjacency exceeds its capacity")
+    if with_bonds:
+        return bst[:nf], (ptr, idx[:ptr[n]])
     return bst[:nf]
 
 

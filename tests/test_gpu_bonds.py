@@ -1,0 +1,189 @@
+"""Observed bonds on the GPU (gs_bond_*, the bond term of the force pass)
+against the oracle (oracle/oracle.cpp, pinned by tests/test_oracle_bonds.py).
+
+Bound pairs, tracker counts, persistence, lifetimes, the species matrix,
+the Bonds adjacency and its components are integer results: bit-exact.
+Forces and energies carry the force-pass tolerance (1e-10, conditioned)."""
+import numpy as np
+import pytest
+
+from genesis_amd.inputs import State, ionic_melt, random_gas
+from genesis_amd.params import DEBOUNCE, GRID_MINIMAGE, Physics, SPECIES_INDEX
+from oracle import oracle as O
+from tests.helpers import force_tolerance_ok
+
+pytestmark = pytest.mark.gpu
+
+H, NA, O_, SI = SPECIES_INDEX["H"], SPECIES_INDEX["Na"], SPECIES_INDEX["O"], SPECIES_INDEX["Si"]
+REL = 1e-10
+
+
+def say(msg):
+    print(f"[gpu-bonds] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def GU():
+    from genesis_amd.universe import GpuUniverse
+    return GpuUniverse
+
+
+def melt(n_side, min_periods=0.02, seed=1105, **kw):
+    w = ionic_melt(seed=seed, nx=n_side, ny=n_side, nz=n_side, steps=60)
+    ph = Physics(**{**w.physics.__dict__, "bond_min_periods": min_periods, **kw})
+    return ph, w.state
+
+
+def frames_of(ph, st, nframes, jitter, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nframes):
+        out.append(tuple(np.asarray(a, dtype=float) + rng.normal(0.0, jitter, st.n) for a in (st.x, st.y, st.z)))
+    return out
+
+
+@pytest.mark.parametrize("L,semantics", [(30.0, 0), (9.0, 0), (9.0, GRID_MINIMAGE), (5.0, 0)])
+def test_bound_pairs_bit_exact(GU, L, semantics):
+    n = 2000 if L > 10 else 60
+    st = random_gas(seed=int(L * 7) + semantics, n=n, box=(L, L, L), species_pool=(H, O_, NA, SI))
+    ph = Physics(box=(L, L, L), grid_semantics=semantics)
+    say(f"bound pairs n={n} L={L} semantics={semantics}")
+    with GU(ph, st) as u:
+        got = u.bound_pairs()
+    ref = O.bound_pairs(ph, st.x, st.y, st.z, st.species)
+    assert len(ref) > 0
+    assert np.array_equal(got, ref)
+
+
+def test_lifetime_and_species_matrix_fixture(GU):
+    # the reference's own observation test (src/systems/bonds.rs:263-343),
+    # frame by frame through upload + observe
+    ph = Physics()
+    thr = O.persistence_ticks(ph, H, H)
+    sp = np.array([H, H], dtype=np.uint8)
+    z2 = np.zeros(2)
+    bound = State(np.array([0.0, 1.0]), z2, z2, z2, z2, z2, sp, z2)
+    apart = State(np.array([0.0, 6.0]), z2, z2, z2, z2, z2, sp, z2)
+    say(f"H-H lifetime fixture, threshold {thr} ticks")
+    with GU(ph, bound) as u:
+        for t in range(1, thr + 501):
+            u.bond_observe(t)
+        s = u.bond_stats()
+        assert s["bonded_pairs"] == 1 and s["bonds_formed"] == 0 and s["species_matrix"][H, H] == 2
+        ptr, idx = u.bonds()
+        assert list(ptr) == [0, 1, 2] and list(idx) == [1, 0]
+        u.upload(apart)
+        for t in range(thr + 501, thr + 501 + DEBOUNCE + 2):
+            u.bond_observe(t)
+        e = u.bond_stats()
+    assert e["bonded_pairs"] == 0 and e["bonds_formed"] == 1 and e["lifetime_sum_ticks"] == 502.0
+
+
+@pytest.mark.parametrize("n_side,jitter", [(8, 0.12), (12, 0.08)])
+def test_observation_frames_match_oracle(GU, n_side, jitter):
+    # a jittered rock salt: pairs flicker across their thresholds, so episodes
+    # open, survive one-tick gaps, close at the debounce, become persistent
+    # and break; every tick's summary and the final adjacency are integers
+    ph, st = melt(n_side, min_periods=0.002)
+    frames = frames_of(ph, st, 40, jitter, seed=n_side)
+    ref, (rptr, ridx) = O.observe_frames(ph, frames, st.species, with_bonds=True)
+    say(f"observation frames n={st.n}, thresholds {O.persistence_ticks(ph, NA, O_)} ticks")
+    with GU(ph, st) as u:
+        for f, (x, y, z) in enumerate(frames):
+            u.upload(State(x, y, z, st.vx, st.vy, st.vz, st.species, st.charge))
+            u.bond_observe(f + 1)
+            s = u.bond_stats()
+            row = ref[f]
+            got = [s["tick"], s["bonded_pairs"], s["bonded_entities"], s["bonds_formed"], s["lifetime_sum_ticks"],
+                   s["mean_coordination"]]
+            assert got == list(row[:6]), (f, got, row[:6])
+            assert np.array_equal(s["species_matrix"].ravel(), row[6:].astype(np.int64))
+        ptr, idx = u.bonds()
+    assert ref[-1][1] > 0 and ref[-1][3] > 0, "bonds formed and broke during the frames"
+    assert np.array_equal(ptr, rptr) and np.array_equal(idx, ridx)
+
+
+def test_bond_force_term_matches_oracle(GU):
+    ph, st = melt(12, enable_bond_interaction=True)
+    # bonds: every Na-O pair closer than 2.3 among the first 400 atoms
+    pairs = O.bound_pairs(ph, st.x, st.y, st.z, st.species)
+    keep = [(a, b) for a, b in pairs if a < 400 and b < 400 and st.species[a] != st.species[b]]
+    adj = [[] for _ in range(st.n)]
+    for a, b in keep:
+        adj[a].append(b)
+        adj[b].append(a)
+    ptr = np.zeros(st.n + 1, dtype=np.int32)
+    ptr[1:] = np.cumsum([len(x) for x in adj])
+    idx = np.array([m for x in adj for m in sorted(x)], dtype=np.int32)
+    say(f"bond force term: {len(keep)} bonds, n={st.n}")
+    ref = O.forces(ph, st, bonds_csr=(ptr, idx))
+    with GU(ph, st) as u:
+        u.upload_bonds(ptr, idx)
+        V, Vb = u.forces()
+        got = u.download()
+    m = np.asarray(ph.mass)[st.species]
+    ok, worst = force_tolerance_ok(got.ax, got.ay, got.az, ref, m, REL)
+    assert ok, worst
+    assert V == pytest.approx(ref["V"], rel=REL) and Vb == pytest.approx(ref["Vb"], rel=REL) and Vb != 0.0
+
+
+@pytest.mark.parametrize("n_side,mode", [(8, 0), (12, 0), (12, 1)])
+def test_bonded_trajectory_matches_oracle(GU, n_side, mode):
+    # observation and interaction on, thresholds of ~24 ticks: bonds appear
+    # mid-run and their springs act from the next tick (Q8)
+    ph, st = melt(n_side)
+    nsteps = 60
+    ref, obs_ref, bs_ref, (rptr, ridx) = O.run_observed(ph, st, nsteps, nthreads=8)
+    say(f"bonded trajectory n={st.n} dims={ph.grid_dims()} mode={mode}")
+    with GU(ph, st) as u:
+        if mode == 1:
+            u.set_tuning(1, 4)
+        obs = u.step(nsteps)
+        got = u.download()
+        s = u.bond_stats()
+        ptr, idx = u.bonds()
+    assert bs_ref[-1][1] > 0 and np.any(obs_ref[:, 2] != 0.0)
+    np.testing.assert_allclose(obs[:, 2], obs_ref[:, 2], rtol=1e-8, atol=1e-12)
+    np.testing.assert_allclose(obs[:, 5], obs_ref[:, 5], rtol=1e-9)
+    np.testing.assert_allclose(got.x, ref.x, rtol=0, atol=1e-8)
+    assert [s["tick"], s["bonded_pairs"], s["bonded_entities"], s["bonds_formed"]] == list(bs_ref[-1][:4])
+    assert np.array_equal(ptr, rptr) and np.array_equal(idx, ridx)
+
+
+def test_bond_components_fixtures(GU):
+    # src/systems/bond_structure.rs:265-327: two Na-O dimers and a monomer,
+    # and the 4-ring; formulas as src/analysis/mod.rs:589-605
+    sp = np.array([NA, O_, NA, O_, NA], dtype=np.uint8)
+    x = np.array([0.0, 2.0, 5.0, 7.0, 10.0])
+    z5 = np.zeros(5)
+    st = State(x, z5, z5, z5, z5, z5, sp, z5)
+    ph = Physics(box=(40.0, 40.0, 40.0))
+    with GU(ph, st) as u:
+        u.upload_bonds(np.array([0, 1, 2, 3, 4, 4], dtype=np.int32), np.array([1, 0, 3, 2], dtype=np.int32))
+        dimers = u.bond_components(sp)
+        u.upload_bonds(np.array([0, 2, 4, 6, 8, 8], dtype=np.int32),
+                       np.array([1, 3, 0, 2, 1, 3, 2, 0], dtype=np.int32))
+        ring = u.bond_components(sp)
+        labels = u.bond_labels()
+    assert [(c["size"], c["formula"], c["members"]) for c in dimers] == [(2, "Na-O", [0, 1]), (2, "Na-O", [2, 3])]
+    assert [(c["size"], c["formula"], c["members"]) for c in ring] == [(4, "Na2-O2", [0, 1, 2, 3])]
+    assert list(labels) == [0, 0, 0, 0, 0xFFFFFFFF]
+    ref = O.bond_components([(e, int(sp[e])) for e in range(4)], [(0, 1), (1, 2), (2, 3), (3, 0)])
+    assert [(c["size"], c["formula"], c["members"]) for c in ref] == [(4, "Na2-O2", [0, 1, 2, 3])]
+
+
+def test_bond_components_of_an_observed_melt_match_oracle(GU):
+    ph, st = melt(12, min_periods=0.002)
+    frames = frames_of(ph, st, 12, 0.1, seed=7)
+    _, (rptr, ridx) = O.observe_frames(ph, frames, st.species, with_bonds=True)
+    ents = [(e, int(st.species[e])) for e in range(st.n) if rptr[e + 1] > rptr[e]]
+    edges = [(e, int(m)) for e in range(st.n) for m in ridx[rptr[e]:rptr[e + 1]]]
+    ref = O.bond_components(ents, edges)
+    with GU(ph, st) as u:
+        for f, (x, y, z) in enumerate(frames):
+            u.upload(State(x, y, z, st.vx, st.vy, st.vz, st.species, st.charge))
+            u.bond_observe(f + 1)
+        got = u.bond_components(st.species)
+    assert len(ref) > 1
+    assert [(c["size"], c["formula"], c["members"]) for c in got] == \
+        [(c["size"], c["formula"], c["members"]) for c in ref]
