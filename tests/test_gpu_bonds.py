@@ -28,9 +28,11 @@ def GU():
     return GpuUniverse
 
 
-def melt(n_side, min_periods=0.02, seed=1105, **kw):
+def melt(n_side, seed=1105, **kw):
+    # bond_min_periods is clamped at 0.1 periods (src/systems/bonds.rs:52):
+    # about 116 ticks for Na-O at dt = 0.5 fs
     w = ionic_melt(seed=seed, nx=n_side, ny=n_side, nz=n_side, steps=60)
-    ph = Physics(**{**w.physics.__dict__, "bond_min_periods": min_periods, **kw})
+    ph = Physics(**{**w.physics.__dict__, "bond_min_periods": 0.1, **kw})
     return ph, w.state
 
 
@@ -44,7 +46,7 @@ def frames_of(ph, st, nframes, jitter, seed):
 
 @pytest.mark.parametrize("L,semantics", [(30.0, 0), (9.0, 0), (9.0, GRID_MINIMAGE), (5.0, 0)])
 def test_bound_pairs_bit_exact(GU, L, semantics):
-    n = 2000 if L > 10 else 60
+    n = 2000 if L > 10 else 25
     st = random_gas(seed=int(L * 7) + semantics, n=n, box=(L, L, L), species_pool=(H, O_, NA, SI))
     ph = Physics(box=(L, L, L), grid_semantics=semantics)
     say(f"bound pairs n={n} L={L} semantics={semantics}")
@@ -84,7 +86,8 @@ def test_observation_frames_match_oracle(GU, n_side, jitter):
     # a jittered rock salt: pairs flicker across their thresholds, so episodes
     # open, survive one-tick gaps, close at the debounce, become persistent
     # and break; every tick's summary and the final adjacency are integers
-    ph, st = melt(n_side, min_periods=0.002)
+    # dt only enters the thresholds here: 0.05 gives about 11 ticks
+    ph, st = melt(n_side, dt=0.05)
     frames = frames_of(ph, st, 40, jitter, seed=n_side)
     ref, (rptr, ridx) = O.observe_frames(ph, frames, st.species, with_bonds=True)
     say(f"observation frames n={st.n}, thresholds {O.persistence_ticks(ph, NA, O_)} ticks")
@@ -129,10 +132,10 @@ def test_bond_force_term_matches_oracle(GU):
 
 @pytest.mark.parametrize("n_side,mode", [(8, 0), (12, 0), (12, 1)])
 def test_bonded_trajectory_matches_oracle(GU, n_side, mode):
-    # observation and interaction on, thresholds of ~24 ticks: bonds appear
-    # mid-run and their springs act from the next tick (Q8)
+    # observation and interaction on, thresholds of ~116 ticks: bonds appear
+    # late in the run and their springs act from the next tick on
     ph, st = melt(n_side)
-    nsteps = 60
+    nsteps = 150
     ref, obs_ref, bs_ref, (rptr, ridx) = O.run_observed(ph, st, nsteps, nthreads=8)
     say(f"bonded trajectory n={st.n} dims={ph.grid_dims()} mode={mode}")
     with GU(ph, st) as u:
@@ -173,8 +176,8 @@ def test_bond_components_fixtures(GU):
 
 
 def test_bond_components_of_an_observed_melt_match_oracle(GU):
-    ph, st = melt(12, min_periods=0.002)
-    frames = frames_of(ph, st, 12, 0.1, seed=7)
+    ph, st = melt(12, dt=0.05)
+    frames = frames_of(ph, st, 20, 0.1, seed=7)
     _, (rptr, ridx) = O.observe_frames(ph, frames, st.species, with_bonds=True)
     ents = [(e, int(st.species[e])) for e in range(st.n) if rptr[e + 1] > rptr[e]]
     edges = [(e, int(m)) for e in range(st.n) for m in ridx[rptr[e]:rptr[e + 1]]]
