@@ -176,8 +176,11 @@ def test_bond_components_fixtures(GU):
 
 
 def test_bond_components_of_an_observed_melt_match_oracle(GU):
-    ph, st = melt(12, dt=0.05)
-    frames = frames_of(ph, st, 20, 0.1, seed=7)
+    # a dilute Na/O gas: a few hundred bonded components of many sizes and
+    # formulas (ties in size exercise the formula and member ordering)
+    st = random_gas(seed=77, n=1200, box=(40.0, 40.0, 40.0), species_pool=(NA, O_))
+    ph = Physics(box=(40.0, 40.0, 40.0), dt=0.05, bond_min_periods=0.1)
+    frames = frames_of(ph, st, 20, 0.05, seed=3)
     _, (rptr, ridx) = O.observe_frames(ph, frames, st.species, with_bonds=True)
     ents = [(e, int(st.species[e])) for e in range(st.n) if rptr[e + 1] > rptr[e]]
     edges = [(e, int(m)) for e in range(st.n) for m in ridx[rptr[e]:rptr[e + 1]]]
@@ -187,6 +190,6 @@ def test_bond_components_of_an_observed_melt_match_oracle(GU):
             u.upload(State(x, y, z, st.vx, st.vy, st.vz, st.species, st.charge))
             u.bond_observe(f + 1)
         got = u.bond_components(st.species)
-    assert len(ref) > 1
+    assert len(ref) > 100
     assert [(c["size"], c["formula"], c["members"]) for c in got] == \
         [(c["size"], c["formula"], c["members"]) for c in ref]
