@@ -2365,6 +2365,10 @@ int gs_autotune(gs_universe* u, int32_t trial_steps, gs_tuning* best) {
     }
     u->timing = was_timing;
     if ((rc = gs_set_tuning(u, &win))) return rc;
+    // the trials observed bonds on copies of the starting state: start the
+    // observation afresh
+    if (u->bonds && (rc = gs_bond_reset(u))) return rc;
+    u->tick = 0;
     u->list_builds = 0;
     if (best) {
         *best = win;

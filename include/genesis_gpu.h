@@ -190,7 +190,9 @@ int gs_get_tuning(gs_universe* u, gs_tuning* out);
 int gs_set_tuning(gs_universe* u, const gs_tuning* t);
 /* Project autotuner: times trial_steps ticks of every candidate setting on
  * the resident state, restores the state after each trial and keeps the
- * fastest.  best (may be NULL) receives it. */
+ * fastest.  best (may be NULL) receives it.  The trials observe bonds too;
+ * the bond observation state (tracker, adjacency, tick counter) is reset
+ * afterwards, as by gs_bond_reset. */
 int gs_autotune(gs_universe* u, int32_t trial_steps, gs_tuning* best);
 
 /* Event timing of gs_step, reset on every call: on = 0 off, 1 the pair
