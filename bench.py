@@ -33,6 +33,13 @@ FP64_PEAK_TFLOPS = 78.6
 STEP_BYTES = 161.0          # x, v, a read+write (144) + species (1) + cell key (8) + charge (8)
 PAIR_BYTES = 57.0           # pair kernel: x, y, z, q, species read (33) + a written (24)
 FLOP_PER_PAIR = 82.0        # LJ + quintic switch + accumulate (70) + Coulomb (12), per ordered pair
+# HBM traffic of one k_pair_list launch of the default setting (4 lanes, skin
+# 0.12 rc) on this workload, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in
+# KiB per dispatch, scaled by what the same counters read for a 256 MiB
+# device copy (FETCH_SIZE 131084.5 KiB -> x2.0, WRITE_SIZE 262144 KiB -> x1.0);
+# profiles/k_pair_list__mixture32768_pmc.md holds the runs.
+PAIR_TRAFFIC_KIB = {"fetch": 14785.3, "write": 1584.1, "fetch_scale": 2.0, "write_scale": 1.0}
+DEFAULT_LANES, DEFAULT_SKIN_FRAC = 4, 0.12
 
 
 def log(msg):
@@ -46,9 +53,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--pair-mode", type=int, default=-1, help="-1 autotune, 0 cell tiles, 1 Verlet list")
-    ap.add_argument("--lanes", type=int, default=8)
-    ap.add_argument("--skin", type=float, default=None, help="list skin (length units) with --pair-mode 1")
+    ap.add_argument("--pair-mode", type=int, default=1, help="-1 autotune, 0 cell tiles, 1 Verlet list")
+    ap.add_argument("--lanes", type=int, default=DEFAULT_LANES)
+    ap.add_argument("--skin", type=float, default=None,
+                    help=f"list skin (length units); default {DEFAULT_SKIN_FRAC} rc")
     ap.add_argument("--sort-every", type=int, default=0, help="list mode: ticks between full re-sorts")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-replica and adjoint legs")
@@ -73,12 +81,15 @@ def main() -> int:
         f"warmup={args.warmup}")
 
     u = GpuUniverse(ph, st)
+    skin = DEFAULT_SKIN_FRAC * ph.force_rc() if args.skin is None else args.skin
+    default_setting = args.pair_mode == 1 and args.lanes == DEFAULT_LANES and args.skin is None
     if args.pair_mode < 0:
         tuning = u.autotune(40)
         log(f"autotuned: {tuning}")
     else:
-        u.set_tuning(args.pair_mode, args.lanes, skin=args.skin, sort_every=args.sort_every)
+        u.set_tuning(args.pair_mode, args.lanes, skin=skin, sort_every=args.sort_every)
         tuning = u.tuning()
+        log(f"tuning: {tuning}")
 
     def run(k, observe=False):
         done = 0
@@ -90,9 +101,9 @@ def main() -> int:
     if args.warmup:
         run(args.warmup)
     builds0 = u.tuning()["list_builds"]
-    # timed region: on every 10th tick two HIP events on the universe's
-    # stream bracket the pair kernel (its average duration feeds the
-    # roofline); sampling keeps the event records' cost out of the rate
+    # timed region: on every 10th tick the pair kernel is launched with start
+    # and stop HIP events bound to its dispatch (its average duration feeds
+    # the roofline); sampling keeps the events' cost out of the rate
     u.timing(enable="pair", every=10)
     t0 = time.perf_counter()
     run(args.steps)
@@ -142,7 +153,10 @@ def main() -> int:
         "roofline": {"bound": "mfma", "pipe": "fp64 VALU (pair forces are not a dense contraction; the MI355X "
                                               "FP64 vector peak equals its FP64 matrix peak)",
                      "kernel": kernel, "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": tflops / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": tflops / FP64_PEAK_TFLOPS,
+                     "traffic": (1024.0 * (PAIR_TRAFFIC_KIB["fetch"] * PAIR_TRAFFIC_KIB["fetch_scale"] +
+                                           PAIR_TRAFFIC_KIB["write"] * PAIR_TRAFFIC_KIB["write_scale"])
+                                 if default_setting else None),
                      "per_launch": {"ordered_pairs": npairs, "flop_per_pair": FLOP_PER_PAIR,
                                     "avg_ms": pair_ms}},
         "roofline_hbm": {"bound": "hbm", "kernel": kernel, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -155,6 +169,13 @@ def main() -> int:
         "list_builds_in_timed_region": builds,
     }
     u.close()
+    if args.pair_mode >= 0 and not args.no_extras:
+        # the project autotuner on the same workload, for comparison with the
+        # fixed setting timed above
+        with GpuUniverse(ph, st) as au:
+            t = au.autotune(40)
+        out["autotune"] = {k: t[k] for k in ("pair_mode", "lanes_per_atom", "skin", "ms_per_tick")}
+        log(f"autotune pick: {out['autotune']}")
     if not args.no_extras:
         out["batched"] = batched_leg()
         out["adjoint"] = adjoint_leg()

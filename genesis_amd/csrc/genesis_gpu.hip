@@ -17,6 +17,8 @@
 // results are bitwise reproducible run to run.
 #include "gs_device.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -1656,22 +1658,36 @@ int rebuild(gs_universe* u, bool kick_drift, const KParams& P) {
     return GS_OK;
 }
 
+// Launches a kernel on the universe's stream; with e0 / e1 the events are
+// bound to the dispatch itself (hipExtLaunchKernelGGL), so their interval is
+// the kernel's execution without the dispatch gap an event record adds.
+template <typename K, typename... A>
+void launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t stream, hipEvent_t e0, hipEvent_t e1, A... args) {
+    if (e0)
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, stream, e0, e1, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, 0, stream, args...);
+}
+
 template <bool EXTRAS>
-int launch_pair(gs_universe* u) {
+int launch_pair(gs_universe* u, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const KParams& P = u->kp;
     const int c = u->cur;
     if (u->small_box) {
         if (u->coul)
-            k_pair_brute<true, EXTRAS><<<nblocks(u->n), BLOCK, 0, u->stream>>>(
-                P, u->pos4[c], u->type[c], u->coef, u->a[0], u->a[1], u->a[2], u->pe, u->sabs, u->vpart);
+            launch_timed(k_pair_brute<true, EXTRAS>, dim3(nblocks(u->n)), dim3(BLOCK), u->stream, e0, e1, P,
+                         (const double4*)u->pos4[c], (const uint8_t*)u->type[c], (const PairCoef*)u->coef, u->a[0],
+                         u->a[1], u->a[2], u->pe, u->sabs, u->vpart);
         else
-            k_pair_brute<false, EXTRAS><<<nblocks(u->n), BLOCK, 0, u->stream>>>(
-                P, u->pos4[c], u->type[c], u->coef, u->a[0], u->a[1], u->a[2], u->pe, u->sabs, u->vpart);
+            launch_timed(k_pair_brute<false, EXTRAS>, dim3(nblocks(u->n)), dim3(BLOCK), u->stream, e0, e1, P,
+                         (const double4*)u->pos4[c], (const uint8_t*)u->type[c], (const PairCoef*)u->coef, u->a[0],
+                         u->a[1], u->a[2], u->pe, u->sabs, u->vpart);
     } else {
-#define GS_PAIR(C, S, E, SG)                                                                                   \
-    k_pair<C, S, E, SG><<<P.ncell, PAIR_THREADS, 0, u->stream>>>(P, u->cstart, u->pos4[c], u->type[c], u->coef, \
-                                                                  u->a[0], u->a[1], u->a[2], u->pe, u->sabs,   \
-                                                                  u->vpart)
+#define GS_PAIR(C, S, E, SG)                                                                                      \
+    launch_timed(k_pair<C, S, E, SG>, dim3(P.ncell), dim3(PAIR_THREADS), u->stream, e0, e1, P,                    \
+                 (const uint32_t*)u->cstart, (const double4*)u->pos4[c], (const uint8_t*)u->type[c],             \
+                 (const PairCoef*)u->coef, u->a[0], u->a[1], u->a[2], u->pe, u->sabs, u->vpart,                   \
+                 (const KParams*)nullptr, (const uint16_t*)nullptr, (const uint32_t*)nullptr)
         if (u->single)
             GS_PAIR(false, false, EXTRAS, true);
         else if (!u->coul)
@@ -1760,7 +1776,7 @@ int snapshot(gs_universe* u, bool restore) {
 
 // List pair pass of one tick, fused with the second kick, the kinetic
 // reduction and the Berendsen lambda; obs (device) receives the observables.
-int launch_pair_list(gs_universe* u, gs_observables* obs) {
+int launch_pair_list(gs_universe* u, gs_observables* obs, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const KParams& P = u->kp;
     const gs_params& p = u->prm;
     const int c = u->cur;
@@ -1768,11 +1784,12 @@ int launch_pair_list(gs_universe* u, gs_observables* obs) {
     const unsigned nb = nblocks((int64_t)N << u->logT);
     const ThermoArgs th{p.enable_thermostat != 0 ? 1 : 0, p.thermostat_temperature, std::fmax(p.thermostat_tau, 1.0),
                         p.thermal_constant};
-#define GS_PL(L, C, S, SG)                                                                                       \
-    k_pair_list<L, C, S, SG><<<nb, BLOCK, 0, u->stream>>>(P, N, u->pos4[c], u->type[c], u->nbr, u->nlen, u->coef, \
-                                                          u->a[0], u->a[1], u->a[2], u->v[c][0], u->v[c][1],     \
-                                                          u->v[c][2], u->vpart, u->kpart, u->flags, u->scal, obs, th, \
-                                                          u->ab_live, u->vb_live, u->nvb_live)
+#define GS_PL(L, C, S, SG)                                                                                    \
+    launch_timed(k_pair_list<L, C, S, SG>, dim3(nb), dim3(BLOCK), u->stream, e0, e1, P, N,                       \
+                 (const double4*)u->pos4[c], (const uint8_t*)u->type[c], (const uint32_t*)u->nbr,               \
+                 (const uint32_t*)u->nlen, (const PairCoef*)u->coef, u->a[0], u->a[1], u->a[2], u->v[c][0],      \
+                 u->v[c][1], u->v[c][2], u->vpart, u->kpart, u->flags, u->scal, obs, th,                          \
+                 (const double*)u->ab_live, (const double*)u->vb_live, u->nvb_live)
 #define GS_PL_T(C, S, SG)                   \
     switch (u->logT) {                      \
         case 0: GS_PL(0, C, S, SG); break;  \
@@ -1844,17 +1861,16 @@ int run_ticks(gs_universe* u, int64_t n_steps, bool observe, bool list) {
         if (list) {
             if ((rc = list_tick_start(u))) return rc;
             if ((rc = bond_force_pass(u, false))) return rc;
-            if (tmode) HIPCHK(hipEventRecord(u->ev[4 * s + 1], u->stream));
-            if ((rc = launch_pair_list(u, observe ? u->obs + s : nullptr))) return rc;
-            if (tmode) HIPCHK(hipEventRecord(u->ev[4 * s + 2], u->stream));
+            if ((rc = launch_pair_list(u, observe ? u->obs + s : nullptr, tmode ? u->ev[4 * s + 1] : nullptr,
+                                       tmode ? u->ev[4 * s + 2] : nullptr)))
+                return rc;
             if (tmode > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
         } else {
             if ((rc = rebuild(u, true, u->kp))) return rc;
             u->list_valid = false;
-            if (tmode) HIPCHK(hipEventRecord(u->ev[4 * s + 1], u->stream));
-            if ((rc = launch_pair<false>(u))) return rc;
+            if ((rc = launch_pair<false>(u, tmode ? u->ev[4 * s + 1] : nullptr, tmode ? u->ev[4 * s + 2] : nullptr)))
+                return rc;
             if ((rc = bond_force_pass(u, true))) return rc;
-            if (tmode) HIPCHK(hipEventRecord(u->ev[4 * s + 2], u->stream));
             const int c = u->cur;
             k_kick_ke<true><<<nblocks(u->n), BLOCK, 0, u->stream>>>(
                 u->kp, u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->type[c], u->kpart,
