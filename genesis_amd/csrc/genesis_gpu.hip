@@ -1074,8 +1074,10 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
         const uint32_t cnt = len > s ? (len - s + T - 1) >> LOGT : 0u;
         const uint4* grp = reinterpret_cast<const uint4*>(nbr) + (((size_t)i << LOGT) + s);
         const size_t gstride = (size_t)N << LOGT;
-        for (uint32_t m0 = 0; m0 < cnt; m0 += 4, grp += gstride) {
-            const uint4 ev = *grp;
+        // the next group's entries are loaded while this group's pairs are
+        // evaluated (the address select keeps the load unconditional)
+        uint4 ev = cnt ? *grp : make_uint4(0u, 0u, 0u, 0u);
+        for (uint32_t m0 = 0; m0 < cnt; m0 += 4) {
             const uint32_t e4[4] = {ev.x, ev.y, ev.z, ev.w};
             double4 pj4[4];
 #pragma unroll
@@ -1083,6 +1085,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
                 const uint32_t jj = m0 + q < cnt ? (e4[q] & NBR_IDX_MASK) : i;  // past the end: self, masked
                 pj4[q] = pos4[jj];
             }
+            grp = m0 + 4 < cnt ? grp + gstride : grp;
+            ev = *grp;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const bool live = m0 + q < cnt;
