@@ -1060,13 +1060,16 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     const uint32_t s = g & (T - 1);
     double fx = 0.0, fy = 0.0, fz = 0.0, vl = 0.0;
     int ti = 0;
+    // a build that met a stencil too dense for LDS or a full bucket left some
+    // lists unwritten: the host replays this call, so no list is read now
+    const bool lists_ok = (flags[2] | flags[6]) == 0u;
     if (i < N) {
         const double4 pi = pos4[i];
         ti = type[i];
         const double kqi = P.ke * pi.w;
         const double Lx = P.box[0], Ly = P.box[1], Lz = P.box[2];
         const double iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
-        const uint32_t len = nlen[i];
+        const uint32_t len = lists_ok ? nlen[i] : 0u;
         const PairCoef* row = SINGLE ? sc : sc + ti * GS_NSPECIES;
         // this lane's entries: s, s+T, ... (cnt of them), four per group;
         // a group's four position gathers are all in flight before its four
@@ -1736,6 +1739,8 @@ int list_alloc(gs_universe* u, uint32_t cap, uint32_t bcap_min = 0) {
         for (int a = 0; a < 3; ++a)
             if ((rc = dalloc(&u->snap_v[a], n)) || (rc = dalloc(&u->snap_a[a], n))) return rc;
         HIPCHK(hipMemsetAsync(u->flags, 0, 8 * sizeof(uint32_t), u->stream));
+        // an atom whose list was never built has an empty one
+        HIPCHK(hipMemsetAsync(u->nlen, 0, sizeof(uint32_t) * n, u->stream));
     }
     // bucket capacity: twice the mean list-cell occupancy plus 32, doubled
     // on overflow; the force grid bounds the list grid's cell count
