@@ -785,6 +785,13 @@ constexpr uint32_t NBR_IDX_MASK = 0x0FFFFFFFu;  // entry = (species << 28) | sor
 constexpr int LDS_IDX_CAP = 2048;               // stencil candidates held (and ordered) in LDS
 constexpr int MAX_LOGT = 4;                     // up to 16 lanes per atom
 
+// Streaming 16-byte load with a non-temporal hint (global_load ... nt).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 load_nt(const uint4* p) {
+    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // Entry k of atom i with T lanes per atom: lane s = k mod T owns entries
 // s, s+T, s+2T, ...; its m-th entry (m = k / T) sits in group m/4, slot
 // m mod 4 of an [groups][N][T][4] array, so a lane reads four entries with
@@ -1077,9 +1084,14 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
         const uint32_t cnt = len > s ? (len - s + T - 1) >> LOGT : 0u;
         const uint4* grp = reinterpret_cast<const uint4*>(nbr) + (((size_t)i << LOGT) + s);
         const size_t gstride = (size_t)N << LOGT;
-        // the next group's entries are loaded while this group's pairs are
-        // evaluated (the address select keeps the load unconditional)
-        uint4 ev = cnt ? *grp : make_uint4(0u, 0u, 0u, 0u);
+        // the entry groups stream from HBM once per tick: they are loaded two
+        // groups ahead of use (address selects keep the loads unconditional)
+        // with a non-temporal hint, so they do not displace the positions the
+        // gathers reuse from L2
+        const uint4* g1 = cnt > 4 ? grp + gstride : grp;
+        uint4 ev = cnt ? load_nt(grp) : make_uint4(0u, 0u, 0u, 0u);
+        uint4 ev1 = cnt ? load_nt(g1) : make_uint4(0u, 0u, 0u, 0u);
+        grp = g1;
         for (uint32_t m0 = 0; m0 < cnt; m0 += 4) {
             const uint32_t e4[4] = {ev.x, ev.y, ev.z, ev.w};
             double4 pj4[4];
@@ -1088,8 +1100,13 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
                 const uint32_t jj = m0 + q < cnt ? (e4[q] & NBR_IDX_MASK) : i;  // past the end: self, masked
                 pj4[q] = pos4[jj];
             }
-            grp = m0 + 4 < cnt ? grp + gstride : grp;
-            ev = *grp;
+            grp = m0 + 8 < cnt ? grp + gstride : grp;
+            ev = ev1;
+            // issue the group load after the four gathers, so waiting for the
+            // gathers never waits for it (memory loads complete in order)
+            __builtin_amdgcn_sched_barrier(0);
+            ev1 = load_nt(grp);
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const bool live = m0 + q < cnt;
