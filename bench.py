@@ -122,8 +122,9 @@ def main() -> int:
     log(f"{value:.4e} atom-steps/s, {ms_per_step:.4f} ms/step; pair kernel {tm['pair_ms'] / tm['steps']:.4f} ms; "
         f"phases per step: rebuild {phases['rebuild_ms']:.4f} ms, pair {phases['pair_ms']:.4f} ms, "
         f"kick {phases['kick_ms']:.4f} ms; list builds {builds}")
-    kernel = f"k_pair_list<{tuning['lanes_per_atom']} lanes> (pair forces + second kick + Berendsen)" \
-        if tuning["pair_mode"] == 1 else "k_pair"
+    kernel = {1: f"k_pair_list<{tuning['lanes_per_atom']} lanes> (pair forces + second kick + Berendsen)",
+              2: "k_pair_cell (cell-staged lists; pair forces + second kick + Berendsen)"}.get(
+                  tuning["pair_mode"], "k_pair")
 
     # pair count of the final state (ordered pairs within rc)
     fin = u.download()

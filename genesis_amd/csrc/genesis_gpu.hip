@@ -238,7 +238,8 @@ __device__ __forceinline__ void switch_q(double r, double r_on, double inv_w, do
 // (src/systems/forces.rs:270-299) composed of force_switched_scalar
 // (src/physics/forces.rs:270-275, hardened core at LJ_NUCLEUS sigma), the
 // Coulomb term with r = max(d, 0.5) switched at d, and gravity.
-template <bool COUL, bool SHARED>
+// GRAVM: 0 no gravity term, 1 gravity term, 2 as P.flags says.
+template <bool COUL, bool SHARED, int GRAVM = 2>
 __device__ __forceinline__ void pair_law(const KParams& P, const PairCoef& pc, double d, double inv_d, double qq,
                                          double& fmag, double& vv) {
     const bool core = d < pc.rcore;
@@ -274,7 +275,7 @@ __device__ __forceinline__ void pair_law(const KParams& P, const PairCoef& pc, d
             vv += vc * swc;
         }
     }
-    if (P.flags & F_GRAVITY) {
+    if (GRAVM == 1 || (GRAVM == 2 && (P.flags & F_GRAVITY))) {
         const bool cc = d < 0.5;
         const double inv_g = cc ? 2.0 : inv_d;
         const double vg = -(P.gconst * pc.mm) * inv_g;
@@ -283,6 +284,200 @@ __device__ __forceinline__ void pair_law(const KParams& P, const PairCoef& pc, d
         switch_q(d, P.g_ron, P.g_inv_w, swg, dswg);
         fmag += fma(fg, swg, -vg * dswg);
         vv += vg * swg;
+    }
+}
+
+// pair_law with the LJ coefficients of the pair formed from per-species
+// factors (SpCoef): sigma_ij and the core radius sigma_ij / 2 as in the
+// pair table, so the hardened-core test takes the same branch; 4 eps_ij =
+// a4_i a4_j and 24 eps_ij s / sigma_ij = 6 (4 eps_ij) / r, equal to the
+// table's values up to rounding.  inv_rc is the table's 1 / rcore of the
+// pair (read by the caller whether or not the pair is inside its core),
+// rowc[tj].mm the mass product of the gravity term.  The function has no
+// branch and no masked region, and GRAV is the gravity switch as a
+// template argument: a runtime test between the pairs of a group would
+// split them into separate basic blocks and keep the compiler from
+// interleaving their FP64 dependency chains.
+template <bool COUL, bool SHARED, bool GRAV>
+__device__ __forceinline__ void pair_law_sp(const KParams& P, double sig_i, double a4_i, const SpCoef& cj,
+                                            double inv_rc, const PairCoef* rowc, int tj, double d, double inv_d,
+                                            double qq, double& fmag, double& vv) {
+    const double sig = 0.5 * (sig_i + cj.sigma);
+    const double eps4 = a4_i * cj.a4;
+    const double rcore = 0.5 * sig;
+    const bool core = d < rcore;
+    const double r = core ? rcore : d;
+    const double inv_r = core ? inv_rc : inv_d;
+    const double s = sig * inv_r;
+    const double s2 = s * s;
+    const double s6 = s2 * s2 * s2;
+    const double s12 = s6 * s6;
+    const double vlj = eps4 * (s12 - s6);
+    const double flj = (6.0 * eps4 * inv_r) * fma(2.0, s12, -s6);
+    double sw, dsw;
+    switch_q(r, P.lj_ron, P.lj_inv_w, sw, dsw);
+    if (COUL && SHARED) {
+        const bool cc = d < 0.5;
+        const double inv_c = cc ? 2.0 : inv_d;
+        const double vc = qq * inv_c;
+        const double fc = vc * inv_c;
+        const double fs = flj + fc, vs = vlj + vc;
+        fmag = fma(fs, sw, -vs * dsw);
+        vv = vs * sw;
+    } else {
+        fmag = fma(flj, sw, -vlj * dsw);
+        vv = vlj * sw;
+        if (COUL) {
+            const bool cc = d < 0.5;
+            const double inv_c = cc ? 2.0 : inv_d;
+            const double vc = qq * inv_c;
+            const double fc = vc * inv_c;
+            double swc, dswc;
+            switch_q(d, P.c_ron, P.c_inv_w, swc, dswc);
+            fmag += fma(fc, swc, -vc * dswc);
+            vv += vc * swc;
+        }
+    }
+    if (GRAV) {
+        const bool cc = d < 0.5;
+        const double inv_g = cc ? 2.0 : inv_d;
+        const double vg = -(P.gconst * rowc[tj].mm) * inv_g;
+        const double fg = vg * inv_g;
+        double swg, dswg;
+        switch_q(d, P.g_ron, P.g_inv_w, swg, dswg);
+        fmag += fma(fg, swg, -vg * dswg);
+        vv += vg * swg;
+    }
+}
+
+// Four pairs of one atom at once, statement by statement (each statement
+// over the four pairs before the next), with the expressions of the list
+// traversal and pair_law_sp: a dependent FP64 operation waits for its
+// operand's result for several issue slots, so the four independent chains
+// are interleaved to fill them.  p: the four neighbour positions; tt: their
+// species + 1 in successive bytes, 0 for a slot past the end of the list.
+template <bool COUL, bool SHARED, bool GRAV>
+__device__ __forceinline__ void pair_eval4(const KParams& P, const double4& pi, double kqi, double sig_i, double a4_i,
+                                           const SpCoef* ssp, const double* irc_row, const PairCoef* row,
+                                           const double4 (&p)[4], uint32_t tt, double& fx, double& fy, double& fz,
+                                           double& vl) {
+    const double Lx = P.box[0], Ly = P.box[1], Lz = P.box[2];
+    const double iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
+    bool live[4], ok[4], core[4];
+    int tj[4];
+    double dx[4], dy[4], dz[4], r2[4], y[4], e[4], inv_d[4], d[4], sig[4], eps4[4], irc[4], r[4], inv_r[4];
+    double s6[4], s12[4], vlj[4], flj[4], sw[4], dsw[4], fmag[4], vv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t tq = (tt >> (8 * q)) & 0xFFu;
+        live[q] = tq != 0u;
+        tj[q] = live[q] ? (int)tq - 1 : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        dx[q] = pi.x - p[q].x;
+        dy[q] = pi.y - p[q].y;
+        dz[q] = pi.z - p[q].z;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        dx[q] = fma(-Lx, rint(dx[q] * iLx), dx[q]);
+        dy[q] = fma(-Ly, rint(dy[q] * iLy), dy[q]);
+        dz[q] = fma(-Lz, rint(dz[q] * iLz), dz[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double t = fma(dz[q], dz[q], fma(dy[q], dy[q], dx[q] * dx[q]));
+        ok[q] = live[q] && t > P.eps2;
+        r2[q] = ok[q] ? t : 1.0;
+    }
+    // rsqrt_refined, four at a time
+#pragma unroll
+    for (int q = 0; q < 4; ++q) y[q] = __builtin_amdgcn_rsq(r2[q]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) e[q] = fma(-r2[q] * y[q], y[q], 1.0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) inv_d[q] = fma(y[q] * e[q], fma(0.375, e[q], 0.5), y[q]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        d[q] = r2[q] * inv_d[q];
+        const SpCoef cj = ssp[tj[q]];
+        irc[q] = irc_row[tj[q]];
+        sig[q] = 0.5 * (sig_i + cj.sigma);
+        eps4[q] = a4_i * cj.a4;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double rcore = 0.5 * sig[q];
+        core[q] = d[q] < rcore;
+        r[q] = core[q] ? rcore : d[q];
+        inv_r[q] = core[q] ? irc[q] : inv_d[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double s = sig[q] * inv_r[q];
+        const double s2 = s * s;
+        s6[q] = s2 * s2 * s2;
+        s12[q] = s6[q] * s6[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        vlj[q] = eps4[q] * (s12[q] - s6[q]);
+        flj[q] = (6.0 * eps4[q] * inv_r[q]) * fma(2.0, s12[q], -s6[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) switch_q(r[q], P.lj_ron, P.lj_inv_w, sw[q], dsw[q]);
+    if (COUL && SHARED) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const bool cc = d[q] < 0.5;
+            const double inv_c = cc ? 2.0 : inv_d[q];
+            const double vc = (kqi * p[q].w) * inv_c;
+            const double fc = vc * inv_c;
+            const double fs = flj[q] + fc, vs = vlj[q] + vc;
+            fmag[q] = fma(fs, sw[q], -vs * dsw[q]);
+            vv[q] = vs * sw[q];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            fmag[q] = fma(flj[q], sw[q], -vlj[q] * dsw[q]);
+            vv[q] = vlj[q] * sw[q];
+        }
+        if (COUL) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const bool cc = d[q] < 0.5;
+                const double inv_c = cc ? 2.0 : inv_d[q];
+                const double vc = (kqi * p[q].w) * inv_c;
+                const double fc = vc * inv_c;
+                double swc, dswc;
+                switch_q(d[q], P.c_ron, P.c_inv_w, swc, dswc);
+                fmag[q] += fma(fc, swc, -vc * dswc);
+                vv[q] += vc * swc;
+            }
+        }
+    }
+    if (GRAV) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const bool cc = d[q] < 0.5;
+            const double inv_g = cc ? 2.0 : inv_d[q];
+            const double vg = -(P.gconst * row[tj[q]].mm) * inv_g;
+            const double fg = vg * inv_g;
+            double swg, dswg;
+            switch_q(d[q], P.g_ron, P.g_inv_w, swg, dswg);
+            fmag[q] += fma(fg, swg, -vg * dswg);
+            vv[q] += vg * swg;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double fs = ok[q] ? fmag[q] * inv_d[q] : 0.0;
+        fx = fma(fs, dx[q], fx);
+        fy = fma(fs, dy[q], fy);
+        fz = fma(fs, dz[q], fz);
+        vl += ok[q] ? vv[q] : 0.0;
     }
 }
 
@@ -565,23 +760,33 @@ __global__ __launch_bounds__(BLOCK) void k_pair_brute(KParams P, const double4* 
 }
 
 // Deterministic block sum: fixed per-thread strides then a fixed tree.
+// Sum of a[0..n) by a workgroup of NT threads (red: NT / 64 doubles of
+// LDS); the order of the additions depends on n and NT only.
+template <int NT = BLOCK>
 __device__ double block_sum_fixed(const double* __restrict__ a, uint32_t n, double* red) {
+    static_assert(NT % WAVE == 0, "whole waves");
     const int tid = threadIdx.x;
     // four independent strided accumulators keep four loads in flight
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     uint32_t k = tid;
-    for (; k + 3 * BLOCK < n; k += 4 * BLOCK) {
+    for (; k + 3 * NT < n; k += 4 * NT) {
         s0 += a[k];
-        s1 += a[k + BLOCK];
-        s2 += a[k + 2 * BLOCK];
-        s3 += a[k + 3 * BLOCK];
+        s1 += a[k + NT];
+        s2 += a[k + 2 * NT];
+        s3 += a[k + 3 * NT];
     }
-    for (; k < n; k += BLOCK) s0 += a[k];
+    for (; k < n; k += NT) s0 += a[k];
     double s = (s0 + s1) + (s2 + s3);
     s = wave_sum(s);
     if ((tid & 63) == 0) red[tid >> 6] = s;
     __syncthreads();
-    double t = (red[0] + red[1]) + (red[2] + red[3]);
+    double t;
+    if (NT == 4 * WAVE) {
+        t = (red[0] + red[1]) + (red[2] + red[3]);
+    } else {
+        t = red[0];
+        for (int w = 1; w < NT / WAVE; ++w) t += red[w];
+    }
     __syncthreads();
     return t;
 }
@@ -785,6 +990,17 @@ constexpr uint32_t NBR_IDX_MASK = 0x0FFFFFFFu;  // entry = (species << 28) | sor
 constexpr int LDS_IDX_CAP = 2048;               // stencil candidates held (and ordered) in LDS
 constexpr int MAX_LOGT = 4;                     // up to 16 lanes per atom
 
+// Cell-staged list pass (pair_mode 2): entries are 16 bits,
+// (species << CELL_IDX_BITS) | position in the home cell's ordered
+// candidates; home atom h of a cell keeps its list in row h of the cell's
+// region (cap entries per row).
+constexpr int CELL_THREADS = 384;    // workgroup of the cell-staged pair kernel
+constexpr int STAGE_CAP = 1280;      // candidate positions staged in LDS per chunk
+constexpr uint32_t CELL_IDX_BITS = 11;
+constexpr uint32_t CELL_IDX_MASK = (1u << CELL_IDX_BITS) - 1u;
+static_assert(LDS_IDX_CAP <= (1 << CELL_IDX_BITS), "a candidate position fits the entry");
+static_assert(GS_NSPECIES <= (1 << (16 - CELL_IDX_BITS)), "a species fits the entry");
+
 // Streaming 16-byte load with a non-temporal hint (global_load ... nt).
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ uint4 load_nt(const uint4* p) {
@@ -911,11 +1127,19 @@ __global__ void k_flag_set(uint32_t* flags, int idx, uint32_t v) { flags[idx] = 
 // one home atom at a time, the candidates within rc + skin with a ballot.
 // The home atoms' positions are recorded as the build positions of the
 // displacement test.
+//
+// LOCAL (cell-staged pair pass): the ordered candidates are stored per cell
+// (cand_store, cand_n) for the pair workgroup to stage, entries hold the
+// candidate's position in that order, and each atom's entries are contiguous
+// (nbr[i * cap + k]); otherwise entries hold the atom index in the
+// lane-grouped layout of nbr_slot.
+template <bool LOCAL>
 __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
     KParams PL, const uint32_t* __restrict__ bcount, const uint32_t* __restrict__ bucket, uint32_t bcap,
     const double4* __restrict__ pos4, const uint8_t* __restrict__ type, uint32_t* __restrict__ nbr,
     uint32_t* __restrict__ nlen, double4* __restrict__ xref, uint32_t N, uint32_t logT, uint32_t cap, double cut2,
-    uint32_t* __restrict__ flags) {
+    uint32_t* __restrict__ flags, uint32_t* __restrict__ cand_store, uint4* __restrict__ cinfo,
+    uint2* __restrict__ hrec, uint16_t* __restrict__ nbr16, size_t region) {
     if (flags[0] == 0u) return;
     // the candidates' indices are staged, unordered, in the memory of the
     // position tile, which is filled only after they have been ordered
@@ -930,7 +1154,10 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
     const uint32_t c = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint32_t nh = min(bcount[c], bcap);
-    if (nh == 0) return;
+    if (nh == 0) {
+        if (LOCAL && tid == 0) cinfo[c] = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
     const uint32_t* home = bucket + (size_t)c * bcap;
     if (tid < 27) {
         const uint32_t nx = PL.dims[0], ny = PL.dims[1], nz = PL.dims[2];
@@ -976,6 +1203,15 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
         const uint32_t gi = home[i];
         xref[gi] = pos4[gi];
     }
+    // LOCAL: the home atoms, in index order, are the centre segment (13) of
+    // the ordered candidates
+    const uint32_t hs = seg_pref[13];
+    uint16_t* reg = LOCAL ? nbr16 + (size_t)c * region : nullptr;
+    if (LOCAL) {
+        __syncthreads();
+        for (uint32_t t = tid; t < ncand; t += PAIR_THREADS) cand_store[(size_t)c * LDS_IDX_CAP + t] = ord[t];
+        if (tid == 0) cinfo[c] = make_uint4(ncand, hs, nh, 0u);
+    }
     const int nchunks = (int)((ncand + TILE_CAP - 1) / TILE_CAP);
     for (int ch = 0; ch < nchunks; ++ch) {
         const uint32_t c0 = (uint32_t)ch * TILE_CAP;
@@ -991,11 +1227,11 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
             pj.y += seg_sh[sg][1];
             pj.z += seg_sh[sg][2];
             tile[t] = pj;
-            tidx[t] = ((uint32_t)type[j] << 28) | j;
+            tidx[t] = LOCAL ? ((uint32_t)type[j] << CELL_IDX_BITS) | idx : ((uint32_t)type[j] << 28) | j;
         }
         __syncthreads();
         for (uint32_t i = w; i < nh; i += PAIR_WAVES) {
-            const uint32_t gi = home[i];
+            const uint32_t gi = LOCAL ? ord[hs + i] : home[i];
             const double4 pi = pos4[gi];
             uint32_t cnt = ch == 0 ? 0u : nlen[gi];
             for (uint32_t base = 0; base < len; base += 64) {
@@ -1007,12 +1243,18 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
                     e = tidx[j];
                     const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                    in = (r2 < cut2) && ((e & NBR_IDX_MASK) != gi);
+                    const uint32_t jg = LOCAL ? ord[c0 + j] : (e & NBR_IDX_MASK);
+                    in = (r2 < cut2) && (jg != gi);
                 }
                 const uint64_t m = __ballot(in);
                 if (in) {
                     const uint32_t k = cnt + mask_rank(m);
-                    if (k < cap) nbr[nbr_slot(k, gi, N, logT)] = e;
+                    if (k < cap) {
+                        if (LOCAL)
+                            reg[(size_t)i * cap + k] = (uint16_t)e;
+                        else
+                            nbr[nbr_slot(k, gi, N, logT)] = e;
+                    }
                 }
                 cnt += (uint32_t)__popcll(m);
             }
@@ -1022,6 +1264,8 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
                     cnt = cap;
                 }
                 nlen[gi] = cnt;
+                // LOCAL: home record ((species << 28) | atom, list length)
+                if (LOCAL && ch == nchunks - 1) hrec[(size_t)c * bcap + i] = make_uint2(((uint32_t)type[gi] << 28) | gi, cnt);
             }
         }
     }
@@ -1036,7 +1280,7 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
 // kinetic energy; the last workgroup reduces the per-block kinetic and
 // potential partials in a fixed order, computes the Berendsen lambda and the
 // observables (close_tick) and clears the rebuild flag for the next tick.
-template <int LOGT, bool COUL, bool SHARED, bool SINGLE>
+template <int LOGT, bool COUL, bool SHARED, bool SINGLE, bool GRAV>
 __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, const double4* __restrict__ pos4,
                                                      const uint8_t* __restrict__ type,
                                                      const uint32_t* __restrict__ nbr,
@@ -1051,6 +1295,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
                                                      const double* __restrict__ vbpart, uint32_t nvb) {
     constexpr int T = 1 << LOGT;
     __shared__ PairCoef sc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];
+    __shared__ SpCoef ssp[GS_NSPECIES];
+    __shared__ double sirc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];  // 1 / rcore, 8-byte stride
     __shared__ double wv[BLOCK / WAVE];
     __shared__ double wk[BLOCK / WAVE];
     __shared__ double red[BLOCK / WAVE];
@@ -1059,7 +1305,11 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     if (SINGLE) {
         if (tid == 0) sc[0] = coef[0];
     } else {
-        for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += BLOCK) sc[t] = coef[t];
+        for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += BLOCK) {
+            sc[t] = coef[t];
+            sirc[t] = coef[t].inv_rcore;
+        }
+        if (tid < GS_NSPECIES) ssp[tid] = reinterpret_cast<const SpCoef*>(coef + COEF_SP_OFFSET)[tid];
     }
     __syncthreads();
     const uint32_t g = blockIdx.x * BLOCK + tid;
@@ -1078,58 +1328,90 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
         const double iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
         const uint32_t len = lists_ok ? nlen[i] : 0u;
         const PairCoef* row = SINGLE ? sc : sc + ti * GS_NSPECIES;
+        const double sig_i = SINGLE ? 0.0 : ssp[ti].sigma, a4_i = SINGLE ? 0.0 : ssp[ti].a4;
         // this lane's entries: s, s+T, ... (cnt of them), four per group;
         // a group's four position gathers are all in flight before its four
         // pairs are evaluated
         const uint32_t cnt = len > s ? (len - s + T - 1) >> LOGT : 0u;
         const uint4* grp = reinterpret_cast<const uint4*>(nbr) + (((size_t)i << LOGT) + s);
         const size_t gstride = (size_t)N << LOGT;
-        // the entry groups stream from HBM once per tick: they are loaded two
-        // groups ahead of use (address selects keep the loads unconditional)
-        // with a non-temporal hint, so they do not displace the positions the
-        // gathers reuse from L2
-        const uint4* g1 = cnt > 4 ? grp + gstride : grp;
-        uint4 ev = cnt ? load_nt(grp) : make_uint4(0u, 0u, 0u, 0u);
-        uint4 ev1 = cnt ? load_nt(g1) : make_uint4(0u, 0u, 0u, 0u);
-        grp = g1;
-        for (uint32_t m0 = 0; m0 < cnt; m0 += 4) {
-            const uint32_t e4[4] = {ev.x, ev.y, ev.z, ev.w};
-            double4 pj4[4];
+        // Software pipeline over the lane's entry groups, two groups per
+        // loop trip with the registers of the two named A and B (so no
+        // register receiving a load in flight is ever copied, which would
+        // make the wave wait for it): a step for group k loads the entries of
+        // group k+2, issues the four position gathers of group k+1 and then
+        // evaluates group k, whose gathers have had the previous step's
+        // arithmetic to arrive in.  Loads are unconditional (address
+        // selects: past the end a lane re-reads its last group and gathers
+        // its own position), so the waits count them statically.  The
+        // entries stream from HBM once per tick and carry a non-temporal
+        // hint, so they do not displace the positions the gathers reuse
+        // from L2.
+        const uint32_t ng = (cnt + 3) >> 2;  // groups of this lane
+        const uint4* glast = grp + (size_t)(ng ? ng - 1 : 0) * gstride;
+        auto group_ptr = [&](uint32_t k) { return k < ng ? grp + (size_t)k * gstride : glast; };
+        auto gather = [&](const uint4& e, uint32_t k, double4 (&p)[4], uint32_t& tt) {
+            const uint32_t f4[4] = {e.x, e.y, e.z, e.w};
+            tt = 0u;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const uint32_t jj = m0 + q < cnt ? (e4[q] & NBR_IDX_MASK) : i;  // past the end: self, masked
-                pj4[q] = pos4[jj];
+                const bool live = 4 * k + q < cnt;
+                p[q] = pos4[live ? (f4[q] & NBR_IDX_MASK) : i];
+                tt |= (live ? (f4[q] >> 28) + 1u : 0u) << (8 * q);  // 0: past the end
             }
-            grp = m0 + 8 < cnt ? grp + gstride : grp;
-            ev = ev1;
-            // issue the group load after the four gathers, so waiting for the
-            // gathers never waits for it (memory loads complete in order)
-            __builtin_amdgcn_sched_barrier(0);
-            ev1 = load_nt(grp);
-            __builtin_amdgcn_sched_barrier(0);
+        };
+        auto evaluate = [&](const double4 (&p)[4], uint32_t tt) {
+            if (SINGLE) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const bool live = m0 + q < cnt;
-                const int tj = live ? (int)(e4[q] >> 28) : 0;
-                const double4 pj = pj4[q];
-                double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                dx = fma(-Lx, rint(dx * iLx), dx);
-                dy = fma(-Ly, rint(dy * iLy), dy);
-                dz = fma(-Lz, rint(dz * iLz), dz);
-                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                const bool ok = live && r2 > P.eps2;
-                const double r2s = ok ? r2 : 1.0;
-                const double inv_d = rsqrt_refined(r2s);
-                const double d = r2s * inv_d;
-                const PairCoef& pc = SINGLE ? row[0] : row[tj];
-                double fmag, vv;
-                pair_law<COUL, SHARED>(P, pc, d, inv_d, kqi * pj.w, fmag, vv);
-                const double fs = ok ? fmag * inv_d : 0.0;
-                fx = fma(fs, dx, fx);
-                fy = fma(fs, dy, fy);
-                fz = fma(fs, dz, fz);
-                vl += ok ? vv : 0.0;
+                for (int q = 0; q < 4; ++q) {
+                    const bool live = ((tt >> (8 * q)) & 0xFFu) != 0u;
+                    const double4 pj = p[q];
+                    double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    dx = fma(-Lx, rint(dx * iLx), dx);
+                    dy = fma(-Ly, rint(dy * iLy), dy);
+                    dz = fma(-Lz, rint(dz * iLz), dz);
+                    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                    const bool ok = live && r2 > P.eps2;
+                    const double r2s = ok ? r2 : 1.0;
+                    const double inv_d = rsqrt_refined(r2s);
+                    const double d = r2s * inv_d;
+                    double fmag, vv;
+                    pair_law<COUL, SHARED, GRAV ? 1 : 0>(P, row[0], d, inv_d, kqi * pj.w, fmag, vv);
+                    const double fs = ok ? fmag * inv_d : 0.0;
+                    fx = fma(fs, dx, fx);
+                    fy = fma(fs, dy, fy);
+                    fz = fma(fs, dz, fz);
+                    vl += ok ? vv : 0.0;
+                }
+            } else {
+                pair_eval4<COUL, SHARED, GRAV>(P, pi, kqi, sig_i, a4_i, ssp, sirc + ti * GS_NSPECIES, row, p, tt, fx, fy,
+                                               fz, vl);
             }
+        };
+        uint4 eA = load_nt(group_ptr(0));
+        uint4 eB = load_nt(group_ptr(1));
+        double4 pA[4], pB[4];
+        uint32_t tA = 0u, tB = 0u;
+        // group 1's entries are issued before group 0's gathers, the order
+        // every later step keeps
+        __builtin_amdgcn_sched_barrier(0);
+        gather(eA, 0, pA, tA);
+        // no exit between a step's loads and the next step (the compiler
+        // would sink the loads past it): a lane with an odd number of
+        // groups evaluates one group of four masked pairs at its end
+        for (uint32_t k = 0; k < ng; k += 2) {
+            // step A: group k
+            __builtin_amdgcn_sched_barrier(0);
+            eA = load_nt(group_ptr(k + 2));
+            gather(eB, k + 1, pB, tB);
+            __builtin_amdgcn_sched_barrier(0);
+            evaluate(pA, tA);
+            // step B: group k + 1
+            __builtin_amdgcn_sched_barrier(0);
+            eB = load_nt(group_ptr(k + 3));
+            gather(eA, k + 2, pA, tA);
+            __builtin_amdgcn_sched_barrier(0);
+            evaluate(pB, tB);
         }
     }
 #pragma unroll
@@ -1178,6 +1460,277 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     const double K = block_sum_fixed(kpart, gridDim.x, red);
     const double V = block_sum_fixed(vpart, gridDim.x, red);
     const double Vb = vbpart ? block_sum_fixed(vbpart, nvb, red) : 0.0;
+    if (tid == 0) {
+        close_tick(K, V, Vb, (int)N, th, lam_out, obs);
+        flags[5] = 0u;
+        flags[0] = 0u;
+    }
+}
+
+// Exclusive prefix sum of one value per thread over a workgroup of NT
+// threads (wsum: NT / 64 words of LDS); total receives the sum.
+template <int NT>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* wsum, uint32_t& total) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, WAVE);
+        if (lane >= off) x += y;
+    }
+    if (lane == WAVE - 1) wsum[w] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int k = 0; k < NT / WAVE; ++k) {
+        const uint32_t t = wsum[k];
+        before += k < w ? t : 0u;
+        all += t;
+    }
+    __syncthreads();
+    total = all;
+    return before + x - v;
+}
+
+// Cell-staged list pass (pair_mode 2), fused like k_pair_list with the
+// second kick, the kinetic reduction and close_tick.  One workgroup per
+// list-grid cell (workgroup b takes cell (b mod 8) * per + b / 8, so each
+// of the eight L2 slices of the device sees a contiguous slab of cells and
+// their shared stencil candidates): the positions of the cell's ordered
+// stencil candidates are staged in LDS (in chunks of STAGE_CAP), and the
+// lanes of each home atom evaluate its list (lane s of T: entries s, s+T,
+// ...) with the candidate read from LDS; per-pair minimum image as in
+// k_pair_list.  The workgroup's lanes are shared out among the home atoms
+// in proportion to their list lengths (atom h gets lanes
+// [L_h, L_h+1) with L_h = floor((NT - nh) E_h / E) + h, E_h the entries of
+// the atoms before it), so every lane has about the same number of pairs;
+// a cell with more than NT/2 home atoms takes them NT at a time, one lane
+// each.  A list is ordered by candidate position, so a lane's entries
+// inside a chunk are a prefix of those it has left.  The partial forces of
+// a home atom's lanes are summed through LDS in lane order.
+template <bool COUL, bool SHARED, bool SINGLE, bool GRAV>
+__global__ __launch_bounds__(CELL_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_pair_cell(
+    KParams P, uint32_t ncell, uint32_t per, const double4* __restrict__ pos4, const uint16_t* __restrict__ nbr16,
+    size_t region, uint32_t cap, const uint2* __restrict__ hrec, uint32_t bcap,
+    const uint32_t* __restrict__ cand_store, const uint4* __restrict__ cinfo, const PairCoef* __restrict__ coef,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az, double* __restrict__ vx,
+    double* __restrict__ vy, double* __restrict__ vz, double* __restrict__ vpart, double* __restrict__ kpart,
+    uint32_t* __restrict__ flags, double* __restrict__ lam_out, gs_observables* __restrict__ obs, ThermoArgs th,
+    const double* __restrict__ abond, const double* __restrict__ vbpart, uint32_t nvb, uint32_t N) {
+    constexpr int NW = CELL_THREADS / WAVE;
+    constexpr int SK = (STAGE_CAP + CELL_THREADS - 1) / CELL_THREADS;  // staged candidates per thread
+    __shared__ PairCoef sc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];
+    __shared__ SpCoef ssp[GS_NSPECIES];
+    __shared__ double sirc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];  // 1 / rcore, 8-byte stride
+    __shared__ double4 tile[STAGE_CAP];
+    __shared__ uint2 srec[CELL_THREADS];       // home records of the pass
+    __shared__ uint32_t sL[CELL_THREADS + 1];  // first lane of each home atom
+    __shared__ uint32_t wsum[NW];
+    __shared__ double wv[NW];
+    __shared__ double wk[NW];
+    __shared__ double red[NW];
+    __shared__ bool last;
+    static_assert(3 * CELL_THREADS * sizeof(double) <= STAGE_CAP * sizeof(double4), "partials fit the tile");
+    const int tid = threadIdx.x;
+    if (SINGLE) {
+        if (tid == 0) sc[0] = coef[0];
+    } else {
+        for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += CELL_THREADS) {
+            sc[t] = coef[t];
+            sirc[t] = coef[t].inv_rcore;
+        }
+        if (tid < GS_NSPECIES) ssp[tid] = reinterpret_cast<const SpCoef*>(coef + COEF_SP_OFFSET)[tid];
+    }
+    const uint32_t c = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    const uint32_t cc = c < ncell ? c : 0u;
+    // a build that met a stencil too dense for LDS or a full bucket left some
+    // lists unwritten: the host replays this call, so no list is read now
+    const uint32_t bad = flags[2] | flags[6];
+    uint4 ci = cinfo[cc];
+    if (bad != 0u || c >= ncell) ci = make_uint4(0u, 0u, 0u, 0u);
+    const uint32_t ncand = ci.x, nh = ci.z;
+    const bool prop = nh <= (uint32_t)CELL_THREADS / 2;  // proportional lanes, one pass
+    const uint32_t* cand = cand_store + (size_t)cc * LDS_IDX_CAP;
+    const uint16_t* reg = nbr16 + (size_t)cc * region;
+    const uint2* hr0 = hrec + (size_t)cc * bcap;
+    const int nchunks = (int)((ncand + STAGE_CAP - 1) / STAGE_CAP);
+    const double Lx = P.box[0], Ly = P.box[1], Lz = P.box[2];
+    const double iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
+    double* part = reinterpret_cast<double*>(tile);
+    double vl = 0.0, ke = 0.0;
+    for (uint32_t h0 = 0; h0 < nh; h0 += CELL_THREADS) {  // one pass unless nh > CELL_THREADS
+        const uint32_t nhp = min(nh - h0, (uint32_t)CELL_THREADS);
+        // this pass's home records, and the lane split
+        const uint2 myrec = (uint32_t)tid < nhp ? hr0[h0 + tid] : make_uint2(0u, 0u);
+        uint32_t E = 0;
+        const uint32_t Eh = block_exclusive_scan<CELL_THREADS>(myrec.y, wsum, E);
+        if ((uint32_t)tid < nhp) {
+            srec[tid] = myrec;
+            sL[tid] = prop ? (E ? (uint32_t)(((uint64_t)(CELL_THREADS - nhp) * Eh) / E) : 0u) + (uint32_t)tid
+                           : (uint32_t)tid;
+        }
+        if (tid == 0) sL[nhp] = prop ? (uint32_t)CELL_THREADS : nhp;
+        __syncthreads();
+        // this lane's home atom: sL[h] <= tid < sL[h + 1]
+        uint32_t h = 0, T = 1;
+        bool act;
+        if (prop) {
+            uint32_t lo = 0, hi = nhp;  // sL[lo] <= tid < sL[hi]
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sL[mid] <= (uint32_t)tid)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            h = lo;
+            T = sL[h + 1] - sL[h];
+            act = nhp > 0;
+        } else {
+            h = (uint32_t)tid;
+            act = h < nhp;
+        }
+        const uint32_t s = act ? (uint32_t)tid - sL[h] : 0u;
+        const uint2 hr = act ? srec[h] : make_uint2(0u, 0u);
+        const uint32_t gi = hr.x & NBR_IDX_MASK;
+        const int ti = (int)(hr.x >> 28);
+        const uint32_t cnt = hr.y > s ? (hr.y - s + T - 1) / T : 0u;
+        const double4 pi = act ? pos4[gi] : make_double4(0.0, 0.0, 0.0, 0.0);
+        const double kqi = P.ke * pi.w;
+        const PairCoef* rowc = SINGLE ? sc : sc + ti * GS_NSPECIES;
+        const double sig_i = SINGLE ? 0.0 : ssp[ti].sigma, a4_i = SINGLE ? 0.0 : ssp[ti].a4;
+        const uint16_t* my = reg + (size_t)(h0 + h) * cap + s;  // entry m at my[m * T]
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        uint32_t m = 0;
+        for (int ch = 0; ch < nchunks; ++ch) {
+            const uint32_t c0 = (uint32_t)ch * STAGE_CAP;
+            const uint32_t c1 = min(ncand, c0 + (uint32_t)STAGE_CAP);
+            // the chunk's candidate indices, then their positions, and this
+            // lane's next four entries are all in flight together
+            uint32_t cix[SK];
+#pragma unroll
+            for (int k = 0; k < SK; ++k) {
+                const uint32_t t = c0 + (uint32_t)tid + (uint32_t)k * CELL_THREADS;
+                cix[k] = t < c1 ? cand[t] : 0u;
+            }
+            uint32_t en[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                en[q] = m + q < cnt ? __builtin_nontemporal_load(my + (size_t)(m + q) * T) : 0u;
+            double4 pst[SK];
+#pragma unroll
+            for (int k = 0; k < SK; ++k) {
+                const uint32_t t = c0 + (uint32_t)tid + (uint32_t)k * CELL_THREADS;
+                pst[k] = t < c1 ? pos4[cix[k]] : make_double4(0.0, 0.0, 0.0, 0.0);
+            }
+            if (ch > 0 || h0 > 0) __syncthreads();  // the previous chunk's readers are done
+#pragma unroll
+            for (int k = 0; k < SK; ++k) {
+                const uint32_t t = (uint32_t)tid + (uint32_t)k * CELL_THREADS;
+                if (c0 + t < c1) tile[t] = pst[k];
+            }
+            __syncthreads();
+            for (;;) {
+                uint32_t e4[4];
+                bool live[4];
+                uint32_t nl = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    e4[q] = en[q];
+                    const uint32_t j = e4[q] & CELL_IDX_MASK;
+                    live[q] = m + q < cnt && j < c1 && j >= c0;
+                    nl += live[q] ? 1u : 0u;
+                }
+                // the following four, assuming this group is whole (a short
+                // group ends the chunk and the loads are made again)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    en[q] = m + 4 + q < cnt ? __builtin_nontemporal_load(my + (size_t)(m + 4 + q) * T) : 0u;
+#pragma unroll 1
+                for (int q = 0; q < 4; ++q) {
+                    const double4 pj = tile[live[q] ? (e4[q] & CELL_IDX_MASK) - c0 : 0u];
+                    const int tj = live[q] ? (int)(e4[q] >> CELL_IDX_BITS) : 0;
+                    double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    dx = fma(-Lx, rint(dx * iLx), dx);
+                    dy = fma(-Ly, rint(dy * iLy), dy);
+                    dz = fma(-Lz, rint(dz * iLz), dz);
+                    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                    const bool ok = live[q] && r2 > P.eps2;
+                    const double r2s = ok ? r2 : 1.0;
+                    const double inv_d = rsqrt_refined(r2s);
+                    const double d = r2s * inv_d;
+                    double fmag, vv;
+                    if (SINGLE)
+                        pair_law<COUL, SHARED, GRAV ? 1 : 0>(P, rowc[0], d, inv_d, kqi * pj.w, fmag, vv);
+                    else
+                        pair_law_sp<COUL, SHARED, GRAV>(P, sig_i, a4_i, ssp[tj], sirc[ti * GS_NSPECIES + tj], rowc,
+                                                        tj, d, inv_d, kqi * pj.w, fmag, vv);
+                    const double fs = ok ? fmag * inv_d : 0.0;
+                    fx = fma(fs, dx, fx);
+                    fy = fma(fs, dy, fy);
+                    fz = fma(fs, dz, fz);
+                    vl += ok ? vv : 0.0;
+                }
+                m += nl;
+                if (nl < 4) break;
+            }
+        }
+        __syncthreads();  // the tile's memory holds the partial forces now
+        part[tid] = fx;
+        part[CELL_THREADS + tid] = fy;
+        part[2 * CELL_THREADS + tid] = fz;
+        __syncthreads();
+        if (act && s == 0) {
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            for (uint32_t q = 0; q < T; ++q) {
+                sx += part[tid + q];
+                sy += part[CELL_THREADS + tid + q];
+                sz += part[2 * CELL_THREADS + tid + q];
+            }
+            const double im = P.inv_mass[ti];
+            double a0 = sx * im, a1 = sy * im, a2 = sz * im;
+            if (abond) {
+                a0 += abond[gi];
+                a1 += abond[N + gi];
+                a2 += abond[2 * (size_t)N + gi];
+            }
+            ax[gi] = a0;
+            ay[gi] = a1;
+            az[gi] = a2;
+            double u0 = vx[gi], u1 = vy[gi], u2 = vz[gi];
+            u0 = u0 + a0 * P.half_dt;
+            u1 = u1 + a1 * P.half_dt;
+            u2 = u2 + a2 * P.half_dt;
+            vx[gi] = u0;
+            vy[gi] = u1;
+            vz[gi] = u2;
+            ke += 0.5 * P.mass[ti] * (u0 * u0 + u1 * u1 + u2 * u2);
+        }
+        __syncthreads();  // srec, sL and the tile are rewritten by the next pass
+    }
+    vl = wave_sum(vl);
+    ke = wave_sum(ke);
+    if ((tid & 63) == 0) {
+        wv[tid >> 6] = vl;
+        wk[tid >> 6] = ke;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double sv = wv[0], sk = wk[0];
+        for (int w = 1; w < NW; ++w) {
+            sv += wv[w];
+            sk += wk[w];
+        }
+        vpart[blockIdx.x] = 0.5 * sv;
+        kpart[blockIdx.x] = sk;
+        __threadfence();
+        last = atomicAdd(&flags[5], 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const double K = block_sum_fixed<CELL_THREADS>(kpart, gridDim.x, red);
+    const double V = block_sum_fixed<CELL_THREADS>(vpart, gridDim.x, red);
+    const double Vb = vbpart ? block_sum_fixed<CELL_THREADS>(vbpart, nvb, red) : 0.0;
     if (tid == 0) {
         close_tick(K, V, Vb, (int)N, th, lam_out, obs);
         flags[5] = 0u;
@@ -1455,6 +2008,7 @@ struct gs_universe {
     PairCoef* coef = nullptr;
     double* vpart = nullptr;
     uint32_t nvpart = 0;
+    uint32_t npart = 0;  // slots of vpart and kpart: every pair pass's workgroups
     double* kpart = nullptr;
     uint32_t* counter = nullptr;
     double* scal = nullptr;  // [0] lambda, [1] V, [2] K, [3] spare
@@ -1481,13 +2035,23 @@ struct gs_universe {
     KParams kpl;                // grid of edge >= rc + skin
     bool list_ok = false;       // the list geometry admits >= 3 cells per axis
     bool list_valid = false;    // lists match the current sorted order
-    int pair_mode = 1;          // 0 = per-tick cell tiles, 1 = Verlet list
+    int pair_mode = 1;          // 0 = per-tick cell tiles, 1 = Verlet list, 2 = cell-staged list
     int logT = 2;               // lanes per atom = 1 << logT
     double skin = 0.0;
     int64_t sort_every = 100;   // ticks between full re-sorts of the state in list mode
     int64_t list_age = 0;       // ticks since the last full re-sort
-    uint32_t nbr_cap = 0;
+    uint32_t nbr_cap = 0;       // list capacity per atom (entries)
+    uint32_t nbr_have = 0;      // capacity the per-atom list array holds
     uint32_t* nbr = nullptr;
+    // cell-staged lists: per-cell regions of 16-bit entries, ordered stencil
+    // candidates and {candidates, home start, home count} per cell
+    uint16_t* nbr16 = nullptr;
+    size_t nbr16_have = 0;
+    uint32_t* cand_store = nullptr;
+    uint4* cinfo = nullptr;
+    uint32_t cell_have = 0;
+    uint2* hrec = nullptr;      // per cell and home slot: (species << 28) | atom, list length
+    size_t hrec_have = 0;
     uint32_t* nlen = nullptr;
     double4* xref = nullptr;
     uint32_t* flags = nullptr;  // device list flags and counters (k_list_integrate)
@@ -1639,8 +2203,17 @@ std::vector<PairCoef> derive(gs_universe* u) {
                 break;
             }
     static_assert(sizeof(PairCoef) == 64, "PairCoef layout");
+    coef.resize(COEF_LEN);
+    SpCoef* sp = reinterpret_cast<SpCoef*>(&coef[COEF_SP_OFFSET]);
+    for (int j = 0; j < GS_NSPECIES; ++j) {
+        sp[j].sigma = p.sigma[j];
+        sp[j].a4 = std::sqrt(4.0 * p.thermal_constant * p.epsilon_k[j]);
+    }
     u->nvpart = u->small_box ? nblocks(u->n) : k.ncell;
     if (u->nvpart == 0) u->nvpart = 1;
+    // the tiles (a cell each), the lists (N << MAX_LOGT lanes at most) and
+    // the cell-staged pass (a list cell each, eight slabs)
+    u->npart = std::max<uint32_t>(std::max<uint32_t>(u->nvpart, nblocks((int64_t)u->n << MAX_LOGT)), k.ncell + 8u);
     return coef;
 }
 
@@ -1745,6 +2318,9 @@ bool list_geometry(gs_universe* u, double skin) {
     return true;
 }
 
+// Entries per cell of the cell-staged lists: a row of cap per bucket slot.
+size_t cell_region(const gs_universe* u) { return (size_t)u->bcap * (size_t)u->nbr_cap; }
+
 int list_alloc(gs_universe* u, uint32_t cap, uint32_t bcap_min = 0) {
     int rc;
     const size_t n = (size_t)std::max(1, u->n);
@@ -1771,14 +2347,46 @@ int list_alloc(gs_universe* u, uint32_t cap, uint32_t bcap_min = 0) {
         if ((rc = dalloc(&u->bucket, (size_t)u->kp.ncell * want))) return rc;
         u->bcap = want;
     }
-    if (u->nbr_cap < cap) {
+    if (cap > u->nbr_cap) u->nbr_cap = cap;
+    cap = u->nbr_cap;
+    if (u->pair_mode == 2) {
+        const uint32_t cells = u->kpl.ncell;
+        if (u->cell_have < cells) {
+            (void)hipFree(u->cand_store);
+            (void)hipFree(u->cinfo);
+            u->cand_store = nullptr;
+            u->cinfo = nullptr;
+            u->cell_have = 0;
+            if ((rc = dalloc(&u->cand_store, (size_t)cells * LDS_IDX_CAP)) || (rc = dalloc(&u->cinfo, cells)))
+                return rc;
+            // a cell never built has no candidates
+            HIPCHK(hipMemsetAsync(u->cinfo, 0, sizeof(uint4) * cells, u->stream));
+            u->cell_have = cells;
+        }
+        const size_t nrec = (size_t)cells * u->bcap;
+        if (u->hrec_have < nrec) {
+            (void)hipFree(u->hrec);
+            u->hrec = nullptr;
+            u->hrec_have = 0;
+            if ((rc = dalloc(&u->hrec, nrec))) return rc;
+            u->hrec_have = nrec;
+        }
+        const size_t need = (size_t)cells * cell_region(u);
+        if (u->nbr16_have < need) {
+            (void)hipFree(u->nbr16);
+            u->nbr16 = nullptr;
+            u->nbr16_have = 0;
+            if ((rc = dalloc(&u->nbr16, need))) return rc;
+            u->nbr16_have = need;
+        }
+    } else if (u->nbr_have < cap) {
         (void)hipFree(u->nbr);
         u->nbr = nullptr;
-        u->nbr_cap = 0;
+        u->nbr_have = 0;
         // groups of 4 entries per lane: (cap + 4 T) entries per atom cover any
         // lanes-per-atom setting up to 16
         if ((rc = dalloc(&u->nbr, ((size_t)cap + (4u << MAX_LOGT)) * n))) return rc;
-        u->nbr_cap = cap;
+        u->nbr_have = cap;
     }
     return GS_OK;
 }
@@ -1810,31 +2418,76 @@ int launch_pair_list(gs_universe* u, gs_observables* obs, hipEvent_t e0 = nullpt
     const unsigned nb = nblocks((int64_t)N << u->logT);
     const ThermoArgs th{p.enable_thermostat != 0 ? 1 : 0, p.thermostat_temperature, std::fmax(p.thermostat_tau, 1.0),
                         p.thermal_constant};
-#define GS_PL(L, C, S, SG)                                                                                    \
-    launch_timed(k_pair_list<L, C, S, SG>, dim3(nb), dim3(BLOCK), u->stream, e0, e1, P, N,                       \
+#define GS_PL(L, C, S, SG, G)                                                                                 \
+    launch_timed(k_pair_list<L, C, S, SG, G>, dim3(nb), dim3(BLOCK), u->stream, e0, e1, P, N,                       \
                  (const double4*)u->pos4[c], (const uint8_t*)u->type[c], (const uint32_t*)u->nbr,               \
                  (const uint32_t*)u->nlen, (const PairCoef*)u->coef, u->a[0], u->a[1], u->a[2], u->v[c][0],      \
                  u->v[c][1], u->v[c][2], u->vpart, u->kpart, u->flags, u->scal, obs, th,                          \
                  (const double*)u->ab_live, (const double*)u->vb_live, u->nvb_live)
-#define GS_PL_T(C, S, SG)                   \
-    switch (u->logT) {                      \
-        case 0: GS_PL(0, C, S, SG); break;  \
-        case 1: GS_PL(1, C, S, SG); break;  \
-        case 2: GS_PL(2, C, S, SG); break;  \
-        case 3: GS_PL(3, C, S, SG); break;  \
-        default: GS_PL(4, C, S, SG); break; \
+#define GS_PL_T(C, S, SG, G)                   \
+    switch (u->logT) {                         \
+        case 0: GS_PL(0, C, S, SG, G); break;  \
+        case 1: GS_PL(1, C, S, SG, G); break;  \
+        case 2: GS_PL(2, C, S, SG, G); break;  \
+        case 3: GS_PL(3, C, S, SG, G); break;  \
+        default: GS_PL(4, C, S, SG, G); break; \
     }
+    const bool grav = u->prm.enable_gravity != 0;
     if (u->single) {
-        GS_PL_T(false, false, true);
+        GS_PL_T(false, false, true, false);
     } else if (!u->coul) {
-        GS_PL_T(false, false, false);
+        if (grav) {
+            GS_PL_T(false, false, false, true);
+        } else {
+            GS_PL_T(false, false, false, false);
+        }
     } else if (u->shared_switch) {
-        GS_PL_T(true, true, false);
+        if (grav) {
+            GS_PL_T(true, true, false, true);
+        } else {
+            GS_PL_T(true, true, false, false);
+        }
     } else {
-        GS_PL_T(true, false, false);
+        if (grav) {
+            GS_PL_T(true, false, false, true);
+        } else {
+            GS_PL_T(true, false, false, false);
+        }
     }
 #undef GS_PL_T
 #undef GS_PL
+    HIPCHK(hipGetLastError());
+    return GS_OK;
+}
+
+// Cell-staged pair pass of one tick (pair_mode 2), fused like launch_pair_list.
+int launch_pair_cell(gs_universe* u, gs_observables* obs, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    const KParams& P = u->kp;
+    const gs_params& p = u->prm;
+    const int c = u->cur;
+    const uint32_t ncell = u->kpl.ncell, per = (ncell + 7u) / 8u;
+    const unsigned nb = 8u * per;
+    if (nb > u->npart) return fail(GS_E_STATE, "cell-staged pass: more workgroups than partial slots");
+    const size_t region = cell_region(u);
+    const ThermoArgs th{p.enable_thermostat != 0 ? 1 : 0, p.thermostat_temperature, std::fmax(p.thermostat_tau, 1.0),
+                        p.thermal_constant};
+#define GS_PC(C, S, SG, G)                                                                                     \
+    launch_timed(k_pair_cell<C, S, SG, G>, dim3(nb), dim3(CELL_THREADS), u->stream, e0, e1, P, ncell, per,          \
+                 (const double4*)u->pos4[c], (const uint16_t*)u->nbr16, region, u->nbr_cap, (const uint2*)u->hrec, \
+                 u->bcap,                                                                                           \
+                 (const uint32_t*)u->cand_store, (const uint4*)u->cinfo, (const PairCoef*)u->coef, u->a[0], u->a[1], u->a[2], u->v[c][0], u->v[c][1], u->v[c][2],         \
+                 u->vpart, u->kpart, u->flags, u->scal, obs, th, (const double*)u->ab_live,                       \
+                 (const double*)u->vb_live, u->nvb_live, (uint32_t)u->n)
+    const bool grav = u->prm.enable_gravity != 0;
+    if (u->single)
+        GS_PC(false, false, true, false);
+    else if (!u->coul)
+        grav ? GS_PC(false, false, false, true) : GS_PC(false, false, false, false);
+    else if (u->shared_switch)
+        grav ? GS_PC(true, true, false, true) : GS_PC(true, true, false, false);
+    else
+        grav ? GS_PC(true, false, false, true) : GS_PC(true, false, false, false);
+#undef GS_PC
     HIPCHK(hipGetLastError());
     return GS_OK;
 }
@@ -1867,9 +2520,14 @@ int list_tick_start(gs_universe* u) {
         k_list_bin<<<nblocks(u->n), BLOCK, 0, u->stream>>>(PL, u->pos4[c], u->bcount, u->bucket, u->bcap, u->flags);
     }
     const int c = u->cur;
-    k_list_build<<<PL.ncell, PAIR_THREADS, 0, u->stream>>>(PL, u->bcount, u->bucket, u->bcap, u->pos4[c],
-                                                           u->type[c], u->nbr, u->nlen, u->xref, N,
-                                                           (uint32_t)u->logT, u->nbr_cap, rl * rl, u->flags);
+    if (u->pair_mode == 2)
+        k_list_build<true><<<PL.ncell, PAIR_THREADS, 0, u->stream>>>(
+            PL, u->bcount, u->bucket, u->bcap, u->pos4[c], u->type[c], nullptr, u->nlen, u->xref, N, 0u, u->nbr_cap,
+            rl * rl, u->flags, u->cand_store, u->cinfo, u->hrec, u->nbr16, cell_region(u));
+    else
+        k_list_build<false><<<PL.ncell, PAIR_THREADS, 0, u->stream>>>(
+            PL, u->bcount, u->bucket, u->bcap, u->pos4[c], u->type[c], u->nbr, u->nlen, u->xref, N,
+            (uint32_t)u->logT, u->nbr_cap, rl * rl, u->flags, nullptr, nullptr, nullptr, nullptr, 0);
     u->list_age += 1;
     HIPCHK(hipGetLastError());
     return GS_OK;
@@ -1887,8 +2545,9 @@ int run_ticks(gs_universe* u, int64_t n_steps, bool observe, bool list) {
         if (list) {
             if ((rc = list_tick_start(u))) return rc;
             if ((rc = bond_force_pass(u, false))) return rc;
-            if ((rc = launch_pair_list(u, observe ? u->obs + s : nullptr, tmode ? u->ev[4 * s + 1] : nullptr,
-                                       tmode ? u->ev[4 * s + 2] : nullptr)))
+            hipEvent_t e0 = tmode ? u->ev[4 * s + 1] : nullptr, e1 = tmode ? u->ev[4 * s + 2] : nullptr;
+            gs_observables* o = observe ? u->obs + s : nullptr;
+            if ((rc = u->pair_mode == 2 ? launch_pair_cell(u, o, e0, e1) : launch_pair_list(u, o, e0, e1)))
                 return rc;
             if (tmode > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
         } else {
@@ -1966,8 +2625,8 @@ int gs_create(const gs_params* params, int64_t n_atoms, gs_universe** out) {
         if ((rc = dalloc(&u->a[a], n))) return bail(rc);
     if ((rc = dalloc(&u->key, n)) || (rc = dalloc(&u->ticket, n)) || (rc = dalloc(&u->perm, n)) ||
         (rc = dalloc(&u->count, (size_t)P.ncell)) || (rc = dalloc(&u->cstart, (size_t)P.ncell + 1)) ||
-        (rc = dalloc(&u->coef, coef.size())) || (rc = dalloc(&u->vpart, (size_t)std::max<uint32_t>(u->nvpart, nblocks((int64_t)n << MAX_LOGT)))) ||
-        (rc = dalloc(&u->kpart, (size_t)nblocks((int64_t)n << MAX_LOGT))) || (rc = dalloc(&u->counter, 4)) ||
+        (rc = dalloc(&u->coef, coef.size())) || (rc = dalloc(&u->vpart, (size_t)u->npart)) ||
+        (rc = dalloc(&u->kpart, (size_t)u->npart)) || (rc = dalloc(&u->counter, 4)) ||
         (rc = dalloc(&u->scal, 4)) || (rc = dalloc(&u->pe, n)) || (rc = dalloc(&u->sabs, n)) ||
         (rc = dalloc(&u->stage, 9 * n)) || (rc = dalloc(&u->stage_u, n)))
         return bail(rc);
@@ -2023,6 +2682,10 @@ int gs_destroy(gs_universe* u) {
     (void)hipFree(u->l_sigma);
     for (hipEvent_t e : u->ev) (void)hipEventDestroy(e);
     (void)hipFree(u->nbr);
+    (void)hipFree(u->nbr16);
+    (void)hipFree(u->cand_store);
+    (void)hipFree(u->cinfo);
+    (void)hipFree(u->hrec);
     (void)hipFree(u->nlen);
     (void)hipFree(u->xref);
     (void)hipFree(u->flags);
@@ -2261,7 +2924,7 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
         if ((rc = dalloc(&u->obs, (size_t)n_steps))) return rc;
         u->obs_cap = n_steps;
     }
-    bool list = u->pair_mode == 1 && u->list_ok && !u->small_box;
+    bool list = (u->pair_mode == 1 || u->pair_mode == 2) && u->list_ok && !u->small_box;
     const size_t nev = u->timing ? (size_t)(4 * n_steps) : 0;
     while (u->ev.size() < nev) {
         hipEvent_t e;
@@ -2330,8 +2993,10 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
 
 int gs_get_tuning(gs_universe* u, gs_tuning* out) {
     if (!u || !out) return fail(GS_E_INVALID, "NULL argument");
-    out->pair_mode = (u->pair_mode == 1 && u->list_ok && !u->small_box) ? 1 : 0;
-    out->lanes_per_atom = 1 << u->logT;
+    const bool list = (u->pair_mode == 1 || u->pair_mode == 2) && u->list_ok && !u->small_box;
+    out->pair_mode = list ? u->pair_mode : 0;
+    // the cell-staged pass picks the lanes per atom cell by cell
+    out->lanes_per_atom = out->pair_mode == 2 ? 0 : 1 << u->logT;
     out->skin = u->skin;
     out->sort_every = u->sort_every;
     out->list_builds = u->list_builds;
@@ -2341,10 +3006,10 @@ int gs_get_tuning(gs_universe* u, gs_tuning* out) {
 
 int gs_set_tuning(gs_universe* u, const gs_tuning* t) {
     if (!u || !t) return fail(GS_E_INVALID, "NULL argument");
-    if (t->pair_mode != 0 && t->pair_mode != 1) return fail(GS_E_INVALID, "pair_mode must be 0 or 1");
-    if (t->pair_mode == 1) {
-        int logT;
-        switch (t->lanes_per_atom) {
+    if (t->pair_mode < 0 || t->pair_mode > 2) return fail(GS_E_INVALID, "pair_mode must be 0, 1 or 2");
+    if (t->pair_mode != 0) {
+        int logT = u->logT;
+        if (t->pair_mode == 1) switch (t->lanes_per_atom) {
             case 1: logT = 0; break;
             case 2: logT = 1; break;
             case 4: logT = 2; break;
@@ -2380,7 +3045,8 @@ int gs_autotune(gs_universe* u, int32_t trial_steps, gs_tuning* best) {
     cand.push_back(gs_tuning{0, 1, 0.0, 0, 0, 0.0});
     if (!u->small_box)
         for (double f : {0.04, 0.08, 0.12, 0.18})
-            for (int T : {2, 4, 8, 16}) cand.push_back(gs_tuning{1, T, f * u->kp.rc, 0, 0, 0.0});
+            for (int T : {0, 2, 4, 8, 16})
+                cand.push_back(gs_tuning{T ? 1 : 2, T, f * u->kp.rc, 0, 0, 0.0});
     const int was_timing = u->timing;
     u->timing = 0;
     gs_tuning win{0, 1, 0.0, 0, 0, 1e300};

@@ -83,6 +83,19 @@ struct PairCoef {
     double bond_req;   // equilibrium_distance(sigma_ij)
 };
 
+// Per-species factors of the Lorentz-Berthelot mixing rules, stored after
+// the pair table in the same buffer: sigma_ij = (sigma_i + sigma_j) / 2
+// (the same expression as PairCoef::sigma) and 4 eps_ij = a4_i a4_j with
+// a4 = sqrt(4 kT eps_k).  A pair kernel reading these from LDS touches ten
+// 16-byte entries instead of up to a hundred 64-byte rows, whose reads by
+// the lanes of a wave fall into the same banks.
+struct SpCoef {
+    double sigma;
+    double a4;
+};
+constexpr int COEF_SP_OFFSET = GS_NSPECIES * GS_NSPECIES;  // in PairCoef units
+constexpr int COEF_LEN = COEF_SP_OFFSET + (int)((GS_NSPECIES * sizeof(SpCoef) + 63) / 64);
+
 // Per-species-pair parameters of the differentiable model (adjoint).
 struct AdjCoef {
     double sigma;            // sigma_ij

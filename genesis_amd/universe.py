@@ -134,7 +134,8 @@ class GpuUniverse:
     def set_tuning(self, pair_mode: int, lanes_per_atom: int = 4, skin: Optional[float] = None,
                    sort_every: int = 0) -> None:
         """pair_mode 0 = per-tick cell tiles, 1 = Verlet list (lanes_per_atom, skin, and
-        sort_every = ticks between full re-sorts of the resident state; 0 keeps it)."""
+        sort_every = ticks between full re-sorts of the resident state; 0 keeps it),
+        2 = the same lists evaluated cell by cell from LDS (lanes_per_atom unused)."""
         cur = self.tuning()
         t = _lib.Tuning(pair_mode, lanes_per_atom, cur["skin"] if skin is None else skin, sort_every, 0, 0.0)
         _lib.check(self.lib.gs_set_tuning(self.h, ctypes.byref(t)))

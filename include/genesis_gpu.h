@@ -114,11 +114,14 @@ typedef struct gs_timing {
  * within skin/2 of its position at the build.  The device tests the
  * displacements every tick and rebuilds the lists itself when needed; the
  * host only schedules a full re-sort of the resident state every
- * sort_every ticks (memory locality).  Both modes give the reference's
- * pair set. */
+ * sort_every ticks (memory locality).  pair_mode 2 keeps the same lists
+ * per list-grid cell: one workgroup per cell stages the positions of the
+ * cell's stencil candidates in LDS and evaluates its atoms' lists from
+ * there (16-bit entries, lanes per atom chosen cell by cell; reported as
+ * lanes_per_atom 0).  Every mode gives the reference's pair set. */
 typedef struct gs_tuning {
-    int32_t pair_mode;       /* 0 = per-tick cell tiles, 1 = Verlet list             */
-    int32_t lanes_per_atom;  /* list mode: 1, 2, 4, 8 or 16 lanes share a list       */
+    int32_t pair_mode;       /* 0 = cell tiles, 1 = Verlet list, 2 = cell-staged list */
+    int32_t lanes_per_atom;  /* mode 1: 1, 2, 4, 8 or 16 lanes share a list           */
     double skin;             /* list mode: list cutoff = rc + skin                   */
     int64_t sort_every;      /* list mode: ticks between full re-sorts (0 = keep)    */
     int64_t list_builds;     /* list builds so far (read only)                       */

@@ -149,7 +149,7 @@ def test_thermostat_and_kinetic(GU):
     np.testing.assert_allclose(got.vx, ref.vx, rtol=1e-14, atol=0)
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_one_tick_matches_oracle(GU, mode):
     # Tick 1 starts with a = 0 (src/universe/mod.rs:272,357): kick, drift and
     # wrap are then bit-exact, forces and the second kick within tolerance.
@@ -173,7 +173,7 @@ def test_one_tick_matches_oracle(GU, mode):
     assert obs[0, 4] == pytest.approx(obs_ref[0, 4], rel=1e-13)  # lambda
 
 
-@pytest.mark.parametrize("mode,lanes", [(0, 1), (1, 1), (1, 4), (1, 8)])
+@pytest.mark.parametrize("mode,lanes", [(0, 1), (1, 1), (1, 4), (1, 8), (2, 0)])
 def test_short_trajectory_tracks_oracle(GU, mode, lanes):
     # 20 ticks: trajectories stay within round-off growth of the oracle's
     # (chaos amplifies ~1e-16 slowly over few steps).
@@ -205,7 +205,7 @@ def test_nve_energy_and_momentum_conservation(GU):
     assert np.linalg.norm(p) < 1e-10 * (m * np.abs(got.vx)).sum()
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_step_is_bitwise_deterministic(GU, mode):
     w = mixture(n=32768, seed=1002, steps=50)
     ph, st = w.physics, w.state
@@ -292,34 +292,37 @@ def mixture_60_ticks():
     return w, ref, obs_ref
 
 
-@pytest.mark.parametrize("lanes", [1, 2, 4, 8])
-def test_list_mode_matches_oracle_over_a_mixture_trajectory(GU, lanes, mixture_60_ticks):
-    # 60 ticks of the 32k mixture with Coulomb and Berendsen: the list path
+@pytest.mark.parametrize("mode,lanes", [(1, 1), (1, 2), (1, 4), (1, 8), (2, 0)])
+def test_list_mode_matches_oracle_over_a_mixture_trajectory(GU, mode, lanes, mixture_60_ticks):
+    # 60 ticks of the 32k mixture with Coulomb and Berendsen: the list paths
     # (lists rebuilt only when an atom moved > skin/2) against the oracle,
-    # which rebuilds its grid every tick.
+    # which rebuilds its grid every tick.  At the default skin a stencil holds
+    # about 1200 candidates, so the cell-staged pass stages many cells in two
+    # chunks.
     w, ref, obs_ref = mixture_60_ticks
     ph, st = w.physics, w.state
-    say(f"list mode lanes {lanes}: 60 ticks n=32768 vs oracle")
+    say(f"list mode {mode} lanes {lanes}: 60 ticks n=32768 vs oracle")
     with GU(ph, st) as u:
-        u.set_tuning(1, lanes)
+        u.set_tuning(mode, lanes, skin=0.12 * ph.force_rc())
         obs = u.step(60)
         got = u.download()
         tun = u.tuning()
     say(f"list builds {tun['list_builds']}, sort every {tun['sort_every']}")
-    assert tun["pair_mode"] == 1 and tun["list_builds"] >= 1
+    assert tun["pair_mode"] == mode and tun["list_builds"] >= 1
     np.testing.assert_allclose(obs[:, 1], obs_ref[:, 1], rtol=1e-8)
     np.testing.assert_allclose(obs[:, 0], obs_ref[:, 0], rtol=1e-8)
     np.testing.assert_allclose(got.x, ref.x, rtol=0, atol=1e-8)
 
 
-def test_list_mode_forces_after_many_ticks_match_a_fresh_force_pass(GU):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_list_mode_forces_after_many_ticks_match_a_fresh_force_pass(GU, mode):
     # After 300 list-mode ticks the accelerations in the state must equal a
     # fresh per-tick-grid force pass (oracle) on the same positions.
     w = mixture(n=8000, seed=31, steps=300)
     ph, st = w.physics, w.state
-    say("list mode 300 ticks then oracle force pass on the final positions")
+    say(f"list mode {mode} 300 ticks then oracle force pass on the final positions")
     with GU(ph, st) as u:
-        u.set_tuning(1, 4)
+        u.set_tuning(mode, 4)
         u.step(300)
         got = u.download()
         tun = u.tuning()
@@ -331,16 +334,17 @@ def test_list_mode_forces_after_many_ticks_match_a_fresh_force_pass(GU):
     assert ok, worst
 
 
-def test_list_rebuilds_on_device_when_atoms_pass_half_skin(GU):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_list_rebuilds_on_device_when_atoms_pass_half_skin(GU, mode):
     # A hot gas with a tiny skin: the device must rebuild the lists on many
     # ticks (no host round trip decides it), and the trajectory must match
     # the per-tick cell tiles.  Two calls, and a short re-sort period, cover
     # the lists surviving across calls and the forced re-sorts.
     w = mixture(n=4096, seed=41, temperature=600.0, steps=40, thermostat=False)
     ph, st = w.physics, w.state
-    say("device-driven list rebuilds, hot gas, skin 0.02")
+    say(f"device-driven list rebuilds, hot gas, skin 0.02, mode {mode}")
     with GU(ph, st) as u:
-        u.set_tuning(1, 2, skin=0.02, sort_every=15)
+        u.set_tuning(mode, 2, skin=0.02, sort_every=15)
         obs_l = np.concatenate([u.step(25), u.step(15)])
         got_l = u.download()
         tun = u.tuning()
@@ -349,20 +353,21 @@ def test_list_rebuilds_on_device_when_atoms_pass_half_skin(GU):
         obs_t = u.step(40)
         got_t = u.download()
     say(f"list builds {tun['list_builds']} in 40 ticks")
-    assert tun["pair_mode"] == 1 and tun["list_builds"] >= 10
+    assert tun["pair_mode"] == mode and tun["list_builds"] >= 10
     np.testing.assert_allclose(obs_l[:, 5], obs_t[:, 5], rtol=1e-9)
     np.testing.assert_allclose(got_l.x, got_t.x, rtol=0, atol=1e-9)
 
 
-def test_list_mode_falls_back_to_tiles_when_a_stencil_is_too_dense(GU):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_list_mode_falls_back_to_tiles_when_a_stencil_is_too_dense(GU, mode):
     # A skin so large that a 27-cell stencil holds more candidates than the
     # list build stages in LDS: the call is replayed on the cell tiles and
     # the tuning reports pair_mode 0.
     w = mixture(n=32768, seed=43, steps=10)
     ph, st = w.physics, w.state
-    say("dense stencil fallback")
+    say(f"dense stencil fallback, mode {mode}")
     with GU(ph, st) as u:
-        u.set_tuning(1, 4, skin=0.6 * ph.force_rc())
+        u.set_tuning(mode, 4, skin=0.6 * ph.force_rc())
         obs_l = u.step(10)
         got_l = u.download()
         tun = u.tuning()
@@ -372,6 +377,48 @@ def test_list_mode_falls_back_to_tiles_when_a_stencil_is_too_dense(GU):
         got_t = u.download()
     assert tun["pair_mode"] == 0
     assert np.array_equal(obs_l, obs_t) and np.array_equal(got_l.x, got_t.x)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_crowded_cell_overflows_and_replays(GU, mode):
+    # 400 atoms packed inside one list-grid cell of an otherwise sparse box:
+    # the first ticks overflow the cell buckets and the lists (399 neighbours
+    # each), so the call is replayed with doubled capacities, and the
+    # cell-staged pass meets more home atoms than lanes (two passes over the
+    # home atoms, one lane each).  One tick against the oracle.
+    rng = np.random.default_rng(17)
+    ph0 = plain_lj(n_side=4, steps=1).physics
+    # six cells per axis: the wrap's half-box translation maps cell centres
+    # to cell centres, so the blob stays inside one cell
+    L = 6.0 * 1.12 * ph0.force_rc() * 1.001
+    ph = Physics(**{**ph0.__dict__, "box": (L, L, L)})
+    dims = int(np.floor(L / (1.12 * ph.force_rc())))
+    assert dims == 6
+    edge = L / dims
+    centre = -0.5 * L + 1.5 * edge
+    nb, ns = 400, 200
+    blob = centre + rng.uniform(-0.3 * edge, 0.3 * edge, (nb, 3))
+    far = rng.uniform(-0.5 * L, 0.5 * L, (ns, 3))
+    far = far[np.linalg.norm(far - centre, axis=1) > 2.5 * edge][:150]
+    p = np.concatenate([blob, far])
+    n = len(p)
+    z = np.zeros(n)
+    st = State(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), z, z.copy(), z.copy(), np.zeros(n, dtype=np.uint8), z.copy())
+    say(f"crowded cell: {nb} atoms in one cell of edge {edge:.3f}, n={n}, mode {mode}")
+    ref, obs_ref = O.run(ph, st, 1)
+    with GU(ph, st) as u:
+        u.set_tuning(mode, 4, skin=0.12 * ph.force_rc())
+        obs = u.step(1)
+        got = u.download()
+        tun = u.tuning()
+    assert tun["pair_mode"] == mode
+    for a in ("x", "y", "z"):
+        assert np.array_equal(getattr(got, a), getattr(ref, a)), a
+    ref_f = O.forces(ph, State(ref.x, ref.y, ref.z, ref.vx, ref.vy, ref.vz, st.species, st.charge))
+    m = np.asarray(ph.mass)[st.species]
+    ok, worst = force_tolerance_ok(got.ax, got.ay, got.az, ref_f, m, REL)
+    assert ok, worst
+    assert obs[0, 1] == pytest.approx(obs_ref[0, 1], rel=REL)
 
 
 def test_autotune_restores_state_and_picks_a_valid_setting(GU):
@@ -385,7 +432,7 @@ def test_autotune_restores_state_and_picks_a_valid_setting(GU):
         say(f"autotune picked {best}")
         for a in ("x", "vx"):
             assert np.array_equal(getattr(before, a), getattr(after, a))
-        assert best["pair_mode"] in (0, 1) and best["ms_per_tick"] > 0
+        assert best["pair_mode"] in (0, 1, 2) and best["ms_per_tick"] > 0
         obs = u.step(10)
         assert np.isfinite(obs).all()
 
