@@ -48,6 +48,8 @@ constexpr int BLOCK = 256;
 constexpr int PAIR_WAVES = 4;
 constexpr int PAIR_THREADS = PAIR_WAVES * WAVE;
 constexpr int TILE_CAP = 1024;  // candidates staged per LDS chunk
+constexpr int BUILD_WAVES = 8;  // waves of a list-build workgroup
+constexpr int BUILD_THREADS = BUILD_WAVES * WAVE;
 constexpr int QCAP = 128;       // per-wave compaction ring (u16 tile indices)
 
 inline unsigned nblocks(int64_t n, int b = BLOCK) { return (unsigned)((n + b - 1) / b); }
@@ -989,6 +991,7 @@ __global__ __launch_bounds__(BLOCK) void k_cells_entity(KParams P, const uint32_
 constexpr uint32_t NBR_IDX_MASK = 0x0FFFFFFFu;  // entry = (species << 28) | sorted index
 constexpr int LDS_IDX_CAP = 2048;               // stencil candidates held (and ordered) in LDS
 constexpr int MAX_LOGT = 4;                     // up to 16 lanes per atom
+constexpr uint32_t HOME_CAP = 256;              // home atoms whose list lengths the build keeps in LDS
 
 // Cell-staged list pass (pair_mode 2): entries are 16 bits,
 // (species << CELL_IDX_BITS) | position in the home cell's ordered
@@ -1066,8 +1069,10 @@ __global__ __launch_bounds__(BLOCK) void k_list_integrate(
 }
 
 // Rebuild binning (only when flags[0] is set): list-grid cell of every
-// atom and its slot in the cell's bucket, one atomic per distinct cell in a
-// wave.  A bucket that overflows its capacity raises flags[6].
+// atom and its slot in the cell's bucket.  Each run of equal cells among a
+// wave's lanes takes its slots with one atomic, and all the runs of the
+// wave issue theirs at once (one round trip per wave).  A bucket that
+// overflows its capacity raises flags[6].
 __global__ __launch_bounds__(BLOCK) void k_list_bin(KParams PL, const double4* __restrict__ pos4,
                                                     uint32_t* __restrict__ bcount, uint32_t* __restrict__ bucket,
                                                     uint32_t bcap, uint32_t* __restrict__ flags) {
@@ -1075,29 +1080,30 @@ __global__ __launch_bounds__(BLOCK) void k_list_bin(KParams PL, const double4* _
     const int lane = threadIdx.x & 63;
     const int k = blockIdx.x * BLOCK + threadIdx.x;
     if (k == 0) flags[3] += 1u;
-    const bool valid = k < PL.n;
-    uint32_t kk = 0;
+    const bool valid = k < PL.n;  // the valid lanes are a prefix of the wave
+    uint32_t kk = 0xFFFFFFFFu;
     if (valid) {
         const double4 p = pos4[k];
         kk = cell_key(PL, p.x, p.y, p.z);
     }
-    uint64_t todo = __ballot(valid);
-    while (todo) {
-        const int leader = __builtin_ctzll(todo);
-        const uint32_t lk = __shfl(kk, leader, WAVE);
-        const bool mine = valid && kk == lk;
-        const uint64_t m = __ballot(mine);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&bcount[lk], (uint32_t)__popcll(m));
-        base = __shfl(base, leader, WAVE);
-        if (mine) {
-            const uint32_t slot = base + mask_rank(m);
-            if (slot < bcap)
-                bucket[(size_t)lk * bcap + slot] = (uint32_t)k;
-            else
-                atomicOr(&flags[6], 1u);
-        }
-        todo &= ~m;
+    const uint32_t prev = __shfl_up(kk, 1, WAVE);
+    const bool head = valid && (lane == 0 || prev != kk);
+    const uint64_t heads = __ballot(head);
+    const int nvalid = __popcll(__ballot(valid));
+    const uint64_t upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    const uint64_t hb = heads & upto;
+    const int rs = hb ? 63 - __builtin_clzll(hb) : 0;  // head of this lane's run
+    const uint64_t ha = heads & ~upto;
+    const int re = ha ? __builtin_ctzll(ha) : nvalid;  // one past this lane's run
+    uint32_t base = 0;
+    if (head) base = atomicAdd(&bcount[kk], (uint32_t)(re - rs));
+    base = __shfl(base, rs, WAVE);
+    if (valid) {
+        const uint32_t slot = base + (uint32_t)(lane - rs);
+        if (slot < bcap)
+            bucket[(size_t)kk * bcap + slot] = (uint32_t)k;
+        else
+            atomicOr(&flags[6], 1u);
     }
 }
 
@@ -1134,7 +1140,7 @@ __global__ void k_flag_set(uint32_t* flags, int idx, uint32_t v) { flags[idx] = 
 // (nbr[i * cap + k]); otherwise entries hold the atom index in the
 // lane-grouped layout of nbr_slot.
 template <bool LOCAL>
-__global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
+__global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
     KParams PL, const uint32_t* __restrict__ bcount, const uint32_t* __restrict__ bucket, uint32_t bcap,
     const double4* __restrict__ pos4, const uint8_t* __restrict__ type, uint32_t* __restrict__ nbr,
     uint32_t* __restrict__ nlen, double4* __restrict__ xref, uint32_t N, uint32_t logT, uint32_t cap, double cut2,
@@ -1146,6 +1152,7 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
     __shared__ double4 tile[TILE_CAP];
     __shared__ uint32_t tidx[TILE_CAP];
     __shared__ uint32_t ord[LDS_IDX_CAP];
+    __shared__ uint32_t hcnt[HOME_CAP];  // running list lengths between chunks
     __shared__ uint32_t seg_base[27];
     __shared__ uint32_t seg_pref[28];
     __shared__ double seg_sh[27][3];
@@ -1185,21 +1192,39 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
         if (tid == 0) atomicOr(&flags[2], 1u);
         return;
     }
-    for (uint32_t t = tid; t < ncand; t += PAIR_THREADS) {
-        int sg = 0;
-        while (seg_pref[sg + 1] <= t) ++sg;
+    // stencil segment of candidate t < ncand: the last s with seg_pref[s] <= t
+    auto seg_of = [&](uint32_t t) {
+        int lo = 0, hi = 27;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (seg_pref[mid] <= t)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        return lo;
+    };
+    for (uint32_t t = tid; t < ncand; t += BUILD_THREADS) {
+        const int sg = seg_of(t);
         cand[t] = bucket[seg_base[sg] + (t - seg_pref[sg])];
     }
     __syncthreads();
-    for (uint32_t t = tid; t < ncand; t += PAIR_THREADS) {
-        int sg = 0;
-        while (seg_pref[sg + 1] <= t) ++sg;
+    for (uint32_t t = tid; t < ncand; t += BUILD_THREADS) {
+        const int sg = seg_of(t);
         const uint32_t v = cand[t], lo = seg_pref[sg], hi = seg_pref[sg + 1];
-        uint32_t r = 0;
-        for (uint32_t f = lo; f < hi; ++f) r += cand[f] < v ? 1u : 0u;
+        // rank of v in its segment, eight independent reads per step
+        uint32_t r = 0, f = lo;
+        for (; f + 8 <= hi; f += 8) {
+            uint32_t x[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) x[q] = cand[f + q];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) r += x[q] < v ? 1u : 0u;
+        }
+        for (; f < hi; ++f) r += cand[f] < v ? 1u : 0u;
         ord[lo + r] = v;
     }
-    for (uint32_t i = tid; i < nh; i += PAIR_THREADS) {
+    for (uint32_t i = tid; i < nh; i += BUILD_THREADS) {
         const uint32_t gi = home[i];
         xref[gi] = pos4[gi];
     }
@@ -1209,7 +1234,7 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
     uint16_t* reg = LOCAL ? nbr16 + (size_t)c * region : nullptr;
     if (LOCAL) {
         __syncthreads();
-        for (uint32_t t = tid; t < ncand; t += PAIR_THREADS) cand_store[(size_t)c * LDS_IDX_CAP + t] = ord[t];
+        for (uint32_t t = tid; t < ncand; t += BUILD_THREADS) cand_store[(size_t)c * LDS_IDX_CAP + t] = ord[t];
         if (tid == 0) cinfo[c] = make_uint4(ncand, hs, nh, 0u);
     }
     const int nchunks = (int)((ncand + TILE_CAP - 1) / TILE_CAP);
@@ -1217,10 +1242,9 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
         const uint32_t c0 = (uint32_t)ch * TILE_CAP;
         const uint32_t len = min((uint32_t)TILE_CAP, ncand - c0);
         __syncthreads();
-        for (uint32_t t = tid; t < len; t += PAIR_THREADS) {
+        for (uint32_t t = tid; t < len; t += BUILD_THREADS) {
             const uint32_t idx = c0 + t;
-            int sg = 0;
-            while (seg_pref[sg + 1] <= idx) ++sg;
+            const int sg = seg_of(idx);
             const uint32_t j = ord[idx];
             double4 pj = pos4[j];
             pj.x += seg_sh[sg][0];
@@ -1230,40 +1254,59 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_list_build(
             tidx[t] = LOCAL ? ((uint32_t)type[j] << CELL_IDX_BITS) | idx : ((uint32_t)type[j] << 28) | j;
         }
         __syncthreads();
-        for (uint32_t i = w; i < nh; i += PAIR_WAVES) {
-            const uint32_t gi = LOCAL ? ord[hs + i] : home[i];
-            const double4 pi = pos4[gi];
-            uint32_t cnt = ch == 0 ? 0u : nlen[gi];
-            for (uint32_t base = 0; base < len; base += 64) {
-                const uint32_t j = base + lane;
-                bool in = false;
-                uint32_t e = 0;
-                if (j < len) {
-                    const double4 pj = tile[j];
-                    e = tidx[j];
+        // the wave's next home atom is fetched while this one is swept, and
+        // a list's running length stays in LDS between chunks (in nlen when
+        // the cell has more than HOME_CAP home atoms)
+        auto home_atom = [&](uint32_t ii) { return LOCAL ? ord[hs + ii] : home[ii]; };
+        uint32_t gi_n = (uint32_t)w < nh ? home_atom((uint32_t)w) : 0u;
+        double4 pi_n = (uint32_t)w < nh ? pos4[gi_n] : make_double4(0.0, 0.0, 0.0, 0.0);
+        for (uint32_t i = w; i < nh; i += BUILD_WAVES) {
+            const uint32_t gi = gi_n;
+            const double4 pi = pi_n;
+            if (i + BUILD_WAVES < nh) {
+                gi_n = home_atom(i + BUILD_WAVES);
+                pi_n = pos4[gi_n];
+            }
+            uint32_t cnt = ch == 0 ? 0u : (nh <= HOME_CAP ? hcnt[i] : nlen[gi]);
+            // 256 candidates per step, four per lane (independent distance
+            // tests); the ballots are taken in candidate order, so the list
+            // is ordered by candidate position as with one per lane
+            for (uint32_t base = 0; base < len; base += 4 * WAVE) {
+                bool in[4];
+                uint32_t e[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t j = base + (uint32_t)(q * WAVE + lane);
+                    const uint32_t jc = j < len ? j : len - 1u;
+                    const double4 pj = tile[jc];
+                    e[q] = tidx[jc];
                     const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                    const uint32_t jg = LOCAL ? ord[c0 + j] : (e & NBR_IDX_MASK);
-                    in = (r2 < cut2) && (jg != gi);
+                    const uint32_t jg = LOCAL ? ord[c0 + jc] : (e[q] & NBR_IDX_MASK);
+                    in[q] = j < len && r2 < cut2 && jg != gi;
                 }
-                const uint64_t m = __ballot(in);
-                if (in) {
-                    const uint32_t k = cnt + mask_rank(m);
-                    if (k < cap) {
-                        if (LOCAL)
-                            reg[(size_t)i * cap + k] = (uint16_t)e;
-                        else
-                            nbr[nbr_slot(k, gi, N, logT)] = e;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint64_t m = __ballot(in[q]);
+                    if (in[q]) {
+                        const uint32_t k = cnt + mask_rank(m);
+                        if (k < cap) {
+                            if (LOCAL)
+                                reg[(size_t)i * cap + k] = (uint16_t)e[q];
+                            else
+                                nbr[nbr_slot(k, gi, N, logT)] = e[q];
+                        }
                     }
+                    cnt += (uint32_t)__popcll(m);
                 }
-                cnt += (uint32_t)__popcll(m);
             }
             if (lane == 0) {
                 if (ch == nchunks - 1 && cnt > cap) {
                     atomicOr(&flags[1], 1u);
                     cnt = cap;
                 }
-                nlen[gi] = cnt;
+                if (nh <= HOME_CAP) hcnt[i] = cnt;
+                if (ch == nchunks - 1 || nh > HOME_CAP) nlen[gi] = cnt;
                 // LOCAL: home record ((species << 28) | atom, list length)
                 if (LOCAL && ch == nchunks - 1) hrec[(size_t)c * bcap + i] = make_uint2(((uint32_t)type[gi] << 28) | gi, cnt);
             }
@@ -1292,7 +1335,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
                                                      double* __restrict__ kpart, uint32_t* __restrict__ flags,
                                                      double* __restrict__ lam_out, gs_observables* __restrict__ obs,
                                                      ThermoArgs th, const double* __restrict__ abond,
-                                                     const double* __restrict__ vbpart, uint32_t nvb) {
+                                                     const double* __restrict__ vbpart, uint32_t nvb, uint32_t rpc) {
     constexpr int T = 1 << LOGT;
     __shared__ PairCoef sc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];
     __shared__ SpCoef ssp[GS_NSPECIES];
@@ -1312,7 +1355,17 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
         if (tid < GS_NSPECIES) ssp[tid] = reinterpret_cast<const SpCoef*>(coef + COEF_SP_OFFSET)[tid];
     }
     __syncthreads();
-    const uint32_t g = blockIdx.x * BLOCK + tid;
+    // Atom chunk of this workgroup.  Workgroup b runs on L2 slice b mod 8
+    // and, when the grid holds rpc workgroups per CU of a 256-CU device,
+    // shares its CU with b + 256, b + 512, ...: those get adjacent chunks
+    // (their neighbour positions share the CU's L1) inside a contiguous
+    // slab of chunks per L2 slice.  rpc = 0 keeps chunk = b.
+    uint32_t chunk = blockIdx.x;
+    if (rpc) {
+        const uint32_t k = blockIdx.x >> 3;
+        chunk = (blockIdx.x & 7u) * (gridDim.x >> 3) + rpc * (k & 31u) + (k >> 5);
+    }
+    const uint32_t g = chunk * BLOCK + tid;
     const uint32_t i = g >> LOGT;
     const uint32_t s = g & (T - 1);
     double fx = 0.0, fy = 0.0, fz = 0.0, vl = 0.0;
@@ -2036,6 +2089,7 @@ struct gs_universe {
     bool list_ok = false;       // the list geometry admits >= 3 cells per axis
     bool list_valid = false;    // lists match the current sorted order
     int pair_mode = 1;          // 0 = per-tick cell tiles, 1 = Verlet list, 2 = cell-staged list
+    int n_cu = 0;               // compute units of the device
     int logT = 2;               // lanes per atom = 1 << logT
     double skin = 0.0;
     int64_t sort_every = 100;   // ticks between full re-sorts of the state in list mode
@@ -2416,6 +2470,8 @@ int launch_pair_list(gs_universe* u, gs_observables* obs, hipEvent_t e0 = nullpt
     const int c = u->cur;
     const uint32_t N = (uint32_t)u->n;
     const unsigned nb = nblocks((int64_t)N << u->logT);
+    // CU-aware chunk order for a grid of whole workgroups per CU
+    const uint32_t rpc = (u->n_cu == 256 && nb % 256u == 0u) ? nb / 256u : 0u;
     const ThermoArgs th{p.enable_thermostat != 0 ? 1 : 0, p.thermostat_temperature, std::fmax(p.thermostat_tau, 1.0),
                         p.thermal_constant};
 #define GS_PL(L, C, S, SG, G)                                                                                 \
@@ -2423,7 +2479,7 @@ int launch_pair_list(gs_universe* u, gs_observables* obs, hipEvent_t e0 = nullpt
                  (const double4*)u->pos4[c], (const uint8_t*)u->type[c], (const uint32_t*)u->nbr,               \
                  (const uint32_t*)u->nlen, (const PairCoef*)u->coef, u->a[0], u->a[1], u->a[2], u->v[c][0],      \
                  u->v[c][1], u->v[c][2], u->vpart, u->kpart, u->flags, u->scal, obs, th,                          \
-                 (const double*)u->ab_live, (const double*)u->vb_live, u->nvb_live)
+                 (const double*)u->ab_live, (const double*)u->vb_live, u->nvb_live, rpc)
 #define GS_PL_T(C, S, SG, G)                   \
     switch (u->logT) {                         \
         case 0: GS_PL(0, C, S, SG, G); break;  \
@@ -2521,11 +2577,11 @@ int list_tick_start(gs_universe* u) {
     }
     const int c = u->cur;
     if (u->pair_mode == 2)
-        k_list_build<true><<<PL.ncell, PAIR_THREADS, 0, u->stream>>>(
+        k_list_build<true><<<PL.ncell, BUILD_THREADS, 0, u->stream>>>(
             PL, u->bcount, u->bucket, u->bcap, u->pos4[c], u->type[c], nullptr, u->nlen, u->xref, N, 0u, u->nbr_cap,
             rl * rl, u->flags, u->cand_store, u->cinfo, u->hrec, u->nbr16, cell_region(u));
     else
-        k_list_build<false><<<PL.ncell, PAIR_THREADS, 0, u->stream>>>(
+        k_list_build<false><<<PL.ncell, BUILD_THREADS, 0, u->stream>>>(
             PL, u->bcount, u->bucket, u->bcap, u->pos4[c], u->type[c], u->nbr, u->nlen, u->xref, N,
             (uint32_t)u->logT, u->nbr_cap, rl * rl, u->flags, nullptr, nullptr, nullptr, nullptr, 0);
     u->list_age += 1;
@@ -2606,6 +2662,10 @@ int gs_create(const gs_params* params, int64_t n_atoms, gs_universe** out) {
     u->list_ok = !u->small_box && list_geometry(u, 0.06 * u->kp.rc);
     if (!u->list_ok) u->pair_mode = 0;
     (void)hipGetDevice(&u->dev);
+    {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, u->dev) == hipSuccess) u->n_cu = ncu;
+    }
     const size_t n = (size_t)std::max(1, u->n);
     const KParams& P = u->kp;
     auto bail = [&](int code) {
