@@ -100,7 +100,7 @@ def main() -> int:
             u.radial_distribution(gr_rmax, gr_bins)
     if args.warmup:
         run(args.warmup)
-    builds0 = u.tuning()["list_builds"]
+    builds0, replays0 = u.tuning()["list_builds"], u.tuning()["list_replays"]
     # timed region: on every 10th tick the pair kernel is launched with start
     # and stop HIP events bound to its dispatch (its average duration feeds
     # the roofline); sampling keeps the events' cost out of the rate
@@ -110,6 +110,7 @@ def main() -> int:
     t1 = time.perf_counter()
     tm = u.timing()
     builds = u.tuning()["list_builds"] - builds0
+    replays = u.tuning()["list_replays"] - replays0
     elapsed = t1 - t0
     value = n * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
@@ -121,7 +122,7 @@ def main() -> int:
     phases = {k: ph_tm[k] / max(ph_tm["steps"], 1) for k in ("rebuild_ms", "pair_ms", "kick_ms")}
     log(f"{value:.4e} atom-steps/s, {ms_per_step:.4f} ms/step; pair kernel {tm['pair_ms'] / tm['steps']:.4f} ms; "
         f"phases per step: rebuild {phases['rebuild_ms']:.4f} ms, pair {phases['pair_ms']:.4f} ms, "
-        f"kick {phases['kick_ms']:.4f} ms; list builds {builds}")
+        f"kick {phases['kick_ms']:.4f} ms; list builds {builds}, replays {replays}")
     kernel = {1: f"k_pair_list<{tuning['lanes_per_atom']} lanes> (pair forces + second kick + Berendsen)",
               2: "k_pair_cell (cell-staged lists; pair forces + second kick + Berendsen)"}.get(
                   tuning["pair_mode"], "k_pair")
@@ -168,6 +169,7 @@ def main() -> int:
         "phases_ms_per_step": phases,
         "tuning": {k: tuning[k] for k in ("pair_mode", "lanes_per_atom", "skin", "sort_every", "ms_per_tick")},
         "list_builds_in_timed_region": builds,
+        "list_replays_in_timed_region": replays,
     }
     u.close()
     if args.pair_mode >= 0 and not args.no_extras:

@@ -51,7 +51,8 @@ class Timing(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("pair_mode", ctypes.c_int32), ("lanes_per_atom", ctypes.c_int32), ("skin", ctypes.c_double),
-                ("sort_every", ctypes.c_int64), ("list_builds", ctypes.c_int64), ("ms_per_tick", ctypes.c_double)]
+                ("sort_every", ctypes.c_int64), ("list_builds", ctypes.c_int64),
+                ("list_replays", ctypes.c_int64), ("ms_per_tick", ctypes.c_double)]
 
 
 class BondStats(ctypes.Structure):

@@ -983,12 +983,26 @@ __global__ __launch_bounds__(BLOCK) void k_cells_entity(KParams P, const uint32_
 // each bucket by atom index in LDS, so the lists do not depend on the order
 // in which the atomics handed out the slots.
 //
-// flags: [0] rebuild this tick, [1] a list overflowed its capacity,
-//        [2] a stencil held more candidates than LDS_IDX_CAP, [3] rebuilds
-//        done by the device, [5] last-workgroup counter of the pair kernel,
-//        [6] a cell bucket overflowed its capacity.
+// flags, reset at every call: [0] rebuild this tick, [1] a list overflowed
+//        its capacity, [2] a stencil held more candidates than LDS_IDX_CAP,
+//        [3] rebuilds done by the device, [4] a tick needed a rebuild that
+//        its binning had not been requested for (predictive binning),
+//        [5] last-workgroup counter of the pair kernel, [6] a cell bucket
+//        overflowed its capacity;
+//        kept across calls: [8] some atom is within reach of skin/2 next
+//        tick, [9] bin (and rebuild) this tick.
+//
+// Predictive binning: the integrate kernel of a tick bins the atoms and
+// forces a rebuild when the previous tick found an atom whose displacement
+// from its build position plus 1.5 times its own displacement of that tick
+// reaches skin/2 (the pair kernel's last workgroup moves [8] to [9]),
+// so no separate binning launch runs on the other ticks.  An atom beyond skin/2 on a tick without
+// binning raises [4] and the host replays the call with a binning launch
+// every tick.
 
 constexpr uint32_t NBR_IDX_MASK = 0x0FFFFFFFu;  // entry = (species << 28) | sorted index
+constexpr int LIST_FLAGS = 16;                  // words of the list flags
+constexpr int LIST_FLAGS_RESET = 8;             // the first ones are reset at every call
 constexpr int LDS_IDX_CAP = 2048;               // stencil candidates held (and ordered) in LDS
 constexpr int MAX_LOGT = 4;                     // up to 16 lanes per atom
 constexpr uint32_t HOME_CAP = 256;              // home atoms whose list lengths the build keeps in LDS
@@ -1021,20 +1035,60 @@ __device__ __forceinline__ size_t nbr_slot(uint32_t k, uint32_t i, uint32_t N, u
     return ((((size_t)(m >> 2) * N + i) << logT) + s) * 4u + (m & 3u);
 }
 
+// Slots of the atoms of a wave in the list-grid cell buckets: each run of
+// equal cells among the wave's lanes takes its slots with one atomic, and
+// all the runs of the wave issue theirs at once (one round trip per wave).
+// kk of a lane past the end is ignored (valid lanes are a prefix of the
+// wave).  A bucket that overflows its capacity raises flags[6].
+__device__ __forceinline__ void bin_wave(uint32_t kk, bool valid, uint32_t k, uint32_t* __restrict__ bcount,
+                                         uint32_t* __restrict__ bucket, uint32_t bcap, uint32_t* __restrict__ flags) {
+    const int lane = threadIdx.x & 63;
+    if (!valid) kk = 0xFFFFFFFFu;
+    const uint32_t prev = __shfl_up(kk, 1, WAVE);
+    const bool head = valid && (lane == 0 || prev != kk);
+    const uint64_t heads = __ballot(head);
+    const int nvalid = __popcll(__ballot(valid));
+    const uint64_t upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    const uint64_t hb = heads & upto;
+    const int rs = hb ? 63 - __builtin_clzll(hb) : 0;  // head of this lane's run
+    const uint64_t ha = heads & ~upto;
+    const int re = ha ? __builtin_ctzll(ha) : nvalid;  // one past this lane's run
+    uint32_t base = 0;
+    if (head) base = atomicAdd(&bcount[kk], (uint32_t)(re - rs));
+    base = __shfl(base, rs, WAVE);
+    if (valid) {
+        const uint32_t slot = base + (uint32_t)(lane - rs);
+        if (slot < bcap)
+            bucket[(size_t)kk * bcap + slot] = k;
+        else
+            atomicOr(&flags[6], 1u);
+    }
+}
+
 // List-mode tick start: thermostat scale, half kick, drift, wrap (the same
 // expressions as k_integrate_key) and the displacement from the build
-// position (flags[0] when beyond skin/2).  Also clears the bucket counts a
-// rebuild of this tick would fill.
+// position, tested against skin/2.  PREBIN (predictive binning, see the
+// flags above): bin the new positions and force a rebuild when flags[9]
+// asks for it, raise flags[4] for an atom beyond skin/2 otherwise, raise
+// flags[8] for an atom within reach of skin/2 next tick, and record the
+// largest one-tick displacement.  Without PREBIN: raise flags[0] for an atom
+// beyond skin/2 (k_list_bin then bins) and clear the bucket counts.
+template <bool PREBIN>
 __global__ __launch_bounds__(BLOCK) void k_list_integrate(
-    KParams P, double4* __restrict__ pos4, double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
-    const double* __restrict__ ax, const double* __restrict__ ay, const double* __restrict__ az,
-    const double* __restrict__ lam_ptr, const double4* __restrict__ xref, double thr2,
-    uint32_t* __restrict__ bcount, uint32_t nbucket, uint32_t* __restrict__ flags) {
+    KParams P, KParams PL, double4* __restrict__ pos4, double* __restrict__ vx, double* __restrict__ vy,
+    double* __restrict__ vz, const double* __restrict__ ax, const double* __restrict__ ay,
+    const double* __restrict__ az, const double* __restrict__ lam_ptr, const double4* __restrict__ xref,
+    double half_skin, uint32_t* __restrict__ bcount, uint32_t* __restrict__ bucket, uint32_t bcap,
+    uint32_t* __restrict__ flags) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int k = blockIdx.x * BLOCK + tid;
-    for (uint32_t c = (uint32_t)k; c < nbucket; c += gridDim.x * BLOCK) bcount[c] = 0u;
-    bool far = false;
-    if (k < P.n) {
+    if (!PREBIN)
+        for (uint32_t c = (uint32_t)k; c < PL.ncell; c += gridDim.x * BLOCK) bcount[c] = 0u;
+    const bool binreq = PREBIN && flags[9] != 0u;
+    const bool valid = k < P.n;
+    bool far = false, warn = false;
+    uint32_t kk = 0;
+    if (valid) {
         double4 p = pos4[k];
         const double lam = *lam_ptr;
         double a = vx[k] * lam, b = vy[k] * lam, c = vz[k] * lam;
@@ -1044,9 +1098,10 @@ __global__ __launch_bounds__(BLOCK) void k_list_integrate(
         vx[k] = a;
         vy[k] = b;
         vz[k] = c;
-        p.x = p.x + a * P.dt;
-        p.y = p.y + b * P.dt;
-        p.z = p.z + c * P.dt;
+        const double sx = a * P.dt, sy = b * P.dt, sz = c * P.dt;
+        p.x = p.x + sx;
+        p.y = p.y + sy;
+        p.z = p.z + sz;
         if (P.flags & F_WRAP) {
             p.x = rem_euclid(p.x, 2.0 * P.half[0]) - P.half[0];
             p.y = rem_euclid(p.y, 2.0 * P.half[1]) - P.half[1];
@@ -1063,48 +1118,42 @@ __global__ __launch_bounds__(BLOCK) void k_list_integrate(
         dy = fma(-P.half[1], rint(dy / P.half[1]), dy);
         dz = fma(-P.half[2], rint(dz / P.half[2]), dz);
         const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        far = !(d2 <= thr2);  // NaN counts as far
+        far = !(d2 <= half_skin * half_skin);  // NaN counts as far
+        if (PREBIN) {
+            // a tick that rebuilds measures the next tick's displacement
+            // from the new build position, this one
+            const double step = sqrt(fma(sz, sz, fma(sy, sy, sx * sx)));
+            warn = !(fma(1.5, step, binreq ? 0.0 : sqrt(d2)) < half_skin);
+            if (binreq) kk = cell_key(PL, p.x, p.y, p.z);
+        }
     }
-    if (__ballot(far) != 0 && lane == 0) atomicOr(&flags[0], 1u);
+    if (!PREBIN) {
+        if (__ballot(far) != 0 && lane == 0) atomicOr(&flags[0], 1u);
+        return;
+    }
+    if (binreq) {
+        bin_wave(kk, valid, (uint32_t)k, bcount, bucket, bcap, flags);
+        if (k == 0) flags[0] = 1u;
+    } else if (__ballot(far) != 0 && lane == 0) {
+        atomicOr(&flags[4], 1u);
+    }
+    if (__ballot(warn) != 0 && lane == 0) atomicOr(&flags[8], 1u);
 }
 
-// Rebuild binning (only when flags[0] is set): list-grid cell of every
-// atom and its slot in the cell's bucket.  Each run of equal cells among a
-// wave's lanes takes its slots with one atomic, and all the runs of the
-// wave issue theirs at once (one round trip per wave).  A bucket that
-// overflows its capacity raises flags[6].
+// Rebuild binning without predictive binning (only when flags[0] is set):
+// the list-grid cell of every atom and its slot in the cell's bucket.
 __global__ __launch_bounds__(BLOCK) void k_list_bin(KParams PL, const double4* __restrict__ pos4,
                                                     uint32_t* __restrict__ bcount, uint32_t* __restrict__ bucket,
                                                     uint32_t bcap, uint32_t* __restrict__ flags) {
     if (flags[0] == 0u) return;
-    const int lane = threadIdx.x & 63;
     const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (k == 0) flags[3] += 1u;
-    const bool valid = k < PL.n;  // the valid lanes are a prefix of the wave
-    uint32_t kk = 0xFFFFFFFFu;
+    const bool valid = k < PL.n;
+    uint32_t kk = 0;
     if (valid) {
         const double4 p = pos4[k];
         kk = cell_key(PL, p.x, p.y, p.z);
     }
-    const uint32_t prev = __shfl_up(kk, 1, WAVE);
-    const bool head = valid && (lane == 0 || prev != kk);
-    const uint64_t heads = __ballot(head);
-    const int nvalid = __popcll(__ballot(valid));
-    const uint64_t upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
-    const uint64_t hb = heads & upto;
-    const int rs = hb ? 63 - __builtin_clzll(hb) : 0;  // head of this lane's run
-    const uint64_t ha = heads & ~upto;
-    const int re = ha ? __builtin_ctzll(ha) : nvalid;  // one past this lane's run
-    uint32_t base = 0;
-    if (head) base = atomicAdd(&bcount[kk], (uint32_t)(re - rs));
-    base = __shfl(base, rs, WAVE);
-    if (valid) {
-        const uint32_t slot = base + (uint32_t)(lane - rs);
-        if (slot < bcap)
-            bucket[(size_t)kk * bcap + slot] = (uint32_t)k;
-        else
-            atomicOr(&flags[6], 1u);
-    }
+    bin_wave(kk, valid, (uint32_t)k, bcount, bucket, bcap, flags);
 }
 
 // Buckets of a freshly cell-sorted state: bucket c holds cstart[c] ..
@@ -1160,6 +1209,7 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
     uint32_t* cand = reinterpret_cast<uint32_t*>(tile);
     const uint32_t c = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (c == 0 && tid == 0) flags[3] += 1u;  // rebuilds done on the device
     const uint32_t nh = min(bcount[c], bcap);
     if (nh == 0) {
         if (LOCAL && tid == 0) cinfo[c] = make_uint4(0u, 0u, 0u, 0u);
@@ -1314,6 +1364,16 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
     }
 }
 
+// End of a list tick (the pair kernel's last workgroup): the rebuild flag
+// and the workgroup counter are cleared, and an atom found within reach of
+// skin/2 asks the next tick to bin.
+__device__ __forceinline__ void close_list_tick(uint32_t* flags) {
+    flags[5] = 0u;
+    flags[0] = 0u;
+    flags[9] = flags[8];
+    flags[8] = 0u;
+}
+
 // Per-atom list traversal fused with the second half of the tick: T lanes
 // per atom split its list (entries s, s+T, ...) and sum with shuffles; the
 // minimum image is taken per pair (dx - L rint(dx / L)); pairs of the skin
@@ -1335,7 +1395,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
                                                      double* __restrict__ kpart, uint32_t* __restrict__ flags,
                                                      double* __restrict__ lam_out, gs_observables* __restrict__ obs,
                                                      ThermoArgs th, const double* __restrict__ abond,
-                                                     const double* __restrict__ vbpart, uint32_t nvb, uint32_t rpc) {
+                                                     const double* __restrict__ vbpart, uint32_t nvb, uint32_t rpc,
+                                                     uint32_t* __restrict__ bcount, uint32_t nbucket) {
     constexpr int T = 1 << LOGT;
     __shared__ PairCoef sc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];
     __shared__ SpCoef ssp[GS_NSPECIES];
@@ -1367,6 +1428,9 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     }
     const uint32_t g = chunk * BLOCK + tid;
     const uint32_t i = g >> LOGT;
+    // a tick that rebuilt leaves the buckets empty for the next binning
+    if (flags[0] != 0u)
+        for (uint32_t b = blockIdx.x * BLOCK + tid; b < nbucket; b += gridDim.x * BLOCK) bcount[b] = 0u;
     const uint32_t s = g & (T - 1);
     double fx = 0.0, fy = 0.0, fz = 0.0, vl = 0.0;
     int ti = 0;
@@ -1515,8 +1579,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     const double Vb = vbpart ? block_sum_fixed(vbpart, nvb, red) : 0.0;
     if (tid == 0) {
         close_tick(K, V, Vb, (int)N, th, lam_out, obs);
-        flags[5] = 0u;
-        flags[0] = 0u;
+        close_list_tick(flags);
     }
 }
 
@@ -1568,7 +1631,8 @@ __global__ __launch_bounds__(CELL_THREADS) __attribute__((amdgpu_waves_per_eu(5)
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az, double* __restrict__ vx,
     double* __restrict__ vy, double* __restrict__ vz, double* __restrict__ vpart, double* __restrict__ kpart,
     uint32_t* __restrict__ flags, double* __restrict__ lam_out, gs_observables* __restrict__ obs, ThermoArgs th,
-    const double* __restrict__ abond, const double* __restrict__ vbpart, uint32_t nvb, uint32_t N) {
+    const double* __restrict__ abond, const double* __restrict__ vbpart, uint32_t nvb, uint32_t N,
+    uint32_t* __restrict__ bcount) {
     constexpr int NW = CELL_THREADS / WAVE;
     constexpr int SK = (STAGE_CAP + CELL_THREADS - 1) / CELL_THREADS;  // staged candidates per thread
     __shared__ PairCoef sc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];
@@ -1595,6 +1659,8 @@ __global__ __launch_bounds__(CELL_THREADS) __attribute__((amdgpu_waves_per_eu(5)
     }
     const uint32_t c = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
     const uint32_t cc = c < ncell ? c : 0u;
+    // a tick that rebuilt leaves the buckets empty for the next binning
+    if (tid == 0 && c < ncell && flags[0] != 0u) bcount[c] = 0u;
     // a build that met a stencil too dense for LDS or a full bucket left some
     // lists unwritten: the host replays this call, so no list is read now
     const uint32_t bad = flags[2] | flags[6];
@@ -1786,8 +1852,7 @@ __global__ __launch_bounds__(CELL_THREADS) __attribute__((amdgpu_waves_per_eu(5)
     const double Vb = vbpart ? block_sum_fixed<CELL_THREADS>(vbpart, nvb, red) : 0.0;
     if (tid == 0) {
         close_tick(K, V, Vb, (int)N, th, lam_out, obs);
-        flags[5] = 0u;
-        flags[0] = 0u;
+        close_list_tick(flags);
     }
 }
 
@@ -2090,6 +2155,7 @@ struct gs_universe {
     bool list_valid = false;    // lists match the current sorted order
     int pair_mode = 1;          // 0 = per-tick cell tiles, 1 = Verlet list, 2 = cell-staged list
     int n_cu = 0;               // compute units of the device
+    bool prebin = true;         // predictive binning in the list integrate kernel
     int logT = 2;               // lanes per atom = 1 << logT
     double skin = 0.0;
     int64_t sort_every = 100;   // ticks between full re-sorts of the state in list mode
@@ -2113,6 +2179,7 @@ struct gs_universe {
     uint32_t* bucket = nullptr;
     uint32_t bcap = 0;
     int64_t list_builds = 0;
+    int64_t list_replays = 0;   // calls replayed from the snapshot
     // snapshot for replays (allocated on first list-mode step)
     double4* snap_pos = nullptr;
     double* snap_v[3] = {nullptr, nullptr, nullptr};
@@ -2379,13 +2446,13 @@ int list_alloc(gs_universe* u, uint32_t cap, uint32_t bcap_min = 0) {
     int rc;
     const size_t n = (size_t)std::max(1, u->n);
     if (!u->nlen) {
-        if ((rc = dalloc(&u->nlen, n)) || (rc = dalloc(&u->xref, n)) || (rc = dalloc(&u->flags, 8)) ||
+        if ((rc = dalloc(&u->nlen, n)) || (rc = dalloc(&u->xref, n)) || (rc = dalloc(&u->flags, LIST_FLAGS)) ||
             (rc = dalloc(&u->snap_pos, n)) || (rc = dalloc(&u->snap_type, n)) ||
             (rc = dalloc(&u->snap_id, n)))
             return rc;
         for (int a = 0; a < 3; ++a)
             if ((rc = dalloc(&u->snap_v[a], n)) || (rc = dalloc(&u->snap_a[a], n))) return rc;
-        HIPCHK(hipMemsetAsync(u->flags, 0, 8 * sizeof(uint32_t), u->stream));
+        HIPCHK(hipMemsetAsync(u->flags, 0, LIST_FLAGS * sizeof(uint32_t), u->stream));
         // an atom whose list was never built has an empty one
         HIPCHK(hipMemsetAsync(u->nlen, 0, sizeof(uint32_t) * n, u->stream));
     }
@@ -2397,7 +2464,11 @@ int list_alloc(gs_universe* u, uint32_t cap, uint32_t bcap_min = 0) {
         (void)hipFree(u->bucket);
         u->bucket = nullptr;
         u->bcap = 0;
-        if (!u->bcount && (rc = dalloc(&u->bcount, (size_t)u->kp.ncell))) return rc;
+        if (!u->bcount) {
+            if ((rc = dalloc(&u->bcount, (size_t)u->kp.ncell))) return rc;
+            // binning fills empty buckets
+            HIPCHK(hipMemsetAsync(u->bcount, 0, sizeof(uint32_t) * u->kp.ncell, u->stream));
+        }
         if ((rc = dalloc(&u->bucket, (size_t)u->kp.ncell * want))) return rc;
         u->bcap = want;
     }
@@ -2479,7 +2550,7 @@ int launch_pair_list(gs_universe* u, gs_observables* obs, hipEvent_t e0 = nullpt
                  (const double4*)u->pos4[c], (const uint8_t*)u->type[c], (const uint32_t*)u->nbr,               \
                  (const uint32_t*)u->nlen, (const PairCoef*)u->coef, u->a[0], u->a[1], u->a[2], u->v[c][0],      \
                  u->v[c][1], u->v[c][2], u->vpart, u->kpart, u->flags, u->scal, obs, th,                          \
-                 (const double*)u->ab_live, (const double*)u->vb_live, u->nvb_live, rpc)
+                 (const double*)u->ab_live, (const double*)u->vb_live, u->nvb_live, rpc, u->bcount, u->kpl.ncell)
 #define GS_PL_T(C, S, SG, G)                   \
     switch (u->logT) {                         \
         case 0: GS_PL(0, C, S, SG, G); break;  \
@@ -2533,7 +2604,7 @@ int launch_pair_cell(gs_universe* u, gs_observables* obs, hipEvent_t e0 = nullpt
                  u->bcap,                                                                                           \
                  (const uint32_t*)u->cand_store, (const uint4*)u->cinfo, (const PairCoef*)u->coef, u->a[0], u->a[1], u->a[2], u->v[c][0], u->v[c][1], u->v[c][2],         \
                  u->vpart, u->kpart, u->flags, u->scal, obs, th, (const double*)u->ab_live,                       \
-                 (const double*)u->vb_live, u->nvb_live, (uint32_t)u->n)
+                 (const double*)u->vb_live, u->nvb_live, (uint32_t)u->n, u->bcount)
     const bool grav = u->prm.enable_gravity != 0;
     if (u->single)
         GS_PC(false, false, true, false);
@@ -2566,14 +2637,20 @@ int list_tick_start(gs_universe* u) {
         k_flag_set<<<1, 1, 0, u->stream>>>(u->flags, 0, 1u);
         u->list_valid = true;
         u->list_age = 0;
-        u->list_builds += 1;
     } else {
         const int c = u->cur;
         const double h = 0.5 * u->skin;
-        k_list_integrate<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->kp, u->pos4[c], u->v[c][0], u->v[c][1],
-                                                                 u->v[c][2], u->a[0], u->a[1], u->a[2], u->scal,
-                                                                 u->xref, h * h, u->bcount, PL.ncell, u->flags);
-        k_list_bin<<<nblocks(u->n), BLOCK, 0, u->stream>>>(PL, u->pos4[c], u->bcount, u->bucket, u->bcap, u->flags);
+        if (u->prebin) {
+            k_list_integrate<true><<<nblocks(u->n), BLOCK, 0, u->stream>>>(
+                u->kp, PL, u->pos4[c], u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->scal,
+                u->xref, h, u->bcount, u->bucket, u->bcap, u->flags);
+        } else {
+            k_list_integrate<false><<<nblocks(u->n), BLOCK, 0, u->stream>>>(
+                u->kp, PL, u->pos4[c], u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->scal,
+                u->xref, h, u->bcount, u->bucket, u->bcap, u->flags);
+            k_list_bin<<<nblocks(u->n), BLOCK, 0, u->stream>>>(PL, u->pos4[c], u->bcount, u->bucket, u->bcap,
+                                                              u->flags);
+        }
     }
     const int c = u->cur;
     if (u->pair_mode == 2)
@@ -2997,7 +3074,7 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
         if ((rc = snapshot(u, false))) return rc;
     }
     for (int attempt = 0;; ++attempt) {
-        if (list) HIPCHK(hipMemsetAsync(u->flags, 0, 8 * sizeof(uint32_t), u->stream));
+        if (list) HIPCHK(hipMemsetAsync(u->flags, 0, LIST_FLAGS_RESET * sizeof(uint32_t), u->stream));
         if ((rc = run_ticks(u, n_steps, obs != nullptr, list))) return rc;
         const int c = u->cur;
         k_apply_lambda<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->n, u->v[c][0], u->v[c][1], u->v[c][2], u->scal);
@@ -3006,15 +3083,19 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
         uint32_t fl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (list) HIPCHK(hipMemcpyAsync(fl, u->flags, sizeof(fl), hipMemcpyDeviceToHost, u->stream));
         HIPCHK(hipStreamSynchronize(u->stream));
-        if (list && (fl[1] || fl[2] || fl[6])) {
-            // A list or a cell bucket overflowed its capacity, or a stencil
-            // held more candidates than the build stages in LDS: replay the
-            // call from the snapshot with larger lists or buckets, or on the
-            // per-tick cell tiles.
+        if (list && (fl[1] || fl[2] || fl[4] || fl[6])) {
+            // A list or a cell bucket overflowed its capacity, a stencil
+            // held more candidates than the build stages in LDS, or a tick
+            // needed a rebuild its predictive binning had not foreseen:
+            // replay the call from the snapshot with larger lists or
+            // buckets, on the per-tick cell tiles, or with a binning launch
+            // every tick.
             if (attempt > 8) return fail(GS_E_OVERFLOW, "neighbour list could not be made valid");
+            u->list_replays += 1;
             u->cur = cur0;
             if ((rc = snapshot(u, true))) return rc;
             u->list_valid = false;
+            if (fl[4]) u->prebin = false;
             if (fl[2]) {
                 u->pair_mode = 0;
                 list = false;
@@ -3026,6 +3107,7 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
             continue;
         }
         if (list) u->list_builds += fl[3];
+        u->prebin = true;  // a replay without predictive binning holds for one call
         break;
     }
     if (obs)
@@ -3060,6 +3142,7 @@ int gs_get_tuning(gs_universe* u, gs_tuning* out) {
     out->skin = u->skin;
     out->sort_every = u->sort_every;
     out->list_builds = u->list_builds;
+    out->list_replays = u->list_replays;
     out->ms_per_tick = 0.0;
     return GS_OK;
 }
@@ -3102,14 +3185,14 @@ int gs_autotune(gs_universe* u, int32_t trial_steps, gs_tuning* best) {
     const std::vector<uint8_t> sp = u->species_h;
     const std::vector<double> q = u->charge_h;
     std::vector<gs_tuning> cand;
-    cand.push_back(gs_tuning{0, 1, 0.0, 0, 0, 0.0});
+    cand.push_back(gs_tuning{0, 1, 0.0, 0, 0, 0, 0.0});
     if (!u->small_box)
         for (double f : {0.04, 0.08, 0.12, 0.18})
             for (int T : {0, 2, 4, 8, 16})
-                cand.push_back(gs_tuning{T ? 1 : 2, T, f * u->kp.rc, 0, 0, 0.0});
+                cand.push_back(gs_tuning{T ? 1 : 2, T, f * u->kp.rc, 0, 0, 0, 0.0});
     const int was_timing = u->timing;
     u->timing = 0;
-    gs_tuning win{0, 1, 0.0, 0, 0, 1e300};
+    gs_tuning win{0, 1, 0.0, 0, 0, 0, 1e300};
     for (gs_tuning c : cand) {
         if (gs_set_tuning(u, &c) != GS_OK) continue;
         if ((rc = gs_step(u, trial_steps / 2, nullptr))) return rc;  // warm up: first lists, list capacity

@@ -137,7 +137,7 @@ class GpuUniverse:
         sort_every = ticks between full re-sorts of the resident state; 0 keeps it),
         2 = the same lists evaluated cell by cell from LDS (lanes_per_atom unused)."""
         cur = self.tuning()
-        t = _lib.Tuning(pair_mode, lanes_per_atom, cur["skin"] if skin is None else skin, sort_every, 0, 0.0)
+        t = _lib.Tuning(pair_mode, lanes_per_atom, cur["skin"] if skin is None else skin, sort_every, 0, 0, 0.0)
         _lib.check(self.lib.gs_set_tuning(self.h, ctypes.byref(t)))
 
     def autotune(self, trial_steps: int = 40) -> dict:

@@ -125,6 +125,7 @@ typedef struct gs_tuning {
     double skin;             /* list mode: list cutoff = rc + skin                   */
     int64_t sort_every;      /* list mode: ticks between full re-sorts (0 = keep)    */
     int64_t list_builds;     /* list builds so far (read only)                       */
+    int64_t list_replays;    /* calls replayed from their snapshot (read only)       */
     double ms_per_tick;      /* gs_autotune: time of the chosen setting              */
 } gs_tuning;
 

@@ -359,6 +359,32 @@ def test_list_rebuilds_on_device_when_atoms_pass_half_skin(GU, mode):
 
 
 @pytest.mark.parametrize("mode", [1, 2])
+def test_list_mode_replays_a_rebuild_its_binning_did_not_foresee(GU, mode):
+    # A cold gas heated a hundredfold in one tick (Berendsen with tau of one
+    # tick): the next ticks' displacements jump far beyond what the previous
+    # tick's predicted, so a tick needs a rebuild whose binning was not
+    # requested.  The call is replayed with a binning launch every tick and
+    # must reproduce the per-tick cell tiles.
+    w = mixture(n=4096, seed=47, temperature=30.0, steps=12)
+    ph = Physics(**{**w.physics.__dict__, "thermostat_temperature": 3000.0, "thermostat_tau": 1.0,
+                    "enable_thermostat": True})
+    st = w.state
+    say(f"unforeseen rebuild replay, mode {mode}")
+    with GU(ph, st) as u:
+        u.set_tuning(mode, 2, skin=0.05, sort_every=100)
+        obs_l = np.concatenate([u.step(4), u.step(8)])
+        got_l = u.download()
+        tun = u.tuning()
+    with GU(ph, st) as u:
+        u.set_tuning(0)
+        obs_t = u.step(12)
+        got_t = u.download()
+    assert tun["pair_mode"] == mode and tun["list_builds"] >= 2
+    np.testing.assert_allclose(obs_l[:, 5], obs_t[:, 5], rtol=1e-9)
+    np.testing.assert_allclose(got_l.x, got_t.x, rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
 def test_list_mode_falls_back_to_tiles_when_a_stencil_is_too_dense(GU, mode):
     # A skin so large that a 27-cell stencil holds more candidates than the
     # list build stages in LDS: the call is replayed on the cell tiles and
