@@ -49,6 +49,7 @@ constexpr int PAIR_WAVES = 4;
 constexpr int PAIR_THREADS = PAIR_WAVES * WAVE;
 constexpr int TILE_CAP = 1024;  // candidates staged per LDS chunk
 constexpr int BUILD_WAVES = 8;  // waves of a list-build workgroup
+constexpr int BUILD_TILE = 1280;  // candidates the list build stages in LDS per chunk
 constexpr int BUILD_THREADS = BUILD_WAVES * WAVE;
 constexpr int QCAP = 128;       // per-wave compaction ring (u16 tile indices)
 
@@ -1198,14 +1199,14 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
     if (flags[0] == 0u) return;
     // the candidates' indices are staged, unordered, in the memory of the
     // position tile, which is filled only after they have been ordered
-    __shared__ double4 tile[TILE_CAP];
-    __shared__ uint32_t tidx[TILE_CAP];
+    __shared__ double4 tile[BUILD_TILE];
+    __shared__ uint8_t ttype[BUILD_TILE];
     __shared__ uint32_t ord[LDS_IDX_CAP];
     __shared__ uint32_t hcnt[HOME_CAP];  // running list lengths between chunks
     __shared__ uint32_t seg_base[27];
     __shared__ uint32_t seg_pref[28];
     __shared__ double seg_sh[27][3];
-    static_assert(LDS_IDX_CAP * sizeof(uint32_t) <= TILE_CAP * sizeof(double4), "index staging fits the tile");
+    static_assert(LDS_IDX_CAP * sizeof(uint32_t) <= BUILD_TILE * sizeof(double4), "index staging fits the tile");
     uint32_t* cand = reinterpret_cast<uint32_t*>(tile);
     const uint32_t c = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -1287,10 +1288,10 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
         for (uint32_t t = tid; t < ncand; t += BUILD_THREADS) cand_store[(size_t)c * LDS_IDX_CAP + t] = ord[t];
         if (tid == 0) cinfo[c] = make_uint4(ncand, hs, nh, 0u);
     }
-    const int nchunks = (int)((ncand + TILE_CAP - 1) / TILE_CAP);
+    const int nchunks = (int)((ncand + BUILD_TILE - 1) / BUILD_TILE);
     for (int ch = 0; ch < nchunks; ++ch) {
-        const uint32_t c0 = (uint32_t)ch * TILE_CAP;
-        const uint32_t len = min((uint32_t)TILE_CAP, ncand - c0);
+        const uint32_t c0 = (uint32_t)ch * BUILD_TILE;
+        const uint32_t len = min((uint32_t)BUILD_TILE, ncand - c0);
         __syncthreads();
         for (uint32_t t = tid; t < len; t += BUILD_THREADS) {
             const uint32_t idx = c0 + t;
@@ -1301,22 +1302,22 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
             pj.y += seg_sh[sg][1];
             pj.z += seg_sh[sg][2];
             tile[t] = pj;
-            tidx[t] = LOCAL ? ((uint32_t)type[j] << CELL_IDX_BITS) | idx : ((uint32_t)type[j] << 28) | j;
+            ttype[t] = type[j];
         }
         __syncthreads();
         // the wave's next home atom is fetched while this one is swept, and
         // a list's running length stays in LDS between chunks (in nlen when
         // the cell has more than HOME_CAP home atoms)
-        auto home_atom = [&](uint32_t ii) { return LOCAL ? ord[hs + ii] : home[ii]; };
-        uint32_t gi_n = (uint32_t)w < nh ? home_atom((uint32_t)w) : 0u;
-        double4 pi_n = (uint32_t)w < nh ? pos4[gi_n] : make_double4(0.0, 0.0, 0.0, 0.0);
+        // (home atom i is the i-th of the centre segment, in index order;
+        // its position comes from the tile when the segment lies in this
+        // chunk, which it does whenever the stencil fits one chunk)
+        const bool home_in_tile = hs >= c0 && hs + nh <= c0 + len;
+        auto home_pos = [&](uint32_t ii) { return home_in_tile ? tile[hs + ii - c0] : pos4[ord[hs + ii]]; };
+        double4 pi_n = (uint32_t)w < nh ? home_pos((uint32_t)w) : make_double4(0.0, 0.0, 0.0, 0.0);
         for (uint32_t i = w; i < nh; i += BUILD_WAVES) {
-            const uint32_t gi = gi_n;
+            const uint32_t gi = ord[hs + i];
             const double4 pi = pi_n;
-            if (i + BUILD_WAVES < nh) {
-                gi_n = home_atom(i + BUILD_WAVES);
-                pi_n = pos4[gi_n];
-            }
+            if (i + BUILD_WAVES < nh) pi_n = home_pos(i + BUILD_WAVES);
             uint32_t cnt = ch == 0 ? 0u : (nh <= HOME_CAP ? hcnt[i] : nlen[gi]);
             // 256 candidates per step, four per lane (independent distance
             // tests); the ballots are taken in candidate order, so the list
@@ -1329,10 +1330,10 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
                     const uint32_t j = base + (uint32_t)(q * WAVE + lane);
                     const uint32_t jc = j < len ? j : len - 1u;
                     const double4 pj = tile[jc];
-                    e[q] = tidx[jc];
+                    const uint32_t jg = ord[c0 + jc];
+                    e[q] = LOCAL ? ((uint32_t)ttype[jc] << CELL_IDX_BITS) | (c0 + jc) : ((uint32_t)ttype[jc] << 28) | jg;
                     const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                    const uint32_t jg = LOCAL ? ord[c0 + jc] : (e[q] & NBR_IDX_MASK);
                     in[q] = j < len && r2 < cut2 && jg != gi;
                 }
 #pragma unroll
