@@ -50,6 +50,10 @@ constexpr int PAIR_THREADS = PAIR_WAVES * WAVE;
 constexpr int TILE_CAP = 1024;  // candidates staged per LDS chunk
 constexpr int BUILD_WAVES = 8;  // waves of a list-build workgroup
 constexpr int BUILD_TILE = 1280;  // candidates the list build stages in LDS per chunk
+#ifndef GS_BUILD_WG_PER_CU
+#define GS_BUILD_WG_PER_CU 1
+#endif
+constexpr uint32_t BUILD_WG_PER_CU = GS_BUILD_WG_PER_CU;  // list-build workgroups per CU
 constexpr int BUILD_THREADS = BUILD_WAVES * WAVE;
 constexpr int QCAP = 128;       // per-wave compaction ring (u16 tile indices)
 
@@ -991,7 +995,9 @@ __global__ __launch_bounds__(BLOCK) void k_cells_entity(KParams P, const uint32_
 //        [5] last-workgroup counter of the pair kernel, [6] a cell bucket
 //        overflowed its capacity;
 //        kept across calls: [8] some atom is within reach of skin/2 next
-//        tick, [9] bin (and rebuild) this tick.
+//        tick, [9] bin (and rebuild) this tick, [10] this tick binned for
+//        a build one tick behind (k_list_step), [11] build this tick from
+//        the previous tick's binning.
 //
 // Predictive binning: the integrate kernel of a tick bins the atoms and
 // forces a rebuild when the previous tick found an atom whose displacement
@@ -1115,9 +1121,9 @@ __global__ __launch_bounds__(BLOCK) void k_list_integrate(
         // translation exactly and keeps any motion below h/2.
         const double4 r = xref[k];
         double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
-        dx = fma(-P.half[0], rint(dx / P.half[0]), dx);
-        dy = fma(-P.half[1], rint(dy / P.half[1]), dy);
-        dz = fma(-P.half[2], rint(dz / P.half[2]), dz);
+        dx = fma(-P.half[0], rint(dx * P.inv_half[0]), dx);
+        dy = fma(-P.half[1], rint(dy * P.inv_half[1]), dy);
+        dz = fma(-P.half[2], rint(dz * P.inv_half[2]), dz);
         const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
         far = !(d2 <= half_skin * half_skin);  // NaN counts as far
         if (PREBIN) {
@@ -1189,14 +1195,17 @@ __global__ void k_flag_set(uint32_t* flags, int idx, uint32_t v) { flags[idx] = 
 // candidate's position in that order, and each atom's entries are contiguous
 // (nbr[i * cap + k]); otherwise entries hold the atom index in the
 // lane-grouped layout of nbr_slot.
-template <bool LOCAL>
-__global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
-    KParams PL, const uint32_t* __restrict__ bcount, const uint32_t* __restrict__ bucket, uint32_t bcap,
-    const double4* __restrict__ pos4, const uint8_t* __restrict__ type, uint32_t* __restrict__ nbr,
-    uint32_t* __restrict__ nlen, double4* __restrict__ xref, uint32_t N, uint32_t logT, uint32_t cap, double cut2,
-    uint32_t* __restrict__ flags, uint32_t* __restrict__ cand_store, uint4* __restrict__ cinfo,
-    uint2* __restrict__ hrec, uint16_t* __restrict__ nbr16, size_t region) {
-    if (flags[0] == 0u) return;
+// The cells b0, b0 + stride, ... of a list build, by a workgroup of NT
+// threads; pos4 holds the positions the lists are built from.
+template <bool LOCAL, int NT>
+__device__ __forceinline__ void build_cells(
+    uint32_t b0, uint32_t stride, KParams PL, const uint32_t* __restrict__ bcount,
+    const uint32_t* __restrict__ bucket, uint32_t bcap, const double4* __restrict__ pos4,
+    const uint8_t* __restrict__ type, uint32_t* __restrict__ nbr, uint32_t* __restrict__ nlen,
+    double4* __restrict__ xref, uint32_t N, uint32_t logT, uint32_t cap, double cut2, uint32_t* __restrict__ flags,
+    uint32_t* __restrict__ cand_store, uint4* __restrict__ cinfo, uint2* __restrict__ hrec,
+    uint16_t* __restrict__ nbr16, size_t region) {
+    constexpr int NWV = NT / WAVE;
     // the candidates' indices are staged, unordered, in the memory of the
     // position tile, which is filled only after they have been ordered
     __shared__ double4 tile[BUILD_TILE];
@@ -1208,13 +1217,14 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
     __shared__ double seg_sh[27][3];
     static_assert(LDS_IDX_CAP * sizeof(uint32_t) <= BUILD_TILE * sizeof(double4), "index staging fits the tile");
     uint32_t* cand = reinterpret_cast<uint32_t*>(tile);
-    const uint32_t c = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    if (c == 0 && tid == 0) flags[3] += 1u;  // rebuilds done on the device
+    if (b0 == 0 && tid == 0) flags[3] += 1u;  // rebuilds done on the device
+    for (uint32_t c = b0; c < PL.ncell; c += stride) {
+    __syncthreads();  // the previous cell's LDS readers are done
     const uint32_t nh = min(bcount[c], bcap);
     if (nh == 0) {
         if (LOCAL && tid == 0) cinfo[c] = make_uint4(0u, 0u, 0u, 0u);
-        return;
+        continue;
     }
     const uint32_t* home = bucket + (size_t)c * bcap;
     if (tid < 27) {
@@ -1241,7 +1251,7 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
     const uint32_t ncand = seg_pref[27];
     if (ncand > (uint32_t)LDS_IDX_CAP) {
         if (tid == 0) atomicOr(&flags[2], 1u);
-        return;
+        continue;
     }
     // stencil segment of candidate t < ncand: the last s with seg_pref[s] <= t
     auto seg_of = [&](uint32_t t) {
@@ -1255,12 +1265,12 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
         }
         return lo;
     };
-    for (uint32_t t = tid; t < ncand; t += BUILD_THREADS) {
+    for (uint32_t t = tid; t < ncand; t += NT) {
         const int sg = seg_of(t);
         cand[t] = bucket[seg_base[sg] + (t - seg_pref[sg])];
     }
     __syncthreads();
-    for (uint32_t t = tid; t < ncand; t += BUILD_THREADS) {
+    for (uint32_t t = tid; t < ncand; t += NT) {
         const int sg = seg_of(t);
         const uint32_t v = cand[t], lo = seg_pref[sg], hi = seg_pref[sg + 1];
         // rank of v in its segment, eight independent reads per step
@@ -1275,7 +1285,7 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
         for (; f < hi; ++f) r += cand[f] < v ? 1u : 0u;
         ord[lo + r] = v;
     }
-    for (uint32_t i = tid; i < nh; i += BUILD_THREADS) {
+    for (uint32_t i = tid; i < nh; i += NT) {
         const uint32_t gi = home[i];
         xref[gi] = pos4[gi];
     }
@@ -1285,7 +1295,7 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
     uint16_t* reg = LOCAL ? nbr16 + (size_t)c * region : nullptr;
     if (LOCAL) {
         __syncthreads();
-        for (uint32_t t = tid; t < ncand; t += BUILD_THREADS) cand_store[(size_t)c * LDS_IDX_CAP + t] = ord[t];
+        for (uint32_t t = tid; t < ncand; t += NT) cand_store[(size_t)c * LDS_IDX_CAP + t] = ord[t];
         if (tid == 0) cinfo[c] = make_uint4(ncand, hs, nh, 0u);
     }
     const int nchunks = (int)((ncand + BUILD_TILE - 1) / BUILD_TILE);
@@ -1293,7 +1303,7 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
         const uint32_t c0 = (uint32_t)ch * BUILD_TILE;
         const uint32_t len = min((uint32_t)BUILD_TILE, ncand - c0);
         __syncthreads();
-        for (uint32_t t = tid; t < len; t += BUILD_THREADS) {
+        for (uint32_t t = tid; t < len; t += NT) {
             const uint32_t idx = c0 + t;
             const int sg = seg_of(idx);
             const uint32_t j = ord[idx];
@@ -1314,10 +1324,10 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
         const bool home_in_tile = hs >= c0 && hs + nh <= c0 + len;
         auto home_pos = [&](uint32_t ii) { return home_in_tile ? tile[hs + ii - c0] : pos4[ord[hs + ii]]; };
         double4 pi_n = (uint32_t)w < nh ? home_pos((uint32_t)w) : make_double4(0.0, 0.0, 0.0, 0.0);
-        for (uint32_t i = w; i < nh; i += BUILD_WAVES) {
+        for (uint32_t i = w; i < nh; i += NWV) {
             const uint32_t gi = ord[hs + i];
             const double4 pi = pi_n;
-            if (i + BUILD_WAVES < nh) pi_n = home_pos(i + BUILD_WAVES);
+            if (i + NWV < nh) pi_n = home_pos(i + NWV);
             uint32_t cnt = ch == 0 ? 0u : (nh <= HOME_CAP ? hcnt[i] : nlen[gi]);
             // 256 candidates per step, four per lane (independent distance
             // tests); the ballots are taken in candidate order, so the list
@@ -1363,6 +1373,96 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
             }
         }
     }
+    }  // cells
+}
+
+template <bool LOCAL>
+__global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
+    KParams PL, const uint32_t* __restrict__ bcount, const uint32_t* __restrict__ bucket, uint32_t bcap,
+    const double4* __restrict__ pos4, const uint8_t* __restrict__ type, uint32_t* __restrict__ nbr,
+    uint32_t* __restrict__ nlen, double4* __restrict__ xref, uint32_t N, uint32_t logT, uint32_t cap, double cut2,
+    uint32_t* __restrict__ flags, uint32_t* __restrict__ cand_store, uint4* __restrict__ cinfo,
+    uint2* __restrict__ hrec, uint16_t* __restrict__ nbr16, size_t region) {
+    if (flags[0] == 0u) return;
+    build_cells<LOCAL, BUILD_THREADS>(blockIdx.x, gridDim.x, PL, bcount, bucket, bcap, pos4, type, nbr, nlen, xref, N,
+                                      logT, cap, cut2, flags, cand_store, cinfo, hrec, nbr16, region);
+}
+
+// Start of a list tick with the build one tick behind the binning
+// (predictive binning, Verlet list): workgroups [0, natom) run the tick's
+// kick, drift, wrap and displacement test for their atoms as
+// k_list_integrate<true>, the others build the lists when the previous tick
+// binned (flags[11]).  That build takes the positions the previous tick
+// binned (xbin), so the lists it makes are valid from this tick on, their
+// build positions being one tick old: this tick measures displacements
+// from xbin and nothing in this launch reads what the build writes.  A
+// binning is requested two ticks ahead (own displacement plus three times
+// this tick's), so the lists in use on the binning tick are still valid.
+// A tick that builds does not bin again.
+__global__ __launch_bounds__(BLOCK) void k_list_step(
+    KParams P, KParams PL, double4* __restrict__ pos4, double* __restrict__ vx, double* __restrict__ vy,
+    double* __restrict__ vz, const double* __restrict__ ax, const double* __restrict__ ay,
+    const double* __restrict__ az, const double* __restrict__ lam_ptr, double4* __restrict__ xref,
+    double4* __restrict__ xbin, double half_skin, uint32_t* __restrict__ bcount, uint32_t* __restrict__ bucket,
+    uint32_t bcap, uint32_t* __restrict__ flags, uint32_t natom, const uint8_t* __restrict__ type,
+    uint32_t* __restrict__ nbr, uint32_t* __restrict__ nlen, uint32_t N, uint32_t logT, uint32_t cap, double cut2) {
+    const bool build_now = flags[11] != 0u;
+    if (blockIdx.x >= natom) {
+        if (!build_now) return;
+        build_cells<false, BLOCK>(blockIdx.x - natom, gridDim.x - natom, PL, bcount, bucket, bcap, xbin, type, nbr,
+                                  nlen, xref, N, logT, cap, cut2, flags, nullptr, nullptr, nullptr, nullptr, 0);
+        return;
+    }
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int k = blockIdx.x * BLOCK + tid;
+    const bool binreq = !build_now && flags[9] != 0u;
+    const bool valid = k < P.n;
+    bool far = false, warn = false;
+    uint32_t kk = 0;
+    if (valid) {
+        double4 p = pos4[k];
+        const double lam = *lam_ptr;
+        double a = vx[k] * lam, b = vy[k] * lam, c = vz[k] * lam;
+        a = a + ax[k] * P.half_dt;
+        b = b + ay[k] * P.half_dt;
+        c = c + az[k] * P.half_dt;
+        vx[k] = a;
+        vy[k] = b;
+        vz[k] = c;
+        const double sx = a * P.dt, sy = b * P.dt, sz = c * P.dt;
+        p.x = p.x + sx;
+        p.y = p.y + sy;
+        p.z = p.z + sz;
+        if (P.flags & F_WRAP) {
+            p.x = rem_euclid(p.x, 2.0 * P.half[0]) - P.half[0];
+            p.y = rem_euclid(p.y, 2.0 * P.half[1]) - P.half[1];
+            p.z = rem_euclid(p.z, 2.0 * P.half[2]) - P.half[2];
+        }
+        pos4[k] = p;
+        // displacement from the build position of the lists this tick uses,
+        // modulo h (the wrap translates the universe by h each tick)
+        const double4 r = build_now ? xbin[k] : xref[k];
+        double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+        dx = fma(-P.half[0], rint(dx * P.inv_half[0]), dx);
+        dy = fma(-P.half[1], rint(dy * P.inv_half[1]), dy);
+        dz = fma(-P.half[2], rint(dz * P.inv_half[2]), dz);
+        const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        far = !(d2 <= half_skin * half_skin);  // NaN counts as far
+        if (binreq) {
+            xbin[k] = p;
+            kk = cell_key(PL, p.x, p.y, p.z);
+        } else {
+            const double step = sqrt(fma(sz, sz, fma(sy, sy, sx * sx)));
+            warn = !(fma(3.0, step, sqrt(d2)) < half_skin);
+        }
+    }
+    if (binreq) {
+        bin_wave(kk, valid, (uint32_t)k, bcount, bucket, bcap, flags);
+        if (k == 0) flags[10] = 1u;  // the next tick builds
+    }
+    if (build_now && k == 0) flags[0] = 1u;  // the pair kernel empties the buckets
+    if (__ballot(far) != 0 && lane == 0) atomicOr(&flags[4], 1u);
+    if (__ballot(warn) != 0 && lane == 0) atomicOr(&flags[8], 1u);
 }
 
 // End of a list tick (the pair kernel's last workgroup): the rebuild flag
@@ -1373,6 +1473,8 @@ __device__ __forceinline__ void close_list_tick(uint32_t* flags) {
     flags[0] = 0u;
     flags[9] = flags[8];
     flags[8] = 0u;
+    flags[11] = flags[10];
+    flags[10] = 0u;
 }
 
 // Per-atom list traversal fused with the second half of the tick: T lanes
@@ -2175,6 +2277,7 @@ struct gs_universe {
     size_t hrec_have = 0;
     uint32_t* nlen = nullptr;
     double4* xref = nullptr;
+    double4* xbin = nullptr;    // positions binned for a build one tick behind
     uint32_t* flags = nullptr;  // device list flags and counters (k_list_integrate)
     uint32_t* bcount = nullptr;  // list-grid cell buckets of a rebuild
     uint32_t* bucket = nullptr;
@@ -2276,6 +2379,7 @@ std::vector<PairCoef> derive(gs_universe* u) {
     for (int a = 0; a < 3; ++a) {
         k.box[a] = p.box[a];
         k.half[a] = p.box[a] * 0.5;
+        k.inv_half[a] = 1.0 / k.half[a];
         double nc = std::floor(p.box[a] / rc);
         if (!(nc >= 1.0)) nc = 1.0;
         if (nc > 1024.0) nc = 1024.0;
@@ -2447,7 +2551,8 @@ int list_alloc(gs_universe* u, uint32_t cap, uint32_t bcap_min = 0) {
     int rc;
     const size_t n = (size_t)std::max(1, u->n);
     if (!u->nlen) {
-        if ((rc = dalloc(&u->nlen, n)) || (rc = dalloc(&u->xref, n)) || (rc = dalloc(&u->flags, LIST_FLAGS)) ||
+        if ((rc = dalloc(&u->nlen, n)) || (rc = dalloc(&u->xref, n)) || (rc = dalloc(&u->xbin, n)) ||
+            (rc = dalloc(&u->flags, LIST_FLAGS)) ||
             (rc = dalloc(&u->snap_pos, n)) || (rc = dalloc(&u->snap_type, n)) ||
             (rc = dalloc(&u->snap_id, n)))
             return rc;
@@ -2636,8 +2741,22 @@ int list_tick_start(gs_universe* u) {
         k_list_fill_buckets<<<nblocks(PL.ncell, BLOCK / WAVE), BLOCK, 0, u->stream>>>(PL.ncell, u->cstart, u->bcount,
                                                                                       u->bucket, u->bcap, u->flags);
         k_flag_set<<<1, 1, 0, u->stream>>>(u->flags, 0, 1u);
+        // fresh lists: no binning or delayed build pending
+        HIPCHK(hipMemsetAsync(u->flags + 8, 0, 4 * sizeof(uint32_t), u->stream));
         u->list_valid = true;
         u->list_age = 0;
+    } else if (u->prebin && u->pair_mode == 1) {
+        // integrate, and the build one tick behind, in one launch
+        const int c = u->cur;
+        const uint32_t natom = nblocks(u->n);
+        const uint32_t nbuild = u->n_cu > 0 ? std::min<uint32_t>(PL.ncell, 4u * (uint32_t)u->n_cu) : PL.ncell;
+        k_list_step<<<natom + nbuild, BLOCK, 0, u->stream>>>(
+            u->kp, PL, u->pos4[c], u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->scal, u->xref,
+            u->xbin, 0.5 * u->skin, u->bcount, u->bucket, u->bcap, u->flags, natom, u->type[c], u->nbr, u->nlen, N,
+            (uint32_t)u->logT, u->nbr_cap, rl * rl);
+        u->list_age += 1;
+        HIPCHK(hipGetLastError());
+        return GS_OK;
     } else {
         const int c = u->cur;
         const double h = 0.5 * u->skin;
@@ -2654,12 +2773,13 @@ int list_tick_start(gs_universe* u) {
         }
     }
     const int c = u->cur;
+    const uint32_t bgrid = u->n_cu > 0 ? std::min<uint32_t>(PL.ncell, BUILD_WG_PER_CU * (uint32_t)u->n_cu) : PL.ncell;
     if (u->pair_mode == 2)
-        k_list_build<true><<<PL.ncell, BUILD_THREADS, 0, u->stream>>>(
+        k_list_build<true><<<bgrid, BUILD_THREADS, 0, u->stream>>>(
             PL, u->bcount, u->bucket, u->bcap, u->pos4[c], u->type[c], nullptr, u->nlen, u->xref, N, 0u, u->nbr_cap,
             rl * rl, u->flags, u->cand_store, u->cinfo, u->hrec, u->nbr16, cell_region(u));
     else
-        k_list_build<false><<<PL.ncell, BUILD_THREADS, 0, u->stream>>>(
+        k_list_build<false><<<bgrid, BUILD_THREADS, 0, u->stream>>>(
             PL, u->bcount, u->bucket, u->bcap, u->pos4[c], u->type[c], u->nbr, u->nlen, u->xref, N,
             (uint32_t)u->logT, u->nbr_cap, rl * rl, u->flags, nullptr, nullptr, nullptr, nullptr, 0);
     u->list_age += 1;
@@ -2826,6 +2946,7 @@ int gs_destroy(gs_universe* u) {
     (void)hipFree(u->hrec);
     (void)hipFree(u->nlen);
     (void)hipFree(u->xref);
+    (void)hipFree(u->xbin);
     (void)hipFree(u->flags);
     (void)hipFree(u->bcount);
     (void)hipFree(u->bucket);

@@ -20,7 +20,9 @@ constexpr double F64_EPS = 2.220446049250313e-16;  // f64::EPSILON
 // f64::rem_euclid (Rust std), src/systems/boundary.rs:28-34 and
 // src/physics/grid.rs:230.
 __device__ __forceinline__ double rem_euclid(double a, double b) {
-    double r = fmod(a, b);
+    // fmod(a, b) is a itself when |a| < |b| (the case of every wrap after
+    // one tick of motion), so the general remainder runs only otherwise
+    const double r = fabs(a) < fabs(b) ? a : fmod(a, b);
     return r < 0.0 ? r + fabs(b) : r;
 }
 
@@ -108,6 +110,7 @@ struct AdjCoef {
 struct KParams {
     double box[3];
     double half[3];
+    double inv_half[3];    // 1 / half
     double cell[3];        // force-grid cell edge per axis
     uint32_t dims[3];      // force-grid cells per axis
     uint32_t ncell;
