@@ -488,6 +488,16 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, const double4& pi, 
     }
 }
 
+// Fixed-order pairwise sum of N (a power of two) values: ((v0 + v1) + (v2 +
+// v3)) + ...
+template <int N>
+__device__ __forceinline__ double tree_sum(const double* v) {
+    if constexpr (N == 1)
+        return v[0];
+    else
+        return tree_sum<N / 2>(v) + tree_sum<N / 2>(v + N / 2);
+}
+
 // Full-shell cell-tile pair kernel (the rayon force pass of
 // src/systems/forces.rs:228-345 reorganised for a GPU: every atom sums its
 // own pairs, so the ordered merge of :330-341 disappears).  Requires >= 3
@@ -499,8 +509,8 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, const double4& pi, 
 // [cell_off[r], cell_off[r+1])); a workgroup reads its replica's parameters
 // from Ptab and skips cells of replicas marked small (those use the
 // brute-force batch kernel).
-template <bool COUL, bool SHARED, bool EXTRAS, bool SINGLE, bool BATCH = false>
-__global__ __launch_bounds__(PAIR_THREADS) void k_pair(KParams Pk, const uint32_t* __restrict__ cstart,
+template <bool COUL, bool SHARED, bool EXTRAS, bool SINGLE, bool BATCH = false, int NW = PAIR_WAVES>
+__global__ __launch_bounds__(NW * WAVE) void k_pair(KParams Pk, const uint32_t* __restrict__ cstart,
                                                        const double4* __restrict__ pos4,
                                                        const uint8_t* __restrict__ type,
                                                        const PairCoef* __restrict__ coef, double* __restrict__ ax,
@@ -516,8 +526,8 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_pair(KParams Pk, const uint32_
     __shared__ uint32_t seg_base[27];
     __shared__ uint32_t seg_pref[28];
     __shared__ double seg_sh[27][3];
-    __shared__ uint16_t queue[PAIR_WAVES][QCAP];
-    __shared__ double wv[PAIR_WAVES];
+    __shared__ uint16_t queue[NW][QCAP];
+    __shared__ double wv[NW];
 
     const uint32_t cg = blockIdx.x;  // global cell index
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -533,7 +543,7 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_pair(KParams Pk, const uint32_
     if (SINGLE) {
         if (tid == 0) sc[0] = coef[0];
     } else {
-        for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += PAIR_THREADS) sc[t] = coef[t];
+        for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += NW * WAVE) sc[t] = coef[t];
     }
     if (tid < 27) {
         const uint32_t nx = P.dims[0], ny = P.dims[1], nz = P.dims[2];
@@ -567,7 +577,7 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_pair(KParams Pk, const uint32_
         const uint32_t c0 = (uint32_t)ch * TILE_CAP;
         const uint32_t len = min((uint32_t)TILE_CAP, ncand - c0);
         __syncthreads();
-        for (uint32_t t = tid; t < len; t += PAIR_THREADS) {
+        for (uint32_t t = tid; t < len; t += NW * WAVE) {
             const uint32_t idx = c0 + t;
             int s = 0;
             while (seg_pref[s + 1] <= idx) ++s;
@@ -581,7 +591,7 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_pair(KParams Pk, const uint32_
         }
         __syncthreads();
 
-        for (uint32_t i = w; i < nh; i += PAIR_WAVES) {
+        for (uint32_t i = w; i < nh; i += NW) {
             const double4 pi = pos4[h0 + i];
             const int ti = type[h0 + i];
             const double kqi = P.ke * pi.w;
@@ -675,7 +685,7 @@ __global__ __launch_bounds__(PAIR_THREADS) void k_pair(KParams Pk, const uint32_
     vlane = wave_sum(vlane);
     if (lane == 0) wv[w] = vlane;
     __syncthreads();
-    if (tid == 0) vpart[cg] = 0.5 * ((wv[0] + wv[1]) + (wv[2] + wv[3]));
+    if (tid == 0) vpart[cg] = 0.5 * tree_sum<NW>(wv);
 }
 
 // Small-box pair kernel (any axis with < 3 cells): every ordered pair, the
@@ -2304,6 +2314,7 @@ struct gs_universe {
     double* adj_qs = nullptr;
     double* adj_acc = nullptr;
     double* adj_gatom = nullptr;
+    double* adj_spart = nullptr;  // species-reduction partials, 32 per block
     double* adj_rows = nullptr;
     size_t adj_rows_cap = 0;
     double* adj_ck = nullptr;
@@ -2506,11 +2517,21 @@ int launch_pair(gs_universe* u, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr
                          (const double4*)u->pos4[c], (const uint8_t*)u->type[c], (const PairCoef*)u->coef, u->a[0],
                          u->a[1], u->a[2], u->pe, u->sabs, u->vpart);
     } else {
-#define GS_PAIR(C, S, E, SG)                                                                                      \
-    launch_timed(k_pair<C, S, E, SG>, dim3(P.ncell), dim3(PAIR_THREADS), u->stream, e0, e1, P,                    \
+    // a grid of few cells gets 16 waves per cell (their home atoms shared
+    // among them), so the device holds several waves per SIMD
+    const bool wide = P.ncell < 2u * (uint32_t)std::max(1, u->n_cu);
+#define GS_PAIR_NW(C, S, E, SG, NW)                                                                               \
+    launch_timed(k_pair<C, S, E, SG, false, NW>, dim3(P.ncell), dim3(NW * WAVE), u->stream, e0, e1, P,            \
                  (const uint32_t*)u->cstart, (const double4*)u->pos4[c], (const uint8_t*)u->type[c],             \
                  (const PairCoef*)u->coef, u->a[0], u->a[1], u->a[2], u->pe, u->sabs, u->vpart,                   \
                  (const KParams*)nullptr, (const uint16_t*)nullptr, (const uint32_t*)nullptr)
+#define GS_PAIR(C, S, E, SG)                    \
+    do {                                        \
+        if (wide)                               \
+            GS_PAIR_NW(C, S, E, SG, 16);        \
+        else                                    \
+            GS_PAIR_NW(C, S, E, SG, PAIR_WAVES); \
+    } while (0)
         if (u->single)
             GS_PAIR(false, false, EXTRAS, true);
         else if (!u->coul)
@@ -2520,6 +2541,7 @@ int launch_pair(gs_universe* u, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr
         else
             GS_PAIR(true, false, EXTRAS, false);
 #undef GS_PAIR
+#undef GS_PAIR_NW
     }
     HIPCHK(hipGetLastError());
     return GS_OK;

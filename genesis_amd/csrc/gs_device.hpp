@@ -103,7 +103,7 @@ struct AdjCoef {
     double sigma;            // sigma_ij
     double eps;              // eps_ij = k sqrt(epsK_i epsK_j)
     double eps_over_epsK_i;  // eps_ij / epsK_i
-    double pad;
+    double inv_sigma;        // 1 / sigma_ij
 };
 
 // Scalar parameters every kernel reads (passed by value).
