@@ -106,6 +106,8 @@ _SIGNATURES = {
     "gs_batch_download": (ctypes.c_int, [_H, ctypes.c_int32] + [_D] * 9),
     "gs_batch_step": (ctypes.c_int, [_H, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(Observables)]),
     "gs_batch_autotune": (ctypes.c_int, [_H, ctypes.c_int32, _D, _D]),
+    "gs_batch_get_tuning": (ctypes.c_int, [_H, ctypes.POINTER(Tuning)]),
+    "gs_batch_set_tuning": (ctypes.c_int, [_H, ctypes.POINTER(Tuning)]),
     "gs_timing_enable": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int]),
     "gs_timing_read": (ctypes.c_int, [_H, ctypes.POINTER(Timing)]),
     "gs_clusters": (ctypes.c_int, [_H, ctypes.c_double, _U32, ctypes.POINTER(ClusterStats)]),

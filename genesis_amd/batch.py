@@ -58,9 +58,29 @@ class GpuBatch:
         return a.reshape(n_steps, self.R, 6).copy()
 
     def autotune(self, trial_steps: int = 20):
+        """Times every candidate setting (cell tiles at three cell sizes,
+        Verlet lists at three skins and two lane counts) and keeps the
+        fastest."""
         s, ms = ctypes.c_double(), ctypes.c_double()
         _lib.check(self.lib.gs_batch_autotune(self.h, trial_steps, ctypes.byref(s), ctypes.byref(ms)))
-        return {"cell_scale": s.value, "ms_per_tick": ms.value}
+        return {"cell_scale": s.value, "ms_per_tick": ms.value, **self.get_tuning()}
+
+    def get_tuning(self) -> dict:
+        t = _lib.Tuning()
+        _lib.check(self.lib.gs_batch_get_tuning(self.h, ctypes.byref(t)))
+        return {"pair_mode": t.pair_mode, "lanes_per_atom": t.lanes_per_atom, "skin": t.skin,
+                "sort_every": t.sort_every, "list_builds": t.list_builds, "list_replays": t.list_replays}
+
+    def set_tuning(self, pair_mode: int, lanes_per_atom: int = 2, skin: Optional[float] = None,
+                   sort_every: int = 0) -> None:
+        """pair_mode 0: cell tiles every tick; 1: Verlet lists (skin in length
+        units, default 0.12 rc)."""
+        t = _lib.Tuning()
+        t.pair_mode = pair_mode
+        t.lanes_per_atom = lanes_per_atom
+        t.skin = self.get_tuning()["skin"] if skin is None else skin
+        t.sort_every = sort_every
+        _lib.check(self.lib.gs_batch_set_tuning(self.h, ctypes.byref(t)))
 
     def close(self) -> None:
         if getattr(self, "h", None):

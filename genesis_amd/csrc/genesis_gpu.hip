@@ -364,12 +364,11 @@ __device__ __forceinline__ void pair_law_sp(const KParams& P, double sig_i, doub
 // are interleaved to fill them.  p: the four neighbour positions; tt: their
 // species + 1 in successive bytes, 0 for a slot past the end of the list.
 template <bool COUL, bool SHARED, bool GRAV>
-__device__ __forceinline__ void pair_eval4(const KParams& P, const double4& pi, double kqi, double sig_i, double a4_i,
+__device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double Ly, double Lz, double iLx, double iLy,
+                                           double iLz, const double4& pi, double kqi, double sig_i, double a4_i,
                                            const SpCoef* ssp, const double* irc_row, const PairCoef* row,
                                            const double4 (&p)[4], uint32_t tt, double& fx, double& fy, double& fz,
                                            double& vl) {
-    const double Lx = P.box[0], Ly = P.box[1], Lz = P.box[2];
-    const double iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
     bool live[4], ok[4], core[4];
     int tj[4];
     double dx[4], dy[4], dz[4], r2[4], y[4], e[4], inv_d[4], d[4], sig[4], eps4[4], irc[4], r[4], inv_r[4];
@@ -688,55 +687,87 @@ __global__ __launch_bounds__(NW * WAVE) void k_pair(KParams Pk, const uint32_t* 
     if (tid == 0) vpart[cg] = 0.5 * tree_sum<NW>(wv);
 }
 
+// Replica tables of a batched launch (gs_batch): the list-grid parameters
+// and first list-grid cell of each replica (a replica on brute-force pairs
+// keeps its force grid there, for the sort only), the replica of each
+// list-grid cell and of each atom, and each replica's box (w = 1 for a
+// replica on Verlet lists, 0 for one on brute-force pairs) with its
+// reciprocal.
+struct BatchTabs {
+    const KParams* PL;
+    const uint32_t* lcell_off;
+    const uint16_t* lcell_rep;
+    const uint16_t* arep;
+    const double4* box;
+    const double4* ibox;
+    uint32_t ncell;
+};
+
+// Workgroup of a batched launch over atoms: its replica and first atom
+// (blocks never straddle two replicas).
+struct BlockRef {
+    uint32_t rep;
+    uint32_t first;
+};
+
+constexpr int BRUTE_WAVES = 4;  // home atoms of a brute-force workgroup, one per wave
+
 // Small-box pair kernel (any axis with < 3 cells): every ordered pair, the
 // reference's min_image, and in GS_GRID_REFERENCE mode the multiplicity the
 // reference grid gives a pair on an axis with exactly 2 cells (each wrapping
 // offset reaches the other cell, src/physics/grid.rs:134-151,275-295): x2 per
-// such axis on which the two atoms' cells differ.
-template <bool COUL, bool EXTRAS>
-__global__ __launch_bounds__(BLOCK) void k_pair_brute(KParams P, const double4* __restrict__ pos4,
-                                                      const uint8_t* __restrict__ type,
-                                                      const PairCoef* __restrict__ coef, double* __restrict__ ax,
-                                                      double* __restrict__ ay, double* __restrict__ az,
-                                                      double* __restrict__ pe_atom, double* __restrict__ s_abs,
-                                                      double* __restrict__ vpart) {
-    __shared__ double4 tp[BLOCK];
-    __shared__ uint8_t tt[BLOCK];
-    __shared__ uint32_t tk[BLOCK];
+// such axis on which the two atoms' cells differ.  One wave per home atom:
+// the lanes sweep the universe's atoms 64 at a time with the distance test,
+// a ballot compacts the in-range candidates into a per-wave LDS ring, and
+// full 64-lane batches evaluate the law, so the expensive part runs on
+// in-range pairs only.  The atom's sums are wave reductions (fixed order).
+// vpart[workgroup] = half the pair energy of its home atoms.
+// BATCH: workgroup b takes home atoms hb[b].first .. + BRUTE_WAVES - 1 of
+// replica hb[b].rep (clipped at its end), with that replica's parameters and
+// atom range.
+template <bool COUL, bool EXTRAS, bool BATCH>
+__global__ __launch_bounds__(BRUTE_WAVES * WAVE) void k_pair_brute(
+    KParams Pk, const double4* __restrict__ pos4, const uint8_t* __restrict__ type,
+    const PairCoef* __restrict__ coef, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    double* __restrict__ pe_atom, double* __restrict__ s_abs, double* __restrict__ vpart,
+    const KParams* __restrict__ Ptab, const BlockRef* __restrict__ hb, const uint32_t* __restrict__ atom_off) {
     __shared__ PairCoef sc[GS_NSPECIES * GS_NSPECIES];
-    __shared__ double wv[BLOCK / WAVE];
-    const int tid = threadIdx.x;
-    for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += BLOCK) sc[t] = coef[t];
-    const int i = blockIdx.x * BLOCK + tid;
-    const bool act = i < P.n;
-    double4 pi = act ? pos4[i] : make_double4(0, 0, 0, 0);
-    const int ti = act ? type[i] : 0;
-    const uint32_t ki = act ? cell_key(P, pi.x, pi.y, pi.z) : 0;
-    const uint32_t cxi = ki % P.dims[0], cyi = (ki / P.dims[0]) % P.dims[1], czi = ki / (P.dims[0] * P.dims[1]);
-    const bool refm = (P.flags & F_REF_MULT) != 0;
-    double fx = 0, fy = 0, fz = 0, pe = 0, sa = 0, vl = 0;
-    for (int base = 0; base < P.n; base += BLOCK) {
-        __syncthreads();
-        if (base + tid < P.n) {
-            tp[tid] = pos4[base + tid];
-            tt[tid] = type[base + tid];
-            tk[tid] = cell_key(P, tp[tid].x, tp[tid].y, tp[tid].z);
-        }
-        __syncthreads();
-        const int lim = min(BLOCK, P.n - base);
-        if (act) {
-            for (int jj = 0; jj < lim; ++jj) {
-                const int j = base + jj;
-                if (j == i) continue;
-                const double4 pj = tp[jj];
+    __shared__ uint32_t queue[BRUTE_WAVES][QCAP];
+    __shared__ double wv[BRUTE_WAVES];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    uint32_t rep = 0, lo = 0, hi = (uint32_t)Pk.n, first = blockIdx.x * BRUTE_WAVES;
+    if (BATCH) {
+        const BlockRef br = hb[blockIdx.x];
+        rep = br.rep;
+        lo = atom_off[rep];
+        hi = atom_off[rep + 1];
+        first = br.first;
+    }
+    const KParams& P = BATCH ? Ptab[rep] : Pk;
+    for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += BRUTE_WAVES * WAVE) sc[t] = coef[t];
+    __syncthreads();
+    const uint32_t i = first + (uint32_t)w;
+    double vl = 0.0;
+    if (i < hi) {  // uniform over the wave
+        const double4 pi = pos4[i];
+        const int ti = type[i];
+        const double kqi = P.ke * pi.w;
+        const bool refm = (P.flags & F_REF_MULT) != 0;
+        const uint32_t ki = refm ? cell_key(P, pi.x, pi.y, pi.z) : 0u;
+        const uint32_t cxi = ki % P.dims[0], cyi = (ki / P.dims[0]) % P.dims[1], czi = ki / (P.dims[0] * P.dims[1]);
+        double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, sa = 0.0;
+        uint32_t qn = 0, qh = 0;
+        auto batch = [&](uint32_t cnt) {
+            if ((uint32_t)lane < cnt) {
+                const uint32_t j = queue[w][(qh + lane) & (QCAP - 1)];
+                const double4 pj = pos4[j];
                 const double dx = min_image1(pi.x - pj.x, P.box[0]);
                 const double dy = min_image1(pi.y - pj.y, P.box[1]);
                 const double dz = min_image1(pi.z - pj.z, P.box[2]);
                 const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                if (!(r2 < P.rc2 && r2 > P.eps2)) continue;
                 double mult = 1.0;
                 if (refm) {
-                    const uint32_t kj = tk[jj];
+                    const uint32_t kj = cell_key(P, pj.x, pj.y, pj.z);
                     const uint32_t cxj = kj % P.dims[0], cyj = (kj / P.dims[0]) % P.dims[1],
                                    czj = kj / (P.dims[0] * P.dims[1]);
                     if (P.dims[0] == 2 && cxi != cxj) mult *= 2.0;
@@ -745,35 +776,57 @@ __global__ __launch_bounds__(BLOCK) void k_pair_brute(KParams P, const double4* 
                 }
                 const double inv_d = rsqrt_refined(r2);
                 const double d = r2 * inv_d;
-                const int tj = tt[jj];
                 double fmag, vv;
-                pair_law<COUL, false>(P, sc[ti * GS_NSPECIES + tj], d, inv_d, P.ke * pi.w * pj.w, fmag, vv);
+                pair_law<COUL, false>(P, sc[ti * GS_NSPECIES + type[j]], d, inv_d, kqi * pj.w, fmag, vv);
                 fmag *= mult;
                 vv *= mult;
                 const double fs = fmag * inv_d;
                 fx = fma(fs, dx, fx);
                 fy = fma(fs, dy, fy);
                 fz = fma(fs, dz, fz);
-                vl += vv;
                 pe += vv;
-                sa += fabs(fmag);
+                if (EXTRAS) sa += fabs(fmag);
+            }
+            qh += cnt;
+        };
+        for (uint32_t base = lo; base < hi; base += 64) {
+            const uint32_t j = base + (uint32_t)lane;
+            bool in = false;
+            if (j < hi && j != i) {
+                const double4 pj = pos4[j];
+                const double dx = min_image1(pi.x - pj.x, P.box[0]);
+                const double dy = min_image1(pi.y - pj.y, P.box[1]);
+                const double dz = min_image1(pi.z - pj.z, P.box[2]);
+                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                in = r2 < P.rc2 && r2 > P.eps2;
+            }
+            const uint64_t m = __ballot(in);
+            if (in) queue[w][(qn + mask_rank(m)) & (QCAP - 1)] = j;
+            qn += (uint32_t)__popcll(m);
+            __builtin_amdgcn_wave_barrier();
+            if (qn - qh >= 64) batch(64);
+        }
+        if (qn > qh) batch(qn - qh);
+        fx = wave_sum(fx);
+        fy = wave_sum(fy);
+        fz = wave_sum(fz);
+        pe = wave_sum(pe);
+        if (EXTRAS) sa = wave_sum(sa);
+        vl = pe;
+        if (lane == 0) {
+            const double im = P.inv_mass[ti];
+            ax[i] = fx * im;
+            ay[i] = fy * im;
+            az[i] = fz * im;
+            if (EXTRAS) {
+                pe_atom[i] = 0.5 * pe;
+                s_abs[i] = sa;
             }
         }
     }
-    if (act) {
-        const double im = P.inv_mass[ti];
-        ax[i] = fx * im;
-        ay[i] = fy * im;
-        az[i] = fz * im;
-        if (EXTRAS) {
-            pe_atom[i] = 0.5 * pe;
-            s_abs[i] = sa;
-        }
-    }
-    vl = wave_sum(vl);
-    if ((tid & 63) == 0) wv[tid >> 6] = vl;
+    if (lane == 0) wv[w] = vl;
     __syncthreads();
-    if (tid == 0) vpart[blockIdx.x] = 0.5 * ((wv[0] + wv[1]) + (wv[2] + wv[3]));
+    if (tid == 0) vpart[blockIdx.x] = 0.5 * tree_sum<BRUTE_WAVES>(wv);
 }
 
 // Deterministic block sum: fixed per-thread strides then a fixed tree.
@@ -1207,14 +1260,17 @@ __global__ void k_flag_set(uint32_t* flags, int idx, uint32_t v) { flags[idx] = 
 // lane-grouped layout of nbr_slot.
 // The cells b0, b0 + stride, ... of a list build, by a workgroup of NT
 // threads; pos4 holds the positions the lists are built from.
-template <bool LOCAL, int NT>
+// BATCH (gs_batch): the cells are the list-grid cells of all replicas; a
+// cell's stencil lies in its own replica's grid (bt), and cells of replicas
+// on brute-force pairs have empty buckets.
+template <bool LOCAL, int NT, bool BATCH = false>
 __device__ __forceinline__ void build_cells(
     uint32_t b0, uint32_t stride, KParams PL, const uint32_t* __restrict__ bcount,
     const uint32_t* __restrict__ bucket, uint32_t bcap, const double4* __restrict__ pos4,
     const uint8_t* __restrict__ type, uint32_t* __restrict__ nbr, uint32_t* __restrict__ nlen,
     double4* __restrict__ xref, uint32_t N, uint32_t logT, uint32_t cap, double cut2, uint32_t* __restrict__ flags,
     uint32_t* __restrict__ cand_store, uint4* __restrict__ cinfo, uint2* __restrict__ hrec,
-    uint16_t* __restrict__ nbr16, size_t region) {
+    uint16_t* __restrict__ nbr16, size_t region, const BatchTabs& bt = BatchTabs{}) {
     constexpr int NWV = NT / WAVE;
     // the candidates' indices are staged, unordered, in the memory of the
     // position tile, which is filled only after they have been ordered
@@ -1229,7 +1285,8 @@ __device__ __forceinline__ void build_cells(
     uint32_t* cand = reinterpret_cast<uint32_t*>(tile);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (b0 == 0 && tid == 0) flags[3] += 1u;  // rebuilds done on the device
-    for (uint32_t c = b0; c < PL.ncell; c += stride) {
+    const uint32_t ncell = BATCH ? bt.ncell : PL.ncell;
+    for (uint32_t c = b0; c < ncell; c += stride) {
     __syncthreads();  // the previous cell's LDS readers are done
     const uint32_t nh = min(bcount[c], bcap);
     if (nh == 0) {
@@ -1238,14 +1295,18 @@ __device__ __forceinline__ void build_cells(
     }
     const uint32_t* home = bucket + (size_t)c * bcap;
     if (tid < 27) {
-        const uint32_t nx = PL.dims[0], ny = PL.dims[1], nz = PL.dims[2];
-        const int cx = (int)(c % nx), cy = (int)((c / nx) % ny), cz = (int)(c / (nx * ny));
+        const uint32_t rep = BATCH ? bt.lcell_rep[c] : 0u;
+        const KParams& Q = BATCH ? bt.PL[rep] : PL;
+        const uint32_t cbase = BATCH ? bt.lcell_off[rep] : 0u;
+        const uint32_t lc = c - cbase;
+        const uint32_t nx = Q.dims[0], ny = Q.dims[1], nz = Q.dims[2];
+        const int cx = (int)(lc % nx), cy = (int)((lc / nx) % ny), cz = (int)(lc / (nx * ny));
         int ix = cx + tid % 3 - 1, iy = cy + (tid / 3) % 3 - 1, iz = cz + tid / 9 - 1;
         double sx = 0.0, sy = 0.0, sz = 0.0;
-        if (ix < 0) { ix += nx; sx = -PL.box[0]; } else if (ix >= (int)nx) { ix -= nx; sx = PL.box[0]; }
-        if (iy < 0) { iy += ny; sy = -PL.box[1]; } else if (iy >= (int)ny) { iy -= ny; sy = PL.box[1]; }
-        if (iz < 0) { iz += nz; sz = -PL.box[2]; } else if (iz >= (int)nz) { iz -= nz; sz = PL.box[2]; }
-        const uint32_t nc = (uint32_t)ix + nx * ((uint32_t)iy + ny * (uint32_t)iz);
+        if (ix < 0) { ix += nx; sx = -Q.box[0]; } else if (ix >= (int)nx) { ix -= nx; sx = Q.box[0]; }
+        if (iy < 0) { iy += ny; sy = -Q.box[1]; } else if (iy >= (int)ny) { iy -= ny; sy = Q.box[1]; }
+        if (iz < 0) { iz += nz; sz = -Q.box[2]; } else if (iz >= (int)nz) { iz -= nz; sz = Q.box[2]; }
+        const uint32_t nc = cbase + (uint32_t)ix + nx * ((uint32_t)iy + ny * (uint32_t)iz);
         seg_base[tid] = nc * bcap;
         seg_pref[tid + 1] = min(bcount[nc], bcap);
         seg_sh[tid][0] = sx;
@@ -1496,7 +1557,12 @@ __device__ __forceinline__ void close_list_tick(uint32_t* flags) {
 // kinetic energy; the last workgroup reduces the per-block kinetic and
 // potential partials in a fixed order, computes the Berendsen lambda and the
 // observables (close_tick) and clears the rebuild flag for the next tick.
-template <int LOGT, bool COUL, bool SHARED, bool SINGLE, bool GRAV>
+// BATCH (gs_batch): the atoms of many replicas; each atom takes its
+// replica's box from the batch tables and atoms of replicas on brute-force
+// pairs are skipped.  The kernel stores a = F/m and the atom's potential
+// energy (half its pair sum) in pe_out; the kick and the per-replica
+// reductions run in the batch's own kernels.
+template <int LOGT, bool COUL, bool SHARED, bool SINGLE, bool GRAV, bool BATCH = false>
 __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, const double4* __restrict__ pos4,
                                                      const uint8_t* __restrict__ type,
                                                      const uint32_t* __restrict__ nbr,
@@ -1509,7 +1575,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
                                                      double* __restrict__ lam_out, gs_observables* __restrict__ obs,
                                                      ThermoArgs th, const double* __restrict__ abond,
                                                      const double* __restrict__ vbpart, uint32_t nvb, uint32_t rpc,
-                                                     uint32_t* __restrict__ bcount, uint32_t nbucket) {
+                                                     uint32_t* __restrict__ bcount, uint32_t nbucket,
+                                                     BatchTabs bt = BatchTabs{}, double* __restrict__ pe_out = nullptr) {
     constexpr int T = 1 << LOGT;
     __shared__ PairCoef sc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];
     __shared__ SpCoef ssp[GS_NSPECIES];
@@ -1550,13 +1617,29 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     // a build that met a stencil too dense for LDS or a full bucket left some
     // lists unwritten: the host replays this call, so no list is read now
     const bool lists_ok = (flags[2] | flags[6]) == 0u;
+    bool own = i < N;  // BATCH: and the atom's replica is on lists
     if (i < N) {
         const double4 pi = pos4[i];
         ti = type[i];
         const double kqi = P.ke * pi.w;
-        const double Lx = P.box[0], Ly = P.box[1], Lz = P.box[2];
-        const double iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
-        const uint32_t len = lists_ok ? nlen[i] : 0u;
+        double Lx = P.box[0], Ly = P.box[1], Lz = P.box[2];
+        double iLx, iLy, iLz;
+        if (BATCH) {
+            const uint32_t rep = bt.arep[i];
+            const double4 bx = bt.box[rep], ib = bt.ibox[rep];
+            Lx = bx.x;
+            Ly = bx.y;
+            Lz = bx.z;
+            iLx = ib.x;
+            iLy = ib.y;
+            iLz = ib.z;
+            own = bx.w != 0.0;
+        } else {
+            iLx = 1.0 / Lx;
+            iLy = 1.0 / Ly;
+            iLz = 1.0 / Lz;
+        }
+        const uint32_t len = lists_ok && own ? nlen[i] : 0u;
         const PairCoef* row = SINGLE ? sc : sc + ti * GS_NSPECIES;
         const double sig_i = SINGLE ? 0.0 : ssp[ti].sigma, a4_i = SINGLE ? 0.0 : ssp[ti].a4;
         // this lane's entries: s, s+T, ... (cnt of them), four per group;
@@ -1614,7 +1697,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
                     vl += ok ? vv : 0.0;
                 }
             } else {
-                pair_eval4<COUL, SHARED, GRAV>(P, pi, kqi, sig_i, a4_i, ssp, sirc + ti * GS_NSPECIES, row, p, tt, fx, fy,
+                pair_eval4<COUL, SHARED, GRAV>(P, Lx, Ly, Lz, iLx, iLy, iLz, pi, kqi, sig_i, a4_i, ssp,
+                                               sirc + ti * GS_NSPECIES, row, p, tt, fx, fy,
                                                fz, vl);
             }
         };
@@ -1649,6 +1733,17 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
         fx += __shfl_xor(fx, off, WAVE);
         fy += __shfl_xor(fy, off, WAVE);
         fz += __shfl_xor(fz, off, WAVE);
+        if (BATCH) vl += __shfl_xor(vl, off, WAVE);
+    }
+    if (BATCH) {
+        if (own && s == 0) {
+            const double im = P.inv_mass[ti];
+            ax[i] = fx * im;
+            ay[i] = fy * im;
+            az[i] = fz * im;
+            pe_out[i] = 0.5 * vl;
+        }
+        return;
     }
     double ke = 0.0;
     if (i < N && s == 0) {
@@ -2446,7 +2541,7 @@ std::vector<PairCoef> derive(gs_universe* u) {
         sp[j].sigma = p.sigma[j];
         sp[j].a4 = std::sqrt(4.0 * p.thermal_constant * p.epsilon_k[j]);
     }
-    u->nvpart = u->small_box ? nblocks(u->n) : k.ncell;
+    u->nvpart = u->small_box ? nblocks(u->n, BRUTE_WAVES) : k.ncell;
     if (u->nvpart == 0) u->nvpart = 1;
     // the tiles (a cell each), the lists (N << MAX_LOGT lanes at most) and
     // the cell-staged pass (a list cell each, eight slabs)
@@ -2508,14 +2603,18 @@ int launch_pair(gs_universe* u, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr
     const KParams& P = u->kp;
     const int c = u->cur;
     if (u->small_box) {
+        const dim3 grid(nblocks(u->n, BRUTE_WAVES)), blk(BRUTE_WAVES * WAVE);
+        const KParams* nt = nullptr;
+        const BlockRef* nb = nullptr;
+        const uint32_t* no = nullptr;
         if (u->coul)
-            launch_timed(k_pair_brute<true, EXTRAS>, dim3(nblocks(u->n)), dim3(BLOCK), u->stream, e0, e1, P,
+            launch_timed(k_pair_brute<true, EXTRAS, false>, grid, blk, u->stream, e0, e1, P,
                          (const double4*)u->pos4[c], (const uint8_t*)u->type[c], (const PairCoef*)u->coef, u->a[0],
-                         u->a[1], u->a[2], u->pe, u->sabs, u->vpart);
+                         u->a[1], u->a[2], u->pe, u->sabs, u->vpart, nt, nb, no);
         else
-            launch_timed(k_pair_brute<false, EXTRAS>, dim3(nblocks(u->n)), dim3(BLOCK), u->stream, e0, e1, P,
+            launch_timed(k_pair_brute<false, EXTRAS, false>, grid, blk, u->stream, e0, e1, P,
                          (const double4*)u->pos4[c], (const uint8_t*)u->type[c], (const PairCoef*)u->coef, u->a[0],
-                         u->a[1], u->a[2], u->pe, u->sabs, u->vpart);
+                         u->a[1], u->a[2], u->pe, u->sabs, u->vpart, nt, nb, no);
     } else {
     // a grid of few cells gets 16 waves per cell (their home atoms shared
     // among them), so the device holds several waves per SIMD
@@ -2678,7 +2777,8 @@ int launch_pair_list(gs_universe* u, gs_observables* obs, hipEvent_t e0 = nullpt
                  (const double4*)u->pos4[c], (const uint8_t*)u->type[c], (const uint32_t*)u->nbr,               \
                  (const uint32_t*)u->nlen, (const PairCoef*)u->coef, u->a[0], u->a[1], u->a[2], u->v[c][0],      \
                  u->v[c][1], u->v[c][2], u->vpart, u->kpart, u->flags, u->scal, obs, th,                          \
-                 (const double*)u->ab_live, (const double*)u->vb_live, u->nvb_live, rpc, u->bcount, u->kpl.ncell)
+                 (const double*)u->ab_live, (const double*)u->vb_live, u->nvb_live, rpc, u->bcount, u->kpl.ncell, \
+                 BatchTabs{}, (double*)nullptr)
 #define GS_PL_T(C, S, SG, G)                   \
     switch (u->logT) {                         \
         case 0: GS_PL(0, C, S, SG, G); break;  \

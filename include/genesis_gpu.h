@@ -308,9 +308,13 @@ int gs_adjoint_backward(gs_universe* u, double* dloss_dtheta);
  * Universe-per-temperature loop of examples/temperature_sweep.rs:70-113) -----
  * R independent universes of uneven sizes stepped together: one launch per
  * phase for all replicas.  params[r] and n_atoms[r] describe replica r; all
- * replicas share the element table and the enabled laws, and may differ in
- * box, dt and thermostat settings.  Replicas with < 3 cells per axis use a
- * brute-force pair kernel (grid_semantics as in gs_params). */
+ * replicas share the element table, the enabled laws and cutoffs, and may
+ * differ in box, dt and thermostat settings.  The pair path is chosen by
+ * gs_batch_set_tuning: Verlet lists (pair_mode 1, the default: 2 lanes per
+ * atom, skin 0.12 rc, re-sort every 100 ticks) for every replica with >= 3
+ * list cells of edge rc + skin per axis, or the per-tick cell tiles
+ * (pair_mode 0) for every replica with >= 3 cells per axis.  The other
+ * replicas use a brute-force pair kernel (grid_semantics as in gs_params). */
 typedef struct gs_batch gs_batch;
 int gs_batch_create(int32_t n_replicas, const gs_params* params, const int64_t* n_atoms, gs_batch** out);
 int gs_batch_destroy(gs_batch* b);
@@ -329,9 +333,17 @@ int gs_batch_download(gs_batch* b, int32_t replica,
  * (rounded up to even) in a hipGraph once and replays it.  obs (may be NULL)
  * receives n_steps * n_replicas records, tick-major. */
 int gs_batch_step(gs_batch* b, int64_t n_steps, int32_t graph_segment, gs_observables* obs);
-/* Project autotuner of the batched path (cell edge = s * rc, s in
- * {1, 1.15, 1.3}); keeps the fastest.  Outputs may be NULL. */
+/* Project autotuner of the batched path: the cell tiles with cell edge
+ * s * rc, s in {1, 1.15, 1.3}, and the Verlet lists with skin in
+ * {0.06, 0.12, 0.18} rc at 2 and 4 lanes per atom; keeps the fastest (read
+ * it with gs_batch_get_tuning).  best_scale: the tiles' cell scale of the
+ * kept setting (1 for the lists).  Outputs may be NULL. */
 int gs_batch_autotune(gs_batch* b, int32_t trial_steps, double* best_scale, double* best_ms_per_tick);
+/* Pair-path tuning of the batch (gs_tuning as for one universe; pair_mode 0
+ * or 1; skin in length units; sort_every 0 keeps the current period).  The
+ * setting applies to every replica whose box admits it. */
+int gs_batch_get_tuning(gs_batch* b, gs_tuning* out);
+int gs_batch_set_tuning(gs_batch* b, const gs_tuning* t);
 
 #ifdef __cplusplus
 }

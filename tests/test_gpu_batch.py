@@ -19,12 +19,18 @@ def small_sweep():
     return out
 
 
-def test_batch_matches_replicas_run_alone():
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_matches_replicas_run_alone(mode):
+    # mode 0: cell tiles every tick; mode 1: Verlet lists (the replicas with
+    # < 3 list cells per axis, 512 and 700 atoms here, on brute force)
     from genesis_amd.batch import GpuBatch
     from genesis_amd.universe import GpuUniverse
     ws = small_sweep()
-    print("[gpu-batch] sizes", [w.state.n for w in ws], "dims", [w.physics.grid_dims() for w in ws], flush=True)
+    print(f"[gpu-batch] mode {mode} sizes", [w.state.n for w in ws], "dims",
+          [w.physics.grid_dims() for w in ws], flush=True)
     with GpuBatch([w.physics for w in ws], [w.state for w in ws]) as b:
+        b.set_tuning(mode)
+        assert b.get_tuning()["pair_mode"] == mode
         obs_b = b.step(30, observe=True)
         got = [b.download(r) for r in range(len(ws))]
     for r, w in enumerate(ws):
@@ -38,12 +44,18 @@ def test_batch_matches_replicas_run_alone():
         np.testing.assert_allclose(got[r].vx, ref.vx, rtol=0, atol=1e-9 * np.abs(ref.vx).max())
 
 
-def test_graph_replay_is_bitwise_identical_to_plain_launches():
+@pytest.mark.parametrize("mode", [0, 1])
+def test_graph_replay_is_bitwise_identical_to_plain_launches(mode):
+    # list mode: re-sort every 25 ticks, so the 60 ticks hold re-sorts (plain
+    # launches) between replays of the 10-tick graph, and list rebuilds
+    # inside it
     from genesis_amd.batch import GpuBatch
     ws = small_sweep()
     res = []
     for seg in (0, 10):
+        print(f"[gpu-batch] graph replay mode {mode} segment {seg}", flush=True)
         with GpuBatch([w.physics for w in ws], [w.state for w in ws]) as b:
+            b.set_tuning(mode, sort_every=25)
             obs = b.step(40, graph_segment=seg, observe=True)
             b.step(20, graph_segment=seg)
             res.append((obs, [b.download(r) for r in range(len(ws))]))
@@ -61,6 +73,7 @@ def test_batch_autotune_restores_state():
         t = b.autotune(6)
         after = [b.download(r) for r in range(3)]
         assert t["cell_scale"] in (1.0, 1.15, 1.3) and t["ms_per_tick"] > 0
+        assert t["pair_mode"] in (0, 1)
         for r in range(3):
             np.testing.assert_array_equal(np.sort(before[r].x), np.sort(after[r].x))
         obs = b.step(10, observe=True)
@@ -74,15 +87,59 @@ def test_temperature_sweep_workload_matches_the_oracle():
     # run of that replica alone.
     from genesis_amd.batch import GpuBatch
     from oracle import oracle as O
+    # Replica 1 (651 atoms) is on a 2-cell grid (brute force with the
+    # reference's multiplicity), 2 and 3 (1001 and 2459 atoms) on lists.
     ws = replica_sweep(n_rep=64, steps=10)
     with GpuBatch([w.physics for w in ws], [w.state for w in ws]) as b:
+        assert b.get_tuning()["pair_mode"] == 1
         obs = b.step(20, graph_segment=10, observe=True)
-        fin = {r: b.download(r) for r in (1, 2)}
+        fin = {r: b.download(r) for r in (1, 2, 3)}
     assert obs.shape == (20, 64, 6) and np.isfinite(obs).all()
     targets = np.array([w.physics.thermostat_temperature for w in ws])
     assert np.corrcoef(obs[0, :, 3], targets)[0, 1] > 0.99
-    for r in (1, 2):
+    for r in (1, 2, 3):
         ref, obs_ref = O.run(ws[r].physics, ws[r].state, 20, nthreads=8)
         print(f"[gpu-batch] replica {r} n={ws[r].state.n} dims={ws[r].physics.grid_dims()}", flush=True)
         np.testing.assert_allclose(obs[:, r, 5], obs_ref[:, 5], rtol=1e-9)
         np.testing.assert_allclose(fin[r].x, ref.x, rtol=0, atol=1e-8)
+
+
+def test_batch_list_overflow_replays():
+    # Two replicas with 400 atoms packed inside one list-grid cell: the first
+    # tick overflows the buckets and the lists, so the call is replayed from
+    # its snapshot with doubled capacities.  One tick against the oracle.
+    from genesis_amd.batch import GpuBatch
+    from genesis_amd.inputs import State, plain_lj
+    from oracle import oracle as O
+    from tests.helpers import force_tolerance_ok
+    ph0 = plain_lj(n_side=4, steps=1).physics
+    L = 6.0 * 1.12 * ph0.force_rc() * 1.001
+    ph = Physics(**{**ph0.__dict__, "box": (L, L, L)})
+    edge = L / 6
+    sts = []
+    for seed in (17, 18):
+        rng = np.random.default_rng(seed)
+        centre = -0.5 * L + 1.5 * edge
+        blob = centre + rng.uniform(-0.3 * edge, 0.3 * edge, (400, 3))
+        far = rng.uniform(-0.5 * L, 0.5 * L, (200, 3))
+        far = far[np.linalg.norm(far - centre, axis=1) > 2.5 * edge][:150]
+        p = np.concatenate([blob, far])
+        z = np.zeros(len(p))
+        sts.append(State(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), z, z.copy(), z.copy(),
+                         np.zeros(len(p), dtype=np.uint8), z.copy()))
+    print("[gpu-batch] crowded cells, sizes", [s.n for s in sts], flush=True)
+    with GpuBatch([ph, ph], sts) as b:
+        b.set_tuning(1, 4, skin=0.12 * ph.force_rc())
+        obs = b.step(1, observe=True)
+        got = [b.download(r) for r in range(2)]
+        tun = b.get_tuning()
+    assert tun["pair_mode"] == 1 and tun["list_replays"] >= 1
+    for r, st in enumerate(sts):
+        ref, obs_ref = O.run(ph, st, 1)
+        for a in ("x", "y", "z"):
+            assert np.array_equal(getattr(got[r], a), getattr(ref, a)), a
+        ref_f = O.forces(ph, State(ref.x, ref.y, ref.z, ref.vx, ref.vy, ref.vz, st.species, st.charge))
+        m = np.asarray(ph.mass)[st.species]
+        ok, worst = force_tolerance_ok(got[r].ax, got[r].ay, got[r].az, ref_f, m, 1e-10)
+        assert ok, worst
+        assert obs[0, r, 1] == pytest.approx(obs_ref[0, 1], rel=1e-10)
