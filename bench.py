@@ -193,8 +193,9 @@ def main() -> int:
 def batched_leg(ticks: int = 200, segment: int = 100) -> dict:
     """Temperature sweep: 64 replicas of log-uniform sizes in [512, 16384]
     (20 K .. 120 K), stepped (a) as one batch (one launch per phase, hipGraph
-    segments of 100 ticks, autotuned cell size) and (b) one universe after
-    another through gs_step (the per-replica-launch plain path)."""
+    segments of up to 100 ticks, autotuned pair path: Verlet lists or cell
+    tiles) and (b) one universe after another through gs_step with its own
+    default setting (the per-replica-launch plain path)."""
     from genesis_amd.batch import GpuBatch
     from genesis_amd.inputs import replica_sweep
     from genesis_amd.universe import GpuUniverse
@@ -203,9 +204,11 @@ def batched_leg(ticks: int = 200, segment: int = 100) -> dict:
     with GpuBatch([w.physics for w in ws], [w.state for w in ws]) as b:
         tune = b.autotune(20)
         b.step(segment, graph_segment=segment)
+        builds0 = b.get_tuning()["list_builds"]
         t0 = time.perf_counter()
         b.step(ticks, graph_segment=segment)
         tb = time.perf_counter() - t0
+        btun = b.get_tuning()
     us = [GpuUniverse(w.physics, w.state) for w in ws]
     for u in us:
         u.step(20, observe=False)
@@ -219,7 +222,11 @@ def batched_leg(ticks: int = 200, segment: int = 100) -> dict:
     log(f"batched sweep: {vb:.4e} atom-steps/s vs per-replica {vp:.4e} ({vb / vp:.2f}x), {total} atoms")
     return {"workload": f"temperature sweep, 64 replicas, {total} atoms, sizes log-uniform in [512, 16384]",
             "batched_atom_steps_per_s": vb, "per_replica_atom_steps_per_s": vp, "speedup": vb / vp,
-            "ticks": ticks, "graph_segment": segment, "autotuned_cell_scale": tune["cell_scale"]}
+            "ticks": ticks, "graph_segment": segment,
+            "tuning": {"pair_mode": btun["pair_mode"], "lanes_per_atom": btun["lanes_per_atom"],
+                       "skin": btun["skin"], "cell_scale": tune["cell_scale"],
+                       "ticks_with_list_builds": btun["list_builds"] - builds0,
+                       "list_replays": btun["list_replays"]}}
 
 
 def adjoint_leg(steps: int = 1000, ckpt: int = 50) -> dict:

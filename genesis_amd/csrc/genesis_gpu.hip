@@ -693,6 +693,10 @@ __global__ __launch_bounds__(NW * WAVE) void k_pair(KParams Pk, const uint32_t* 
 // list-grid cell and of each atom, and each replica's box (w = 1 for a
 // replica on Verlet lists, 0 for one on brute-force pairs) with its
 // reciprocal.
+// box.w is the replica's half skin (0: brute-force pairs), ibox.w its list
+// cutoff squared; rflags holds four words per replica (list mode): [0] an
+// atom is within reach of skin/2, [1] bin this tick, [2] binned for a build
+// next tick, [3] build this tick.
 struct BatchTabs {
     const KParams* PL;
     const uint32_t* lcell_off;
@@ -701,6 +705,7 @@ struct BatchTabs {
     const double4* box;
     const double4* ibox;
     uint32_t ncell;
+    uint32_t* rflags;
 };
 
 // Workgroup of a batched launch over atoms: its replica and first atom
@@ -710,18 +715,24 @@ struct BlockRef {
     uint32_t first;
 };
 
-constexpr int BRUTE_WAVES = 4;  // home atoms of a brute-force workgroup, one per wave
+constexpr int BRUTE_WAVES = 16;    // home atoms of a brute-force workgroup, one per wave
+constexpr int BRUTE_TILE = 1024;   // candidate positions staged in LDS per chunk
 
 // Small-box pair kernel (any axis with < 3 cells): every ordered pair, the
 // reference's min_image, and in GS_GRID_REFERENCE mode the multiplicity the
 // reference grid gives a pair on an axis with exactly 2 cells (each wrapping
 // offset reaches the other cell, src/physics/grid.rs:134-151,275-295): x2 per
-// such axis on which the two atoms' cells differ.  One wave per home atom:
-// the lanes sweep the universe's atoms 64 at a time with the distance test,
-// a ballot compacts the in-range candidates into a per-wave LDS ring, and
-// full 64-lane batches evaluate the law, so the expensive part runs on
-// in-range pairs only.  The atom's sums are wave reductions (fixed order).
-// vpart[workgroup] = half the pair energy of its home atoms.
+// such axis on which the two atoms' cells differ.  One wave per home atom,
+// sixteen per workgroup: the universe's positions are staged in LDS chunks
+// shared by the sixteen, the lanes sweep a chunk 64 candidates at a time
+// with the distance test, a ballot compacts the in-range candidates into a
+// per-wave LDS ring, and full 64-lane batches evaluate the law, so the
+// expensive part runs on in-range pairs only.  The sweep's test takes the
+// image with a product by 1/L (no divide) against a cutoff widened by 1e-12
+// relative, and every compacted pair is tested again with the reference's
+// min_image, so the pair set is exactly the reference's.  The atom's sums
+// are wave reductions (fixed order).  vpart[workgroup] = half the pair
+// energy of its home atoms.
 // BATCH: workgroup b takes home atoms hb[b].first .. + BRUTE_WAVES - 1 of
 // replica hb[b].rep (clipped at its end), with that replica's parameters and
 // atom range.
@@ -731,8 +742,11 @@ __global__ __launch_bounds__(BRUTE_WAVES * WAVE) void k_pair_brute(
     const PairCoef* __restrict__ coef, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
     double* __restrict__ pe_atom, double* __restrict__ s_abs, double* __restrict__ vpart,
     const KParams* __restrict__ Ptab, const BlockRef* __restrict__ hb, const uint32_t* __restrict__ atom_off) {
+    constexpr int NT = BRUTE_WAVES * WAVE;
     __shared__ PairCoef sc[GS_NSPECIES * GS_NSPECIES];
-    __shared__ uint32_t queue[BRUTE_WAVES][QCAP];
+    __shared__ double4 tile[BRUTE_TILE];
+    __shared__ uint8_t ttype[BRUTE_TILE];
+    __shared__ uint16_t queue[BRUTE_WAVES][QCAP];
     __shared__ double wv[BRUTE_WAVES];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     uint32_t rep = 0, lo = 0, hi = (uint32_t)Pk.n, first = blockIdx.x * BRUTE_WAVES;
@@ -744,69 +758,86 @@ __global__ __launch_bounds__(BRUTE_WAVES * WAVE) void k_pair_brute(
         first = br.first;
     }
     const KParams& P = BATCH ? Ptab[rep] : Pk;
-    for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += BRUTE_WAVES * WAVE) sc[t] = coef[t];
-    __syncthreads();
+    for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += NT) sc[t] = coef[t];
     const uint32_t i = first + (uint32_t)w;
-    double vl = 0.0;
-    if (i < hi) {  // uniform over the wave
-        const double4 pi = pos4[i];
-        const int ti = type[i];
-        const double kqi = P.ke * pi.w;
-        const bool refm = (P.flags & F_REF_MULT) != 0;
-        const uint32_t ki = refm ? cell_key(P, pi.x, pi.y, pi.z) : 0u;
-        const uint32_t cxi = ki % P.dims[0], cyi = (ki / P.dims[0]) % P.dims[1], czi = ki / (P.dims[0] * P.dims[1]);
-        double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, sa = 0.0;
-        uint32_t qn = 0, qh = 0;
-        auto batch = [&](uint32_t cnt) {
-            if ((uint32_t)lane < cnt) {
-                const uint32_t j = queue[w][(qh + lane) & (QCAP - 1)];
-                const double4 pj = pos4[j];
-                const double dx = min_image1(pi.x - pj.x, P.box[0]);
-                const double dy = min_image1(pi.y - pj.y, P.box[1]);
-                const double dz = min_image1(pi.z - pj.z, P.box[2]);
-                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                double mult = 1.0;
-                if (refm) {
-                    const uint32_t kj = cell_key(P, pj.x, pj.y, pj.z);
-                    const uint32_t cxj = kj % P.dims[0], cyj = (kj / P.dims[0]) % P.dims[1],
-                                   czj = kj / (P.dims[0] * P.dims[1]);
-                    if (P.dims[0] == 2 && cxi != cxj) mult *= 2.0;
-                    if (P.dims[1] == 2 && cyi != cyj) mult *= 2.0;
-                    if (P.dims[2] == 2 && czi != czj) mult *= 2.0;
-                }
-                const double inv_d = rsqrt_refined(r2);
-                const double d = r2 * inv_d;
-                double fmag, vv;
-                pair_law<COUL, false>(P, sc[ti * GS_NSPECIES + type[j]], d, inv_d, kqi * pj.w, fmag, vv);
-                fmag *= mult;
-                vv *= mult;
-                const double fs = fmag * inv_d;
-                fx = fma(fs, dx, fx);
-                fy = fma(fs, dy, fy);
-                fz = fma(fs, dz, fz);
-                pe += vv;
-                if (EXTRAS) sa += fabs(fmag);
+    const bool act = i < hi;  // uniform over the wave
+    const double4 pi = act ? pos4[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const int ti = act ? type[i] : 0;
+    const double kqi = P.ke * pi.w;
+    const bool refm = (P.flags & F_REF_MULT) != 0;
+    const uint32_t ki = refm && act ? cell_key(P, pi.x, pi.y, pi.z) : 0u;
+    const uint32_t cxi = ki % P.dims[0], cyi = (ki / P.dims[0]) % P.dims[1], czi = ki / (P.dims[0] * P.dims[1]);
+    const double iLx = 1.0 / P.box[0], iLy = 1.0 / P.box[1], iLz = 1.0 / P.box[2];
+    const double rc2w = P.rc2 * (1.0 + 1e-12);
+    double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, sa = 0.0;
+    uint32_t qn = 0, qh = 0;
+    auto batch = [&](uint32_t cnt) {
+        if ((uint32_t)lane < cnt) {
+            const uint32_t t = queue[w][(qh + lane) & (QCAP - 1)];
+            const double4 pj = tile[t];
+            const double dx = min_image1(pi.x - pj.x, P.box[0]);
+            const double dy = min_image1(pi.y - pj.y, P.box[1]);
+            const double dz = min_image1(pi.z - pj.z, P.box[2]);
+            const double r2x = fma(dz, dz, fma(dy, dy, dx * dx));
+            const bool ok = r2x < P.rc2 && r2x > P.eps2;
+            const double r2 = ok ? r2x : 1.0;
+            double mult = ok ? 1.0 : 0.0;
+            if (refm) {
+                const uint32_t kj = cell_key(P, pj.x, pj.y, pj.z);
+                const uint32_t cxj = kj % P.dims[0], cyj = (kj / P.dims[0]) % P.dims[1],
+                               czj = kj / (P.dims[0] * P.dims[1]);
+                if (P.dims[0] == 2 && cxi != cxj) mult *= 2.0;
+                if (P.dims[1] == 2 && cyi != cyj) mult *= 2.0;
+                if (P.dims[2] == 2 && czi != czj) mult *= 2.0;
             }
-            qh += cnt;
-        };
-        for (uint32_t base = lo; base < hi; base += 64) {
-            const uint32_t j = base + (uint32_t)lane;
-            bool in = false;
-            if (j < hi && j != i) {
-                const double4 pj = pos4[j];
-                const double dx = min_image1(pi.x - pj.x, P.box[0]);
-                const double dy = min_image1(pi.y - pj.y, P.box[1]);
-                const double dz = min_image1(pi.z - pj.z, P.box[2]);
-                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                in = r2 < P.rc2 && r2 > P.eps2;
-            }
-            const uint64_t m = __ballot(in);
-            if (in) queue[w][(qn + mask_rank(m)) & (QCAP - 1)] = j;
-            qn += (uint32_t)__popcll(m);
-            __builtin_amdgcn_wave_barrier();
-            if (qn - qh >= 64) batch(64);
+            const double inv_d = rsqrt_refined(r2);
+            const double d = r2 * inv_d;
+            double fmag, vv;
+            pair_law<COUL, false>(P, sc[ti * GS_NSPECIES + ttype[t]], d, inv_d, kqi * pj.w, fmag, vv);
+            fmag *= mult;
+            vv *= mult;
+            const double fs = fmag * inv_d;
+            fx = fma(fs, dx, fx);
+            fy = fma(fs, dy, fy);
+            fz = fma(fs, dz, fz);
+            pe += vv;
+            if (EXTRAS) sa += fabs(fmag);
         }
-        if (qn > qh) batch(qn - qh);
+        qh += cnt;
+    };
+    for (uint32_t c0 = lo; c0 < hi; c0 += BRUTE_TILE) {
+        const uint32_t len = min((uint32_t)BRUTE_TILE, hi - c0);
+        __syncthreads();
+        for (uint32_t t = tid; t < len; t += NT) {
+            tile[t] = pos4[c0 + t];
+            ttype[t] = type[c0 + t];
+        }
+        __syncthreads();
+        if (act) {
+            for (uint32_t base = 0; base < len; base += 64) {
+                const uint32_t t = base + (uint32_t)lane;
+                bool in = false;
+                if (t < len && c0 + t != i) {
+                    const double4 pj = tile[t];
+                    double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    dx = fma(-P.box[0], rint(dx * iLx), dx);
+                    dy = fma(-P.box[1], rint(dy * iLy), dy);
+                    dz = fma(-P.box[2], rint(dz * iLz), dz);
+                    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                    in = r2 < rc2w;
+                }
+                const uint64_t m = __ballot(in);
+                if (in) queue[w][(qn + mask_rank(m)) & (QCAP - 1)] = (uint16_t)t;
+                qn += (uint32_t)__popcll(m);
+                __builtin_amdgcn_wave_barrier();
+                if (qn - qh >= 64) batch(64);
+            }
+            // the queue refers to this chunk's tile: drain it
+            if (qn > qh) batch(qn - qh);
+        }
+    }
+    double vl = 0.0;
+    if (act) {
         fx = wave_sum(fx);
         fy = wave_sum(fy);
         fz = wave_sum(fz);
@@ -1287,6 +1318,14 @@ __device__ __forceinline__ void build_cells(
     if (b0 == 0 && tid == 0) flags[3] += 1u;  // rebuilds done on the device
     const uint32_t ncell = BATCH ? bt.ncell : PL.ncell;
     for (uint32_t c = b0; c < ncell; c += stride) {
+    // BATCH: a cell of a replica that does not build now is skipped (a
+    // full build, flags[0], takes every cell); cut2 is its replica's
+    double cutc = cut2;
+    if (BATCH) {
+        const uint32_t rep = bt.lcell_rep[c];
+        if (flags[0] == 0u && bt.rflags[4 * rep + 3] == 0u) continue;
+        cutc = bt.ibox[rep].w;
+    }
     __syncthreads();  // the previous cell's LDS readers are done
     const uint32_t nh = min(bcount[c], bcap);
     if (nh == 0) {
@@ -1415,7 +1454,7 @@ __device__ __forceinline__ void build_cells(
                     e[q] = LOCAL ? ((uint32_t)ttype[jc] << CELL_IDX_BITS) | (c0 + jc) : ((uint32_t)ttype[jc] << 28) | jg;
                     const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                    in[q] = j < len && r2 < cut2 && jg != gi;
+                    in[q] = j < len && r2 < cutc && jg != gi;
                 }
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
@@ -1609,8 +1648,12 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     const uint32_t g = chunk * BLOCK + tid;
     const uint32_t i = g >> LOGT;
     // a tick that rebuilt leaves the buckets empty for the next binning
+    // (BATCH: those of the replicas that built, flags[11] if any did)
     if (flags[0] != 0u)
         for (uint32_t b = blockIdx.x * BLOCK + tid; b < nbucket; b += gridDim.x * BLOCK) bcount[b] = 0u;
+    else if (BATCH && flags[11] != 0u)
+        for (uint32_t b = blockIdx.x * BLOCK + tid; b < nbucket; b += gridDim.x * BLOCK)
+            if (bt.rflags[4 * bt.lcell_rep[b] + 3] != 0u) bcount[b] = 0u;
     const uint32_t s = g & (T - 1);
     double fx = 0.0, fy = 0.0, fz = 0.0, vl = 0.0;
     int ti = 0;
