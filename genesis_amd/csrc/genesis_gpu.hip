@@ -1083,7 +1083,8 @@ __global__ __launch_bounds__(BLOCK) void k_cells_entity(KParams P, const uint32_
 // in which the atomics handed out the slots.
 //
 // flags, reset at every call: [0] rebuild this tick, [1] a list overflowed
-//        its capacity, [2] a stencil held more candidates than LDS_IDX_CAP,
+//        its capacity, [2] a stencil held more candidates than the build
+//        orders in LDS (LDS_IDX_CAP, LDS_IDX_CAP_LOCAL),
 //        [3] rebuilds done by the device, [4] a tick needed a rebuild that
 //        its binning had not been requested for (predictive binning),
 //        [5] last-workgroup counter of the pair kernel, [6] a cell bucket
@@ -1104,7 +1105,8 @@ __global__ __launch_bounds__(BLOCK) void k_cells_entity(KParams P, const uint32_
 constexpr uint32_t NBR_IDX_MASK = 0x0FFFFFFFu;  // entry = (species << 28) | sorted index
 constexpr int LIST_FLAGS = 16;                  // words of the list flags
 constexpr int LIST_FLAGS_RESET = 8;             // the first ones are reset at every call
-constexpr int LDS_IDX_CAP = 2048;               // stencil candidates held (and ordered) in LDS
+constexpr int LDS_IDX_CAP = 4096;               // stencil candidates held (and ordered) in LDS
+constexpr int LDS_IDX_CAP_LOCAL = 2048;         // the same for the cell-staged lists (11-bit positions)
 constexpr int MAX_LOGT = 4;                     // up to 16 lanes per atom
 constexpr uint32_t HOME_CAP = 256;              // home atoms whose list lengths the build keeps in LDS
 
@@ -1116,7 +1118,7 @@ constexpr int CELL_THREADS = 384;    // workgroup of the cell-staged pair kernel
 constexpr int STAGE_CAP = 1280;      // candidate positions staged in LDS per chunk
 constexpr uint32_t CELL_IDX_BITS = 11;
 constexpr uint32_t CELL_IDX_MASK = (1u << CELL_IDX_BITS) - 1u;
-static_assert(LDS_IDX_CAP <= (1 << CELL_IDX_BITS), "a candidate position fits the entry");
+static_assert(LDS_IDX_CAP_LOCAL <= (1 << CELL_IDX_BITS), "a candidate position fits the entry");
 static_assert(GS_NSPECIES <= (1 << (16 - CELL_IDX_BITS)), "a species fits the entry");
 
 // Streaming 16-byte load with a non-temporal hint (global_load ... nt).
@@ -1307,12 +1309,13 @@ __device__ __forceinline__ void build_cells(
     // position tile, which is filled only after they have been ordered
     __shared__ double4 tile[BUILD_TILE];
     __shared__ uint8_t ttype[BUILD_TILE];
-    __shared__ uint32_t ord[LDS_IDX_CAP];
+    constexpr int IDX_CAP = LOCAL ? LDS_IDX_CAP_LOCAL : LDS_IDX_CAP;
+    __shared__ uint32_t ord[IDX_CAP];
     __shared__ uint32_t hcnt[HOME_CAP];  // running list lengths between chunks
     __shared__ uint32_t seg_base[27];
     __shared__ uint32_t seg_pref[28];
     __shared__ double seg_sh[27][3];
-    static_assert(LDS_IDX_CAP * sizeof(uint32_t) <= BUILD_TILE * sizeof(double4), "index staging fits the tile");
+    static_assert(IDX_CAP * sizeof(uint32_t) <= BUILD_TILE * sizeof(double4), "index staging fits the tile");
     uint32_t* cand = reinterpret_cast<uint32_t*>(tile);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (b0 == 0 && tid == 0) flags[3] += 1u;  // rebuilds done on the device
@@ -1359,7 +1362,7 @@ __device__ __forceinline__ void build_cells(
     }
     __syncthreads();
     const uint32_t ncand = seg_pref[27];
-    if (ncand > (uint32_t)LDS_IDX_CAP) {
+    if (ncand > (uint32_t)IDX_CAP) {
         if (tid == 0) atomicOr(&flags[2], 1u);
         continue;
     }
@@ -1405,7 +1408,7 @@ __device__ __forceinline__ void build_cells(
     uint16_t* reg = LOCAL ? nbr16 + (size_t)c * region : nullptr;
     if (LOCAL) {
         __syncthreads();
-        for (uint32_t t = tid; t < ncand; t += NT) cand_store[(size_t)c * LDS_IDX_CAP + t] = ord[t];
+        for (uint32_t t = tid; t < ncand; t += NT) cand_store[(size_t)c * LDS_IDX_CAP_LOCAL + t] = ord[t];
         if (tid == 0) cinfo[c] = make_uint4(ncand, hs, nh, 0u);
     }
     const int nchunks = (int)((ncand + BUILD_TILE - 1) / BUILD_TILE);
@@ -1919,7 +1922,7 @@ __global__ __launch_bounds__(CELL_THREADS) __attribute__((amdgpu_waves_per_eu(5)
     if (bad != 0u || c >= ncell) ci = make_uint4(0u, 0u, 0u, 0u);
     const uint32_t ncand = ci.x, nh = ci.z;
     const bool prop = nh <= (uint32_t)CELL_THREADS / 2;  // proportional lanes, one pass
-    const uint32_t* cand = cand_store + (size_t)cc * LDS_IDX_CAP;
+    const uint32_t* cand = cand_store + (size_t)cc * LDS_IDX_CAP_LOCAL;
     const uint16_t* reg = nbr16 + (size_t)cc * region;
     const uint2* hr0 = hrec + (size_t)cc * bcap;
     const int nchunks = (int)((ncand + STAGE_CAP - 1) / STAGE_CAP);
@@ -2752,7 +2755,7 @@ int list_alloc(gs_universe* u, uint32_t cap, uint32_t bcap_min = 0) {
             u->cand_store = nullptr;
             u->cinfo = nullptr;
             u->cell_have = 0;
-            if ((rc = dalloc(&u->cand_store, (size_t)cells * LDS_IDX_CAP)) || (rc = dalloc(&u->cinfo, cells)))
+            if ((rc = dalloc(&u->cand_store, (size_t)cells * LDS_IDX_CAP_LOCAL)) || (rc = dalloc(&u->cinfo, cells)))
                 return rc;
             // a cell never built has no candidates
             HIPCHK(hipMemsetAsync(u->cinfo, 0, sizeof(uint4) * cells, u->stream));
