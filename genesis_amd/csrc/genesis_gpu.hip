@@ -171,6 +171,89 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ count, uin
     if (tid == 0) start[ncell] = carry;
 }
 
+// Multi-block exclusive scan of a large cell histogram (more than one
+// k_scan pass of 16384 cells): k_scan_part scans each 16384-cell chunk in
+// its own workgroup (and resets the histogram), k_scan of the chunk totals
+// gives each chunk's offset, and k_scan_fix adds it.  The cell starts are
+// the same integers as k_scan's.
+constexpr uint32_t SCAN_CHUNK = 1024u * 16u;
+__global__ __launch_bounds__(1024) void k_scan_part(uint32_t* __restrict__ count, uint32_t* __restrict__ start,
+                                                    uint32_t ncell, uint32_t* __restrict__ bsum) {
+    __shared__ uint32_t wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t i0 = blockIdx.x * SCAN_CHUNK + (uint32_t)tid * 16u;
+    uint32_t v[16];
+    uint32_t s = 0;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const uint32_t idx = i0 + e;
+        v[e] = idx < ncell ? count[idx] : 0u;
+        s += v[e];
+    }
+    uint32_t incl = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    if (w == 0) {
+        const uint32_t x = lane < 16 ? wsum[lane] : 0u;
+        uint32_t xi = x;
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            const uint32_t t = __shfl_up(xi, off, 64);
+            if (lane >= off) xi += t;
+        }
+        if (lane < 16) wsum[lane] = xi - x;
+        if (lane == 15) bsum[blockIdx.x] = xi;
+    }
+    __syncthreads();
+    uint32_t excl = wsum[w] + incl - s;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const uint32_t idx = i0 + e;
+        if (idx < ncell) {
+            start[idx] = excl;
+            count[idx] = 0u;
+        }
+        excl += v[e];
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_fix(uint32_t* __restrict__ start, uint32_t ncell,
+                                                   const uint32_t* __restrict__ boff) {
+    const uint32_t off = boff[blockIdx.x];
+    const uint32_t i0 = blockIdx.x * SCAN_CHUNK + threadIdx.x * 16u;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const uint32_t idx = i0 + e;
+        if (idx < ncell) start[idx] += off;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) start[ncell] = boff[gridDim.x];
+}
+
+// Scratch words the multi-block scan of ncell cells needs (chunk totals and
+// their offsets).
+inline size_t scan_tmp_words(uint32_t ncell) {
+    const size_t nb = (ncell + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    return 2 * (nb + 1);
+}
+
+// Exclusive scan of count[0..ncell) into start[0..ncell] (start[ncell] =
+// total) with count reset to zero; tmp: scan_tmp_words(ncell) words.
+inline void launch_scan(hipStream_t st, uint32_t* count, uint32_t* start, uint32_t ncell, uint32_t* tmp) {
+    if (ncell <= SCAN_CHUNK || !tmp) {
+        k_scan<<<1, 1024, 0, st>>>(count, start, ncell);
+        return;
+    }
+    const uint32_t nb = (ncell + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    k_scan_part<<<nb, 1024, 0, st>>>(count, start, ncell, tmp);
+    k_scan<<<1, 1024, 0, st>>>(tmp, tmp + nb + 1, nb);
+    k_scan_fix<<<nb, 1024, 0, st>>>(start, ncell, tmp + nb + 1);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_scatter(int n, const uint32_t* __restrict__ key,
                                                    const uint32_t* __restrict__ ticket,
                                                    const uint32_t* __restrict__ start, uint32_t* __restrict__ perm) {
@@ -1084,7 +1167,7 @@ __global__ __launch_bounds__(BLOCK) void k_cells_entity(KParams P, const uint32_
 //
 // flags, reset at every call: [0] rebuild this tick, [1] a list overflowed
 //        its capacity, [2] a stencil held more candidates than the build
-//        orders in LDS (LDS_IDX_CAP, LDS_IDX_CAP_LOCAL),
+//        orders in LDS (LDS_IDX_CAP, _LOCAL, _BATCH),
 //        [3] rebuilds done by the device, [4] a tick needed a rebuild that
 //        its binning had not been requested for (predictive binning),
 //        [5] last-workgroup counter of the pair kernel, [6] a cell bucket
@@ -1105,8 +1188,9 @@ __global__ __launch_bounds__(BLOCK) void k_cells_entity(KParams P, const uint32_
 constexpr uint32_t NBR_IDX_MASK = 0x0FFFFFFFu;  // entry = (species << 28) | sorted index
 constexpr int LIST_FLAGS = 16;                  // words of the list flags
 constexpr int LIST_FLAGS_RESET = 8;             // the first ones are reset at every call
-constexpr int LDS_IDX_CAP = 4096;               // stencil candidates held (and ordered) in LDS
+constexpr int LDS_IDX_CAP = 2048;               // stencil candidates held (and ordered) in LDS
 constexpr int LDS_IDX_CAP_LOCAL = 2048;         // the same for the cell-staged lists (11-bit positions)
+constexpr int LDS_IDX_CAP_BATCH = 4096;         // the same for a batch's lists (a build launch of its own)
 constexpr int MAX_LOGT = 4;                     // up to 16 lanes per atom
 constexpr uint32_t HOME_CAP = 256;              // home atoms whose list lengths the build keeps in LDS
 
@@ -1309,7 +1393,7 @@ __device__ __forceinline__ void build_cells(
     // position tile, which is filled only after they have been ordered
     __shared__ double4 tile[BUILD_TILE];
     __shared__ uint8_t ttype[BUILD_TILE];
-    constexpr int IDX_CAP = LOCAL ? LDS_IDX_CAP_LOCAL : LDS_IDX_CAP;
+    constexpr int IDX_CAP = LOCAL ? LDS_IDX_CAP_LOCAL : (BATCH ? LDS_IDX_CAP_BATCH : LDS_IDX_CAP);
     __shared__ uint32_t ord[IDX_CAP];
     __shared__ uint32_t hcnt[HOME_CAP];  // running list lengths between chunks
     __shared__ uint32_t seg_base[27];
@@ -2376,6 +2460,8 @@ struct gs_universe {
     double* a[3] = {nullptr, nullptr, nullptr};
     // cell list
     uint32_t *key = nullptr, *ticket = nullptr, *count = nullptr, *cstart = nullptr, *perm = nullptr;
+    uint32_t* scan_tmp = nullptr;  // multi-block scan scratch (scan_tmp_words of the largest grid)
+    size_t scan_tmp_cap = 0;
     // coefficients, reductions, scalars
     PairCoef* coef = nullptr;
     double* vpart = nullptr;
@@ -2393,6 +2479,7 @@ struct gs_universe {
     double4* l_pts = nullptr;
     uint8_t* l_type = nullptr;
     uint32_t *l_key = nullptr, *l_ticket = nullptr, *l_perm = nullptr, *l_count = nullptr, *l_start = nullptr;
+    uint32_t* l_id = nullptr;  // entity ids in lens-grid order
     uint32_t l_ncell_cap = 0;
     uint32_t* l_parent = nullptr;
     uint32_t* l_size = nullptr;
@@ -2476,6 +2563,8 @@ namespace {
 // observed bonds (gs_bonds.inc)
 int bond_force_pass(gs_universe* u, bool accumulate);
 int bond_observe_pass(gs_universe* u, uint64_t now);
+int bond_call_begin(gs_universe* u);
+int bond_call_restore(gs_universe* u, bool existed);
 int bond_refresh_species(gs_universe* u, const uint8_t* species);
 
 int validate_params(const gs_params* p, int64_t n) {
@@ -2603,6 +2692,19 @@ int dalloc(T** p, size_t count) {
     return GS_OK;
 }
 
+// Scan scratch for grids of up to ncell cells.
+int scan_reserve(uint32_t** tmp, size_t* cap, uint32_t ncell) {
+    const size_t need = scan_tmp_words(ncell);
+    if (*cap >= need) return GS_OK;
+    (void)hipFree(*tmp);
+    *tmp = nullptr;
+    *cap = 0;
+    int rc;
+    if ((rc = dalloc(tmp, need))) return rc;
+    *cap = need;
+    return GS_OK;
+}
+
 int check_state(gs_universe* u) {
     if (!u) return fail(GS_E_INVALID, "universe handle is NULL");
     if (!u->have_state) return fail(GS_E_STATE, "no state uploaded");
@@ -2623,7 +2725,7 @@ int rebuild(gs_universe* u, bool kick_drift, const KParams& P) {
         k_integrate_key<false><<<nblocks(n), BLOCK, 0, u->stream>>>(P, u->pos4[c], u->v[c][0], u->v[c][1],
                                                                    u->v[c][2], u->a[0], u->a[1], u->a[2], u->scal,
                                                                    u->key, u->ticket, u->count);
-    k_scan<<<1, 1024, 0, u->stream>>>(u->count, u->cstart, P.ncell);
+    launch_scan(u->stream, u->count, u->cstart, P.ncell, u->scan_tmp_cap >= scan_tmp_words(P.ncell) ? u->scan_tmp : nullptr);
     k_scatter<<<nblocks(n), BLOCK, 0, u->stream>>>(n, u->key, u->ticket, u->cstart, u->perm);
     k_sort_gather<<<nblocks(P.ncell, BLOCK / WAVE), BLOCK, 0, u->stream>>>(
         P.ncell, u->cstart, u->perm, u->pos4[c], u->v[c][0], u->v[c][1], u->v[c][2], u->type[c], u->id[c],
@@ -3054,7 +3156,8 @@ int gs_create(const gs_params* params, int64_t n_atoms, gs_universe** out) {
         (rc = dalloc(&u->coef, coef.size())) || (rc = dalloc(&u->vpart, (size_t)u->npart)) ||
         (rc = dalloc(&u->kpart, (size_t)u->npart)) || (rc = dalloc(&u->counter, 4)) ||
         (rc = dalloc(&u->scal, 4)) || (rc = dalloc(&u->pe, n)) || (rc = dalloc(&u->sabs, n)) ||
-        (rc = dalloc(&u->stage, 9 * n)) || (rc = dalloc(&u->stage_u, n)))
+        (rc = dalloc(&u->stage, 9 * n)) || (rc = dalloc(&u->stage_u, n)) ||
+        (rc = scan_reserve(&u->scan_tmp, &u->scan_tmp_cap, P.ncell)))
         return bail(rc);
     if (hipMemsetAsync(u->count, 0, sizeof(uint32_t) * P.ncell, u->stream) != hipSuccess ||
         hipMemsetAsync(u->counter, 0, sizeof(uint32_t) * 4, u->stream) != hipSuccess ||
@@ -3084,6 +3187,7 @@ int gs_destroy(gs_universe* u) {
     (void)hipFree(u->perm);
     (void)hipFree(u->count);
     (void)hipFree(u->cstart);
+    (void)hipFree(u->scan_tmp);
     (void)hipFree(u->coef);
     (void)hipFree(u->vpart);
     (void)hipFree(u->kpart);
@@ -3095,6 +3199,7 @@ int gs_destroy(gs_universe* u) {
     (void)hipFree(u->stage_u);
     (void)hipFree(u->obs);
     (void)hipFree(u->l_pts);
+    (void)hipFree(u->l_id);
     (void)hipFree(u->l_type);
     (void)hipFree(u->l_key);
     (void)hipFree(u->l_ticket);
@@ -3359,9 +3464,12 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
         u->ev.push_back(e);
     }
     const int cur0 = u->cur;
+    const int64_t tick0 = u->tick;
+    const bool bonds0 = u->bonds != nullptr;
     if (list) {
         if ((rc = list_alloc(u, u->nbr_cap ? u->nbr_cap : 384))) return rc;
         if ((rc = snapshot(u, false))) return rc;
+        if ((rc = bond_call_begin(u))) return rc;
     }
     for (int attempt = 0;; ++attempt) {
         if (list) HIPCHK(hipMemsetAsync(u->flags, 0, LIST_FLAGS_RESET * sizeof(uint32_t), u->stream));
@@ -3383,7 +3491,9 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
             if (attempt > 8) return fail(GS_E_OVERFLOW, "neighbour list could not be made valid");
             u->list_replays += 1;
             u->cur = cur0;
+            u->tick = tick0;
             if ((rc = snapshot(u, true))) return rc;
+            if ((rc = bond_call_restore(u, bonds0))) return rc;
             u->list_valid = false;
             if (fl[4]) u->prebin = false;
             if (fl[2]) {
@@ -3569,6 +3679,7 @@ int lens_alloc(gs_universe* u, uint32_t ncell) {
     const size_t n = (size_t)std::max(1, u->n);
     if (!u->l_pts) {
         if ((rc = dalloc(&u->l_pts, n)) || (rc = dalloc(&u->l_type, n)) || (rc = dalloc(&u->l_key, n)) ||
+            (rc = dalloc(&u->l_id, n)) ||
             (rc = dalloc(&u->l_ticket, n)) || (rc = dalloc(&u->l_perm, n)) || (rc = dalloc(&u->l_parent, n)) ||
             (rc = dalloc(&u->l_size, n)) || (rc = dalloc(&u->l_u64, 8)) || (rc = dalloc(&u->l_sigma, GS_NSPECIES)))
             return rc;
@@ -3582,6 +3693,7 @@ int lens_alloc(gs_universe* u, uint32_t ncell) {
         if ((rc = dalloc(&u->l_count, ncell)) || (rc = dalloc(&u->l_start, (size_t)ncell + 1))) return rc;
         u->l_ncell_cap = ncell;
     }
+    if ((rc = scan_reserve(&u->scan_tmp, &u->scan_tmp_cap, ncell))) return rc;
     HIPCHK(hipMemsetAsync(u->l_count, 0, sizeof(uint32_t) * ncell, u->stream));
     return GS_OK;
 }
@@ -3589,7 +3701,7 @@ int lens_alloc(gs_universe* u, uint32_t ncell) {
 // Counting sort of m points into the lens grid: l_start, l_perm.
 int lens_sort(gs_universe* u, const AuxGrid& G, const double4* pts, int m) {
     k_aux_key<<<nblocks(m), BLOCK, 0, u->stream>>>(m, G, u->kp, pts, u->l_key, u->l_ticket, u->l_count);
-    k_scan<<<1, 1024, 0, u->stream>>>(u->l_count, u->l_start, G.ncell);
+    launch_scan(u->stream, u->l_count, u->l_start, G.ncell, u->scan_tmp_cap >= scan_tmp_words(G.ncell) ? u->scan_tmp : nullptr);
     k_scatter<<<nblocks(m), BLOCK, 0, u->stream>>>(m, u->l_key, u->l_ticket, u->l_start, u->l_perm);
     HIPCHK(hipGetLastError());
     return GS_OK;

@@ -193,3 +193,28 @@ def test_bond_components_of_an_observed_melt_match_oracle(GU):
     assert len(ref) > 100
     assert [(c["size"], c["formula"], c["members"]) for c in got] == \
         [(c["size"], c["formula"], c["members"]) for c in ref]
+
+
+def test_replayed_call_restores_the_bond_state(GU):
+    # A list skin so large that the first call's lists overflow their
+    # capacity and its stencils the build's LDS: the call is replayed from
+    # its snapshot (on the cell tiles), and the bond tracker, adjacency,
+    # counters and tick return to the call's start with the positions.  The
+    # observed bonds then match the oracle's run.
+    ph, st = melt(16)
+    nsteps = 150
+    ref, obs_ref, bs_ref, (rptr, ridx) = O.run_observed(ph, st, nsteps, nthreads=8)
+    say(f"replayed call with bonds n={st.n}")
+    with GU(ph, st) as u:
+        u.set_tuning(1, 4, skin=0.65 * ph.force_rc())
+        obs = u.step(nsteps)
+        tun = u.tuning()
+        got = u.download()
+        s = u.bond_stats()
+        ptr, idx = u.bonds()
+    assert tun["list_replays"] >= 1
+    assert bs_ref[-1][1] > 0
+    np.testing.assert_allclose(obs[:, 5], obs_ref[:, 5], rtol=1e-9)
+    np.testing.assert_allclose(got.x, ref.x, rtol=0, atol=1e-8)
+    assert [s["tick"], s["bonded_pairs"], s["bonded_entities"], s["bonds_formed"]] == list(bs_ref[-1][:4])
+    assert np.array_equal(ptr, rptr) and np.array_equal(idx, ridx)

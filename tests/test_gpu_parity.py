@@ -393,8 +393,7 @@ def test_list_mode_falls_back_to_tiles_when_a_stencil_is_too_dense(GU, mode):
     ph, st = w.physics, w.state
     say(f"dense stencil fallback, mode {mode}")
     with GU(ph, st) as u:
-        # list cells of edge >= 1.8 rc: ~7000 candidates per stencil
-        u.set_tuning(mode, 4, skin=0.8 * ph.force_rc())
+        u.set_tuning(mode, 4, skin=0.6 * ph.force_rc())
         obs_l = u.step(10)
         got_l = u.download()
         tun = u.tuning()
