@@ -172,20 +172,20 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ count, uin
 }
 
 // Multi-block exclusive scan of a large cell histogram (more than one
-// k_scan pass of 16384 cells): k_scan_part scans each 16384-cell chunk in
-// its own workgroup (and resets the histogram), k_scan of the chunk totals
-// gives each chunk's offset, and k_scan_fix adds it.  The cell starts are
-// the same integers as k_scan's.
-constexpr uint32_t SCAN_CHUNK = 1024u * 16u;
-__global__ __launch_bounds__(1024) void k_scan_part(uint32_t* __restrict__ count, uint32_t* __restrict__ start,
-                                                    uint32_t ncell, uint32_t* __restrict__ bsum) {
-    __shared__ uint32_t wsum[16];
+// k_scan pass of 16384 cells): k_scan_part scans each chunk of SCAN_CHUNK
+// cells in its own 256-thread workgroup (and resets the histogram), k_scan
+// of the chunk totals gives each chunk's offset, and k_scan_fix adds it.
+// The cell starts are the same integers as k_scan's.
+constexpr uint32_t SCAN_CHUNK = 256u * 8u;
+__global__ __launch_bounds__(256) void k_scan_part(uint32_t* __restrict__ count, uint32_t* __restrict__ start,
+                                                   uint32_t ncell, uint32_t* __restrict__ bsum) {
+    __shared__ uint32_t wsum[4];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t i0 = blockIdx.x * SCAN_CHUNK + (uint32_t)tid * 16u;
-    uint32_t v[16];
+    const uint32_t i0 = blockIdx.x * SCAN_CHUNK + (uint32_t)tid * 8u;
+    uint32_t v[8];
     uint32_t s = 0;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
+    for (int e = 0; e < 8; ++e) {
         const uint32_t idx = i0 + e;
         v[e] = idx < ncell ? count[idx] : 0u;
         s += v[e];
@@ -198,21 +198,16 @@ __global__ __launch_bounds__(1024) void k_scan_part(uint32_t* __restrict__ count
     }
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
-    if (w == 0) {
-        const uint32_t x = lane < 16 ? wsum[lane] : 0u;
-        uint32_t xi = x;
+    uint32_t before = 0, total = 0;
 #pragma unroll
-        for (int off = 1; off < 16; off <<= 1) {
-            const uint32_t t = __shfl_up(xi, off, 64);
-            if (lane >= off) xi += t;
-        }
-        if (lane < 16) wsum[lane] = xi - x;
-        if (lane == 15) bsum[blockIdx.x] = xi;
+    for (int k = 0; k < 4; ++k) {
+        before += k < w ? wsum[k] : 0u;
+        total += wsum[k];
     }
-    __syncthreads();
-    uint32_t excl = wsum[w] + incl - s;
+    if (tid == 0) bsum[blockIdx.x] = total;
+    uint32_t excl = before + incl - s;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
+    for (int e = 0; e < 8; ++e) {
         const uint32_t idx = i0 + e;
         if (idx < ncell) {
             start[idx] = excl;
@@ -222,12 +217,12 @@ __global__ __launch_bounds__(1024) void k_scan_part(uint32_t* __restrict__ count
     }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_fix(uint32_t* __restrict__ start, uint32_t ncell,
-                                                   const uint32_t* __restrict__ boff) {
+__global__ __launch_bounds__(256) void k_scan_fix(uint32_t* __restrict__ start, uint32_t ncell,
+                                                  const uint32_t* __restrict__ boff) {
     const uint32_t off = boff[blockIdx.x];
-    const uint32_t i0 = blockIdx.x * SCAN_CHUNK + threadIdx.x * 16u;
+    const uint32_t i0 = blockIdx.x * SCAN_CHUNK + threadIdx.x * 8u;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
+    for (int e = 0; e < 8; ++e) {
         const uint32_t idx = i0 + e;
         if (idx < ncell) start[idx] += off;
     }
@@ -244,14 +239,14 @@ inline size_t scan_tmp_words(uint32_t ncell) {
 // Exclusive scan of count[0..ncell) into start[0..ncell] (start[ncell] =
 // total) with count reset to zero; tmp: scan_tmp_words(ncell) words.
 inline void launch_scan(hipStream_t st, uint32_t* count, uint32_t* start, uint32_t ncell, uint32_t* tmp) {
-    if (ncell <= SCAN_CHUNK || !tmp) {
+    if (ncell <= 16384u || !tmp) {
         k_scan<<<1, 1024, 0, st>>>(count, start, ncell);
         return;
     }
     const uint32_t nb = (ncell + SCAN_CHUNK - 1) / SCAN_CHUNK;
-    k_scan_part<<<nb, 1024, 0, st>>>(count, start, ncell, tmp);
+    k_scan_part<<<nb, 256, 0, st>>>(count, start, ncell, tmp);
     k_scan<<<1, 1024, 0, st>>>(tmp, tmp + nb + 1, nb);
-    k_scan_fix<<<nb, 1024, 0, st>>>(start, ncell, tmp + nb + 1);
+    k_scan_fix<<<nb, 256, 0, st>>>(start, ncell, tmp + nb + 1);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_scatter(int n, const uint32_t* __restrict__ key,

@@ -57,6 +57,20 @@ def test_bound_pairs_bit_exact(GU, L, semantics):
     assert np.array_equal(got, ref)
 
 
+def test_bound_pairs_bit_exact_on_a_large_aux_grid(GU):
+    # aux cells of edge 1.5 max sigma in a 100 A box: 28^3 > 16384 cells, so
+    # the lens-grid scan runs in several workgroups
+    L = 100.0
+    st = random_gas(seed=91, n=40000, box=(L, L, L), species_pool=(H, O_, NA, SI))
+    ph = Physics(box=(L, L, L))
+    say(f"bound pairs on a large aux grid n={st.n} L={L}")
+    with GU(ph, st) as u:
+        got = u.bound_pairs()
+    ref = O.bound_pairs(ph, st.x, st.y, st.z, st.species)
+    assert len(ref) > 1000
+    assert np.array_equal(got, ref)
+
+
 def test_lifetime_and_species_matrix_fixture(GU):
     # the reference's own observation test (src/systems/bonds.rs:263-343),
     # frame by frame through upload + observe

@@ -114,6 +114,20 @@ def test_cell_membership_bit_exact(GU):
     assert np.array_equal(cells, ref)
 
 
+def test_cell_membership_bit_exact_on_a_grid_beyond_one_scan_pass(GU):
+    # 26^3 = 17576 cells: the histogram scan runs in several workgroups
+    L = 26 * 5.75 * 1.001
+    ph = Physics(box=(L, L, L))
+    st = random_gas(seed=31, n=40000, box=ph.box, species_pool=(0, 4, 5))
+    ref, dims = O.grid_cells(st.x, st.y, st.z, np.asarray(ph.box), ph.force_rc())
+    assert np.prod(dims) > 16384
+    say(f"cells beyond one scan pass: n={st.n} dims={dims}")
+    with GU(ph, st) as u:
+        cells, gdims = u.cells()
+    assert gdims == dims
+    assert np.array_equal(cells, ref)
+
+
 def test_integrator_systems_bit_exact(GU):
     w = mixture(n=4096, seed=4, steps=1)
     ph, st = w.physics, w.state
