@@ -62,9 +62,20 @@ def main() -> int:
                     help="skip the batched-replica and adjoint legs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device copy (HBM counter calibration) and exit")
+    ap.add_argument("--leg", choices=("batched", "adjoint", "ionic"), default=None,
+                    help="run only that leg (ticks from --steps) and print its JSON record, for profiling")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe()
+    if args.leg:
+        if args.leg == "batched":
+            rec = batched_leg(ticks=args.steps, segment=min(100, args.steps))
+        elif args.leg == "adjoint":
+            rec = adjoint_leg(steps=args.steps, ckpt=min(50, args.steps))
+        else:
+            rec = ionic_leg(ticks=args.steps, gr_every=min(100, args.steps))
+        print(json.dumps(rec), flush=True)
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only: the path would shard by spatial "
                                    "domain with halo exchange, but no multi-GPU figure is produced; run with --gpus 1"}))
@@ -244,11 +255,12 @@ def adjoint_leg(steps: int = 1000, ckpt: int = 50) -> dict:
     th0 = theta_of(ph).numpy()
     gr = (200, ph.box[0] / 4.0, 0.05)
     with GpuUniverse(ph, st) as u:
-        _, g0 = u.adjoint_forward(th0, keep, steps, ckpt, steps - 199, gr=gr)
+        win = max(1, steps - 199)  # the last 200 ticks
+        _, g0 = u.adjoint_forward(th0, keep, steps, ckpt, win, gr=gr)
         th = th0 * (1.0 + 0.005 * np.cos(np.arange(30)))
         u.upload(st)
         t0 = time.perf_counter()
-        loss, _ = u.adjoint_forward(th, keep, steps, ckpt, steps - 199, gr=gr, g0=g0)
+        loss, _ = u.adjoint_forward(th, keep, steps, ckpt, win, gr=gr, g0=g0)
         tf = time.perf_counter() - t0
         t0 = time.perf_counter()
         grad = u.adjoint_backward()
