@@ -278,6 +278,7 @@ def ionic_leg(ticks: int = 200, gr_every: int = 100) -> dict:
     interaction on every tick, g(r) (256 bins) every 100 ticks, and the
     components of the Bonds graph once at the end (timed apart)."""
     from genesis_amd.inputs import ionic_melt
+    from genesis_amd.structure import BondStructureSystem
     from genesis_amd.universe import GpuUniverse
     w = ionic_melt()
     ph, st = w.physics, w.state
@@ -296,6 +297,13 @@ def ionic_leg(ticks: int = 200, gr_every: int = 100) -> dict:
         t0 = time.perf_counter()
         comps = u.bond_labels()
         tc = time.perf_counter() - t0
+        # the bond-graph stoichiometry lens (components, formulas, binding
+        # energies, lifecycle against the previous sample), as sampled every
+        # 500 ticks in this workload
+        lens = BondStructureSystem(1)
+        t0 = time.perf_counter()
+        chem = lens.run(u, ticks, st.species)
+        ts = time.perf_counter() - t0
         bs = u.bond_stats()
     v = n * ticks / te
     log(f"ionic melt: {v:.4e} atom-steps/s ({1e3 * te / ticks:.3f} ms/tick), {n} atoms, tuning {tune}, "
@@ -303,7 +311,8 @@ def ionic_leg(ticks: int = 200, gr_every: int = 100) -> dict:
     return {"workload": f"{w.name}: Na/O rock-salt melt, bonds observed every tick, interaction on, "
                         f"g(r) every {gr_every}",
             "atoms": n, "ticks": ticks, "atom_steps_per_s": v, "ms_per_tick": 1e3 * te / ticks,
-            "bond_components_ms": 1e3 * tc, "bonded_pairs_at_end": bs["bonded_pairs"],
+            "bond_components_ms": 1e3 * tc, "bond_structure_ms": 1e3 * ts,
+            "aggregates_at_end": chem["aggregates"], "bonded_pairs_at_end": bs["bonded_pairs"],
             "tuning": {k: tune[k] for k in ("pair_mode", "lanes_per_atom", "skin")}}
 
 

@@ -117,6 +117,7 @@ _SIGNATURES = {
     "gs_upload_bonds": (ctypes.c_int, [_H, _I32, _I32]),
     "gs_bound_pairs": (ctypes.c_int, [_H, _U32, _U32, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     "gs_bond_components": (ctypes.c_int, [_H, _U32, ctypes.POINTER(ctypes.c_int64)]),
+    "gs_bond_binding": (ctypes.c_int, [_H, _D]),
     "gs_bond_reset": (ctypes.c_int, [_H]),
 }
 

@@ -271,6 +271,14 @@ class GpuUniverse:
                                                ctypes.byref(nc)))
         return labels[:self.n]
 
+    def bond_binding(self) -> np.ndarray:
+        """Observed binding energy per entity (gs_bond_binding): the raw pair potential
+        summed over the entity's bonded partners with a larger id, on the resident
+        positions; a component's binding energy is the sum over its members."""
+        out = np.zeros(max(self.n, 1))
+        _lib.check(self.lib.gs_bond_binding(self.h, _dp(out)))
+        return out[:self.n]
+
     def bond_components(self, species) -> list:
         """bond_components (src/analysis/mod.rs:337-412) of the Bonds graph: one dict per
         component (size, composition, formula, members ascending, edges) ordered by size

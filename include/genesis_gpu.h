@@ -268,6 +268,14 @@ int gs_bound_pairs(gs_universe* u, uint32_t* a, uint32_t* b, int64_t cap, int64_
  * src/analysis/mod.rs:337-412): labels[n] by entity id = smallest entity
  * id of its component, 0xFFFFFFFF for entities without bonds. */
 int gs_bond_components(gs_universe* u, uint32_t* labels, int64_t* n_components);
+/* Observed binding energy of the Bonds graph on the resident positions
+ * (BondStructureSystem::run, src/systems/bond_structure.rs:95-123, with
+ * pair_potential_raw, src/physics/forces.rs:360-384): binding[e] (by entity
+ * id) = sum over e's bonded partners m > e of the raw, unswitched LJ term
+ * plus, when enabled, the Coulomb term with the species charges and the
+ * harmonic bond term, at r = max(|min_image(x_e - x_m)|, 1e-9).  A
+ * component's binding energy is the sum over its members. */
+int gs_bond_binding(gs_universe* u, double* binding);
 /* Clears the tracker, the Bonds adjacency, the cumulative counters and the
  * tick counter. */
 int gs_bond_reset(gs_universe* u);

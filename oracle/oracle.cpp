@@ -1156,6 +1156,48 @@ int or_clusters(int64_t n, const double* x, const double* y, const double* z, co
 // index; edges: pairs of ids.  Writes, per component in output order (size
 // desc, then formula asc), its size into comp_size and its members (sorted
 // ids) concatenated into members.  Returns the number of components.
+// pair_potential_raw, src/physics/forces.rs:360-384: the raw (unswitched)
+// LJ term of the mixed pair, plus k_e q_a q_b / r with the table's species
+// charges when with_coulomb, plus 0.5 k (r - r_eq)^2 with the pair's well
+// curvature and equilibrium distance when with_bond.
+double or_pair_potential_raw(const gs_params* p, int a, int b, double r, int with_coulomb, int with_bond) {
+    const double sig = 0.5 * (p->sigma[a] + p->sigma[b]);
+    const double eps = p->thermal_constant * std::sqrt(p->epsilon_k[a] * p->epsilon_k[b]);
+    double m, v;
+    lj_raw(LjPair{sig, eps}, r, &m, &v);
+    double total = v;
+    if (with_coulomb) {
+        const double q = p->coulomb_constant * p->charge[a] * p->charge[b];
+        total += q / r;
+    }
+    if (with_bond) {
+        const double k = well_curvature(eps, sig);
+        const double r_eq = equilibrium_distance(sig);
+        total += 0.5 * k * (r - r_eq) * (r - r_eq);
+    }
+    return total;
+}
+
+// The binding energy of each bonded edge (a, b) as BondStructureSystem::run
+// evaluates it (src/systems/bond_structure.rs:101-115): r =
+// max(|min_image(x_a - x_b)|, 1e-9), pair_potential_raw with the system's
+// Coulomb and bond switches.
+void or_bond_binding(const gs_params* p, int64_t n, const double* x, const double* y, const double* z,
+                     const uint8_t* species, int64_t nedge, const uint32_t* ea, const uint32_t* eb, double* out) {
+    const V3 size = v3(p->box[0], p->box[1], p->box[2]);
+    for (int64_t k = 0; k < nedge; ++k) {
+        const uint32_t a = ea[k], b = eb[k];
+        if ((int64_t)a >= n || (int64_t)b >= n) {
+            out[k] = 0.0;
+            continue;
+        }
+        const V3 d = min_image(v3(x[a], y[a], z[a]) - v3(x[b], y[b], z[b]), size);
+        const double r = fmax_r(std::sqrt(len2(d)), 1e-9);
+        out[k] = or_pair_potential_raw(p, species[a], species[b], r, p->enable_electrostatics,
+                                       p->enable_bond_interaction);
+    }
+}
+
 int or_bond_components(int64_t ne, const uint32_t* ent_id, const uint8_t* ent_species, int64_t nedge,
                        const uint32_t* ea, const uint32_t* eb, int64_t* comp_size, uint32_t* members,
                        char* formulas, int64_t formula_stride) {

@@ -78,6 +78,9 @@ def lib():
             "or_observe_frames": (ctypes.c_int, [P, ctypes.c_int64, ctypes.c_int64, _D, _D, _D, _U8, ctypes.c_int64,
                                                  _D, _I32, _I32, ctypes.c_int64]),
             "or_bound_pairs": (ctypes.c_int64, [P, ctypes.c_int64, _D, _D, _D, _U8, _U32, _U32, ctypes.c_int64]),
+            "or_pair_potential_raw": (ctypes.c_double, [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                                        ctypes.c_int]),
+            "or_bond_binding": (None, [P, ctypes.c_int64, _D, _D, _D, _U8, ctypes.c_int64, _U32, _U32, _D]),
             "or_persistence_ticks": (ctypes.c_uint64, [P, ctypes.c_int, ctypes.c_int]),
             "or_track": (ctypes.c_int64, [ctypes.c_int64, _I64, _U32, _U32, ctypes.c_uint64, _U32, _U32, _U64,
                                           ctypes.c_int64]),
@@ -281,6 +284,28 @@ def bound_pairs(physics, x, y, z, species):
     k = lib().or_bound_pairs(ctypes.byref(p), n, _p(x), _p(y), _p(z), _p(sp, ctypes.c_uint8),
                              _p(pa, ctypes.c_uint32), _p(pb, ctypes.c_uint32), cap)
     return np.stack([pa[:k], pb[:k]], 1)
+
+
+def pair_potential_raw(physics, a, b, r, with_coulomb=None, with_bond=None):
+    """pair_potential_raw (src/physics/forces.rs:360-384); the switches default to the physics'."""
+    c = physics.to_c()
+    wc = physics.enable_electrostatics if with_coulomb is None else with_coulomb
+    wb = physics.enable_bond_interaction if with_bond is None else with_bond
+    return lib().or_pair_potential_raw(ctypes.byref(c), int(a), int(b), float(r), int(bool(wc)), int(bool(wb)))
+
+
+def bond_binding(physics, x, y, z, species, edges):
+    """Binding energy of each edge (a, b) as BondStructureSystem::run evaluates it
+    (src/systems/bond_structure.rs:101-115)."""
+    c = physics.to_c()
+    x, y, z = _d(x), _d(y), _d(z)
+    sp = np.ascontiguousarray(species, dtype=np.uint8)
+    e = np.asarray(edges, dtype=np.uint32).reshape(-1, 2)
+    ea, eb = np.ascontiguousarray(e[:, 0]), np.ascontiguousarray(e[:, 1])
+    out = np.zeros(len(e))
+    lib().or_bond_binding(ctypes.byref(c), len(x), _p(x), _p(y), _p(z), _p(sp, ctypes.c_uint8), len(e),
+                          _p(ea, ctypes.c_uint32), _p(eb, ctypes.c_uint32), _p(out))
+    return out
 
 
 def persistence_ticks(physics, a, b):
