@@ -118,6 +118,8 @@ _SIGNATURES = {
     "gs_bound_pairs": (ctypes.c_int, [_H, _U32, _U32, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     "gs_bond_components": (ctypes.c_int, [_H, _U32, ctypes.POINTER(ctypes.c_int64)]),
     "gs_bond_binding": (ctypes.c_int, [_H, _D]),
+    "gs_velocity_histogram": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_int32, _U64,
+                                             ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), _D]),
     "gs_bond_reset": (ctypes.c_int, [_H]),
 }
 

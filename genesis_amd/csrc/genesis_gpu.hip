@@ -2193,6 +2193,36 @@ __global__ __launch_bounds__(CELL_THREADS) __attribute__((amdgpu_waves_per_eu(5)
 
 // Auxiliary grid of a lens: SpatialGrid::new(size, min_cell) over a point set
 // (src/physics/grid.rs:67-86).
+// velocity_histogram (src/stats/mod.rs:269-297): |v| = sqrt(x^2 + y^2 + z^2)
+// (Vec3::length, left to right); speeds >= max_speed are overflow, the
+// others fall in bin min((s / bin_width) as usize, nbins - 1).  Integer
+// counts, so the order of the atomic adds does not matter.
+constexpr int VHIST_MAX_BINS = 512;
+__global__ __launch_bounds__(BLOCK) void k_vhist(int n, const double* __restrict__ vx, const double* __restrict__ vy,
+                                                 const double* __restrict__ vz, double max_speed, double bin_width,
+                                                 int nbins, unsigned long long* __restrict__ counts) {
+    __shared__ uint32_t h[VHIST_MAX_BINS + 1];  // [nbins] = overflow
+    for (int b = threadIdx.x; b <= nbins; b += BLOCK) h[b] = 0u;
+    __syncthreads();
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k < n) {
+        const double x = vx[k], y = vy[k], z = vz[k];
+        const double sp = sqrt(x * x + y * y + z * z);
+        int bin;
+        if (sp >= max_speed) {
+            bin = nbins;
+        } else {
+            const double q = sp / bin_width;
+            const uint32_t i = q > 0.0 ? (q < 4294967295.0 ? (uint32_t)q : 0xFFFFFFFFu) : 0u;
+            bin = (int)min(i, (uint32_t)(nbins - 1));
+        }
+        atomicAdd(&h[bin], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b <= nbins; b += BLOCK)
+        if (h[b]) atomicAdd(&counts[b], (unsigned long long)h[b]);
+}
+
 struct AuxGrid {
     uint32_t dims[3];
     double cell[3];
@@ -3794,6 +3824,39 @@ int radial_impl(gs_universe* u, bool between, int ta, int tb, double r_max_in, i
 }  // namespace
 
 extern "C" {
+
+int gs_velocity_histogram(gs_universe* u, double max_speed, int32_t nbins_in, uint64_t* counts,
+                          uint64_t* samples, uint64_t* overflow, double* bin_width) {
+    int rc = check_state(u);
+    if (rc) return rc;
+    const int nbins = std::min(std::max(nbins_in, 1), VHIST_MAX_BINS);
+    const double ms = std::fmax(max_speed, 1e-9);
+    const double bw = ms / (double)nbins;
+    if ((rc = lens_alloc(u, 1))) return rc;
+    unsigned long long* d = u->l_bins;
+    if (!d || u->l_bins_cap < nbins + 1) {
+        (void)hipFree(u->l_bins);
+        u->l_bins = nullptr;
+        u->l_bins_cap = 0;
+        if ((rc = dalloc(&u->l_bins, (size_t)std::max(nbins + 1, VHIST_MAX_BINS + 1)))) return rc;
+        u->l_bins_cap = std::max(nbins + 1, VHIST_MAX_BINS + 1);
+        d = u->l_bins;
+    }
+    HIPCHK(hipMemsetAsync(d, 0, sizeof(unsigned long long) * (size_t)(nbins + 1), u->stream));
+    const int c = u->cur;
+    if (u->n > 0)
+        k_vhist<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->n, u->v[c][0], u->v[c][1], u->v[c][2], ms, bw, nbins, d);
+    HIPCHK(hipGetLastError());
+    std::vector<unsigned long long> h((size_t)nbins + 1);
+    HIPCHK(hipMemcpyAsync(h.data(), d, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, u->stream));
+    HIPCHK(hipStreamSynchronize(u->stream));
+    if (counts)
+        for (int k = 0; k < nbins; ++k) counts[k] = h[k];
+    if (samples) *samples = (uint64_t)u->n;
+    if (overflow) *overflow = h[nbins];
+    if (bin_width) *bin_width = bw;
+    return GS_OK;
+}
 
 int gs_radial_distribution(gs_universe* u, double r_max, int32_t nbins, uint64_t* counts, double* g,
                            double* r_max_out, double* dr_out) {

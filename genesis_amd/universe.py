@@ -193,6 +193,18 @@ class GpuUniverse:
         return g
 
     # -- lenses --------------------------------------------------------------------
+    def velocity_histogram(self, max_speed: float, nbins: int) -> dict:
+        """velocity_histogram (src/stats/mod.rs:255-297) of the resident velocities:
+        {max_speed, bin_width, bins (u64), samples, overflow}."""
+        nb = min(max(int(nbins), 1), 512)
+        counts = np.zeros(nb, dtype=np.uint64)
+        samples, overflow, bw = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_double()
+        _lib.check(self.lib.gs_velocity_histogram(self.h, float(max_speed), nb,
+                                                  counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                  ctypes.byref(samples), ctypes.byref(overflow), ctypes.byref(bw)))
+        return {"max_speed": max(float(max_speed), 1e-9), "bin_width": bw.value, "bins": counts,
+                "samples": samples.value, "overflow": overflow.value}
+
     def radial_distribution(self, r_max: float, nbins: int):
         counts = np.zeros(max(1, min(nbins, 1 << 16)), dtype=np.uint64)
         g = np.zeros(counts.shape[0])

@@ -210,6 +210,12 @@ int gs_timing_read(gs_universe* u, gs_timing* out);
 
 /* ---- observation lenses ---------------------------------------------------- */
 
+/* velocity_histogram (src/stats/mod.rs:255-297): speeds |v| of every atom in
+ * nbins (clamped to 1..512) bins of width max(max_speed, 1e-9) / nbins;
+ * counts: u64[nbins]; speeds >= max_speed count as overflow.  Any output may
+ * be NULL. */
+int gs_velocity_histogram(gs_universe* u, double max_speed, int32_t nbins, uint64_t* counts,
+                          uint64_t* samples, uint64_t* overflow, double* bin_width);
 /* radial_distribution (src/analysis/mod.rs:87-133), incl. the deterministic
  * subsample above 8192 atoms (:28-46).  counts: u64[nbins] pair counts,
  * g: f64[nbins]; r_max_out/dr_out receive the clipped r_max and bin width.

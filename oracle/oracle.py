@@ -308,6 +308,26 @@ def bond_binding(physics, x, y, z, species, edges):
     return out
 
 
+def velocity_histogram(vx, vy, vz, max_speed, bins):
+    """velocity_histogram (src/stats/mod.rs:269-297) in numpy: bins clamped to
+    1..512, width max(max_speed, 1e-9) / bins, |v| = sqrt((x^2 + y^2) + z^2),
+    speeds >= max_speed counted as overflow, the others in
+    min((s / width) as usize, bins - 1)."""
+    bins = min(max(int(bins), 1), 512)
+    max_speed = max(float(max_speed), 1e-9)
+    width = max_speed / bins
+    vx, vy, vz = (np.asarray(a, dtype=np.float64) for a in (vx, vy, vz))
+    s = np.sqrt(vx * vx + vy * vy + vz * vz)
+    over = s >= max_speed
+    q = s[~over] / width
+    # `as usize`: NaN -> 0, saturating above; then the last bin
+    idx = np.where(q > 0.0, np.minimum(q, 2.0 ** 63), 0.0).astype(np.uint64)
+    idx = np.minimum(idx, np.uint64(bins - 1))
+    counts = np.bincount(idx.astype(np.int64), minlength=bins).astype(np.uint64)
+    return {"max_speed": max_speed, "bin_width": width, "bins": counts, "samples": int(s.size),
+            "overflow": int(over.sum())}
+
+
 def persistence_ticks(physics, a, b):
     p = physics.to_c()
     return int(lib().or_persistence_ticks(ctypes.byref(p), a, b))

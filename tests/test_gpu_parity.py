@@ -486,3 +486,27 @@ def test_invalid_tuning_is_rejected(GU):
             u.set_tuning(1, 4)
         with pytest.raises(GsError):
             u.set_tuning(1, 3)
+
+
+@pytest.mark.parametrize("nbins,max_speed", [(64, None), (512, None), (7, 0.0)])
+def test_velocity_histogram_bit_exact(GU, nbins, max_speed):
+    # velocity_histogram (src/stats/mod.rs:269-297) of the resident state
+    # after 20 ticks, with speeds planted on bin edges, at max_speed and zero
+    w = mixture(n=4096, seed=12, steps=20)
+    ph, st = w.physics, w.state
+    vmax = float(np.sqrt(st.vx ** 2 + st.vy ** 2 + st.vz ** 2).max()) * 0.8 if max_speed is None else max_speed
+    st = st.copy()
+    width = max(vmax, 1e-9) / nbins
+    st.vx[:4] = [0.0, width, 3 * width, max(vmax, 1e-9)]
+    st.vy[:4] = st.vz[:4] = 0.0
+    say(f"velocity histogram nbins={nbins} max={vmax:.4g}")
+    with GU(ph, st) as u:
+        h0 = u.velocity_histogram(vmax, nbins)
+        u.step(20)
+        got = u.download()
+        h1 = u.velocity_histogram(vmax, nbins)
+    for h, s in ((h0, st), (h1, got)):
+        ref = O.velocity_histogram(s.vx, s.vy, s.vz, vmax, nbins)
+        assert h["bin_width"] == ref["bin_width"] and h["samples"] == ref["samples"] == st.n
+        assert h["overflow"] == ref["overflow"]
+        assert np.array_equal(h["bins"], ref["bins"])

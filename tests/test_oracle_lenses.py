@@ -168,3 +168,39 @@ def test_formula_orders_multiplicity_then_symbol():
     ents = [(i, H) for i in range(6)] + [(6, C), (7, C)]
     edges = [(i, i + 1) for i in range(7)]
     assert O.bond_components(ents, edges)[0]["formula"] == "H6-C2"
+
+
+def test_velocity_histogram_hand_fixture():
+    # speeds on bin edges land in the upper bin ((s / w) as usize), the speed
+    # equal to max_speed and above is overflow, a zero speed is bin 0, and a
+    # NaN speed (s >= max fails) lands in bin 0 as `NaN as usize` = 0
+    from oracle import oracle as O
+    w = 0.25
+    vx = np.array([0.0, 0.25, 0.5, 0.74, 0.99, 1.0, 3.0, np.nan])
+    z = np.zeros_like(vx)
+    h = O.velocity_histogram(vx, z, z, 1.0, 4)
+    assert h["bin_width"] == w and h["samples"] == 8 and h["overflow"] == 2
+    assert list(h["bins"]) == [2, 1, 2, 1]
+    # 3-4-5: |(0.3, 0.4, 0)| = 0.5 exactly -> bin 2 of width 0.25
+    h = O.velocity_histogram([0.3], [0.4], [0.0], 1.0, 4)
+    assert list(h["bins"]) == [0, 0, 1, 0]
+    # bins clamp to 1..512; max_speed floors at 1e-9
+    assert len(O.velocity_histogram([1.0], [0.0], [0.0], 2.0, 0)["bins"]) == 1
+    assert len(O.velocity_histogram([1.0], [0.0], [0.0], 2.0, 10_000)["bins"]) == 512
+    assert O.velocity_histogram([0.0], [0.0], [0.0], -1.0, 4)["max_speed"] == 1e-9
+
+
+def test_velocity_histogram_brute_force():
+    from oracle import oracle as O
+    rng = np.random.default_rng(11)
+    v = rng.normal(0.0, 1.0, (5000, 3))
+    h = O.velocity_histogram(v[:, 0], v[:, 1], v[:, 2], 3.0, 37)
+    s = [math.sqrt(a * a + b * b + c * c) for a, b, c in v]
+    ref = [0] * 37
+    over = 0
+    for x in s:
+        if x >= 3.0:
+            over += 1
+        else:
+            ref[min(int(x / (3.0 / 37)), 36)] += 1
+    assert list(h["bins"]) == ref and h["overflow"] == over and sum(ref) + over == 5000
