@@ -938,23 +938,60 @@ __global__ __launch_bounds__(BRUTE_WAVES * WAVE) void k_pair_brute(
     if (tid == 0) vpart[blockIdx.x] = 0.5 * tree_sum<BRUTE_WAVES>(wv);
 }
 
+// Hand-off of per-workgroup partials to the last workgroup of a launch.
+// A device-scope fence (__threadfence) compiles to a write-back and an
+// invalidation of the whole L2 of the workgroup's XCD; issued by every
+// workgroup of a launch it costs microseconds and evicts the data the
+// other workgroups of that XCD are still reading.  Instead a partial is
+// stored as a device-scope relaxed atomic (written through to the device's
+// coherence point), the writing wave waits for its stores to complete
+// before the ticket is taken, and the last workgroup reads the partials
+// back with device-scope atomic loads.  Ticket increments are device-scope
+// atomics, performed at the coherence point.
+template <typename T>
+__device__ __forceinline__ void publish(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T collect(const T* p) {
+    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Waits until this wave's memory operations, its published stores among
+// them, have completed (the signal fences keep the compiler from moving
+// memory operations across the wait).
+__device__ __forceinline__ void published() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+// Takes the launch's ticket once this wave's published stores are complete;
+// the workgroup that draws gridDim.x - 1 is the last.
+__device__ __forceinline__ uint32_t take_ticket(uint32_t* counter) {
+    published();
+    const uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    return t;
+}
+
 // Deterministic block sum: fixed per-thread strides then a fixed tree.
 // Sum of a[0..n) by a workgroup of NT threads (red: NT / 64 doubles of
-// LDS); the order of the additions depends on n and NT only.
-template <int NT = BLOCK>
+// LDS); the order of the additions depends on n and NT only.  COH: a[] was
+// published by other workgroups of the running launch.
+template <int NT = BLOCK, bool COH = false>
 __device__ double block_sum_fixed(const double* __restrict__ a, uint32_t n, double* red) {
     static_assert(NT % WAVE == 0, "whole waves");
     const int tid = threadIdx.x;
+    auto at = [&](uint32_t k) { return COH ? collect(a + k) : a[k]; };
     // four independent strided accumulators keep four loads in flight
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     uint32_t k = tid;
     for (; k + 3 * NT < n; k += 4 * NT) {
-        s0 += a[k];
-        s1 += a[k + NT];
-        s2 += a[k + 2 * NT];
-        s3 += a[k + 3 * NT];
+        s0 += at(k);
+        s1 += at(k + NT);
+        s2 += at(k + 2 * NT);
+        s3 += at(k + 3 * NT);
     }
-    for (; k < n; k += NT) s0 += a[k];
+    for (; k < n; k += NT) s0 += at(k);
     double s = (s0 + s1) + (s2 + s3);
     s = wave_sum(s);
     if ((tid & 63) == 0) red[tid >> 6] = s;
@@ -1038,15 +1075,13 @@ __global__ __launch_bounds__(BLOCK) void k_kick_ke(KParams P, double* __restrict
     if ((tid & 63) == 0) red[tid >> 6] = ke;
     __syncthreads();
     if (tid == 0) {
-        kpart[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
-        __threadfence();
-        const uint32_t t = atomicAdd(counter, 1u);
-        last = (t == gridDim.x - 1);
+        publish(kpart + blockIdx.x, (red[0] + red[1]) + (red[2] + red[3]));
+        last = take_ticket(counter) == gridDim.x - 1;
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();
-    const double K = block_sum_fixed(kpart, gridDim.x, red);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    const double K = block_sum_fixed<BLOCK, true>(kpart, gridDim.x, red);
     const double V = vpart ? block_sum_fixed(vpart, nvpart, red) : 0.0;
     const double Vb = vbpart ? block_sum_fixed(vbpart, nvb, red) : 0.0;
     if (tid == 0) {
@@ -1899,16 +1934,15 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     }
     __syncthreads();
     if (tid == 0) {
-        vpart[blockIdx.x] = 0.5 * ((wv[0] + wv[1]) + (wv[2] + wv[3]));
-        kpart[blockIdx.x] = (wk[0] + wk[1]) + (wk[2] + wk[3]);
-        __threadfence();
-        last = atomicAdd(&flags[5], 1u) == gridDim.x - 1;
+        publish(vpart + blockIdx.x, 0.5 * ((wv[0] + wv[1]) + (wv[2] + wv[3])));
+        publish(kpart + blockIdx.x, (wk[0] + wk[1]) + (wk[2] + wk[3]));
+        last = take_ticket(&flags[5]) == gridDim.x - 1;
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();
-    const double K = block_sum_fixed(kpart, gridDim.x, red);
-    const double V = block_sum_fixed(vpart, gridDim.x, red);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    const double K = block_sum_fixed<BLOCK, true>(kpart, gridDim.x, red);
+    const double V = block_sum_fixed<BLOCK, true>(vpart, gridDim.x, red);
     const double Vb = vbpart ? block_sum_fixed(vbpart, nvb, red) : 0.0;
     if (tid == 0) {
         close_tick(K, V, Vb, (int)N, th, lam_out, obs);
@@ -2172,16 +2206,15 @@ __global__ __launch_bounds__(CELL_THREADS) __attribute__((amdgpu_waves_per_eu(5)
             sv += wv[w];
             sk += wk[w];
         }
-        vpart[blockIdx.x] = 0.5 * sv;
-        kpart[blockIdx.x] = sk;
-        __threadfence();
-        last = atomicAdd(&flags[5], 1u) == gridDim.x - 1;
+        publish(vpart + blockIdx.x, 0.5 * sv);
+        publish(kpart + blockIdx.x, sk);
+        last = take_ticket(&flags[5]) == gridDim.x - 1;
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();
-    const double K = block_sum_fixed<CELL_THREADS>(kpart, gridDim.x, red);
-    const double V = block_sum_fixed<CELL_THREADS>(vpart, gridDim.x, red);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    const double K = block_sum_fixed<CELL_THREADS, true>(kpart, gridDim.x, red);
+    const double V = block_sum_fixed<CELL_THREADS, true>(vpart, gridDim.x, red);
     const double Vb = vbpart ? block_sum_fixed<CELL_THREADS>(vbpart, nvb, red) : 0.0;
     if (tid == 0) {
         close_tick(K, V, Vb, (int)N, th, lam_out, obs);
