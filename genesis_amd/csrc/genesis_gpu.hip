@@ -1007,6 +1007,54 @@ __device__ double block_sum_fixed(const double* __restrict__ a, uint32_t n, doub
     return t;
 }
 
+// block_sum_fixed of two arrays of the same length at once (each result
+// bit-identical to block_sum_fixed of its array): the loads of both share
+// one round of latency.  red: 2 NT / 64 doubles of LDS.
+template <int NT = BLOCK, bool COH = false>
+__device__ void block_sum2_fixed(const double* __restrict__ a, const double* __restrict__ b, uint32_t n,
+                                 double* red, double& sa, double& sb) {
+    static_assert(NT % WAVE == 0, "whole waves");
+    const int tid = threadIdx.x;
+    auto at = [&](const double* p, uint32_t k) { return COH ? collect(p + k) : p[k]; };
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
+    uint32_t k = tid;
+    for (; k + 3 * NT < n; k += 4 * NT) {
+        a0 += at(a, k);
+        b0 += at(b, k);
+        a1 += at(a, k + NT);
+        b1 += at(b, k + NT);
+        a2 += at(a, k + 2 * NT);
+        b2 += at(b, k + 2 * NT);
+        a3 += at(a, k + 3 * NT);
+        b3 += at(b, k + 3 * NT);
+    }
+    for (; k < n; k += NT) {
+        a0 += at(a, k);
+        b0 += at(b, k);
+    }
+    double x = (a0 + a1) + (a2 + a3), y = (b0 + b1) + (b2 + b3);
+    x = wave_sum(x);
+    y = wave_sum(y);
+    constexpr int W = NT / WAVE;
+    if ((tid & 63) == 0) {
+        red[tid >> 6] = x;
+        red[W + (tid >> 6)] = y;
+    }
+    __syncthreads();
+    if (W == 4) {
+        sa = (red[0] + red[1]) + (red[2] + red[3]);
+        sb = (red[4] + red[5]) + (red[6] + red[7]);
+    } else {
+        sa = red[0];
+        sb = red[W];
+        for (int w = 1; w < W; ++w) {
+            sa += red[w];
+            sb += red[W + w];
+        }
+    }
+    __syncthreads();
+}
+
 // Berendsen lambda from the kinetic energy after the second kick
 // (src/systems/thermostat.rs:47-71, lambda = sqrt(max(0, 1 + (T0/T - 1)/tau)),
 // tau in ticks) and the tick's observables (src/systems/aggregate.rs:86-121,
@@ -1739,19 +1787,9 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     __shared__ double sirc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];  // 1 / rcore, 8-byte stride
     __shared__ double wv[BLOCK / WAVE];
     __shared__ double wk[BLOCK / WAVE];
-    __shared__ double red[BLOCK / WAVE];
+    __shared__ double red[2 * BLOCK / WAVE];
     __shared__ bool last;
     const int tid = threadIdx.x;
-    if (SINGLE) {
-        if (tid == 0) sc[0] = coef[0];
-    } else {
-        for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += BLOCK) {
-            sc[t] = coef[t];
-            sirc[t] = coef[t].inv_rcore;
-        }
-        if (tid < GS_NSPECIES) ssp[tid] = reinterpret_cast<const SpCoef*>(coef + COEF_SP_OFFSET)[tid];
-    }
-    __syncthreads();
     // Atom chunk of this workgroup.  Workgroup b runs on L2 slice b mod 8
     // and, when the grid holds rpc workgroups per CU of a 256-CU device,
     // shares its CU with b + 256, b + 512, ...: those get adjacent chunks
@@ -1764,6 +1802,26 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     }
     const uint32_t g = chunk * BLOCK + tid;
     const uint32_t i = g >> LOGT;
+    // the atom's own words are loaded before the coefficient tables are
+    // staged, so the two latencies overlap
+    double4 pi0 = make_double4(0.0, 0.0, 0.0, 0.0);
+    int ti0 = 0;
+    uint32_t nl0 = 0u;
+    if (i < N) {
+        pi0 = pos4[i];
+        ti0 = type[i];
+        nl0 = nlen[i];
+    }
+    if (SINGLE) {
+        if (tid == 0) sc[0] = coef[0];
+    } else {
+        for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += BLOCK) {
+            sc[t] = coef[t];
+            sirc[t] = coef[t].inv_rcore;
+        }
+        if (tid < GS_NSPECIES) ssp[tid] = reinterpret_cast<const SpCoef*>(coef + COEF_SP_OFFSET)[tid];
+    }
+    __syncthreads();
     // a tick that rebuilt leaves the buckets empty for the next binning
     // (BATCH: those of the replicas that built, flags[11] if any did)
     if (flags[0] != 0u)
@@ -1779,8 +1837,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     const bool lists_ok = (flags[2] | flags[6]) == 0u;
     bool own = i < N;  // BATCH: and the atom's replica is on lists
     if (i < N) {
-        const double4 pi = pos4[i];
-        ti = type[i];
+        const double4 pi = pi0;
+        ti = ti0;
         const double kqi = P.ke * pi.w;
         double Lx = P.box[0], Ly = P.box[1], Lz = P.box[2];
         double iLx, iLy, iLz;
@@ -1799,7 +1857,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
             iLy = 1.0 / Ly;
             iLz = 1.0 / Lz;
         }
-        const uint32_t len = lists_ok && own ? nlen[i] : 0u;
+        const uint32_t len = lists_ok && own ? nl0 : 0u;
         const PairCoef* row = SINGLE ? sc : sc + ti * GS_NSPECIES;
         const double sig_i = SINGLE ? 0.0 : ssp[ti].sigma, a4_i = SINGLE ? 0.0 : ssp[ti].a4;
         // this lane's entries: s, s+T, ... (cnt of them), four per group;
@@ -1941,8 +1999,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     __syncthreads();
     if (!last) return;
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    const double K = block_sum_fixed<BLOCK, true>(kpart, gridDim.x, red);
-    const double V = block_sum_fixed<BLOCK, true>(vpart, gridDim.x, red);
+    double K, V;
+    block_sum2_fixed<BLOCK, true>(kpart, vpart, gridDim.x, red, K, V);
     const double Vb = vbpart ? block_sum_fixed(vbpart, nvb, red) : 0.0;
     if (tid == 0) {
         close_tick(K, V, Vb, (int)N, th, lam_out, obs);
@@ -2011,7 +2069,7 @@ __global__ __launch_bounds__(CELL_THREADS) __attribute__((amdgpu_waves_per_eu(5)
     __shared__ uint32_t wsum[NW];
     __shared__ double wv[NW];
     __shared__ double wk[NW];
-    __shared__ double red[NW];
+    __shared__ double red[2 * NW];
     __shared__ bool last;
     static_assert(3 * CELL_THREADS * sizeof(double) <= STAGE_CAP * sizeof(double4), "partials fit the tile");
     const int tid = threadIdx.x;
@@ -2213,8 +2271,8 @@ __global__ __launch_bounds__(CELL_THREADS) __attribute__((amdgpu_waves_per_eu(5)
     __syncthreads();
     if (!last) return;
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    const double K = block_sum_fixed<CELL_THREADS, true>(kpart, gridDim.x, red);
-    const double V = block_sum_fixed<CELL_THREADS, true>(vpart, gridDim.x, red);
+    double K, V;
+    block_sum2_fixed<CELL_THREADS, true>(kpart, vpart, gridDim.x, red, K, V);
     const double Vb = vbpart ? block_sum_fixed<CELL_THREADS>(vbpart, nvb, red) : 0.0;
     if (tid == 0) {
         close_tick(K, V, Vb, (int)N, th, lam_out, obs);
