@@ -1264,6 +1264,14 @@ __global__ __launch_bounds__(BLOCK) void k_cells_entity(KParams P, const uint32_
 // every tick.
 
 constexpr uint32_t NBR_IDX_MASK = 0x0FFFFFFFu;  // entry = (species << 28) | sorted index
+// A candidate as the list build stages it in LDS: its position relative to
+// the centre of the home cell in FP32 (a Verlet list may hold a superset of
+// the pairs within rc + skin, so the build's distance test need not be FP64)
+// and its list entry; one 16-byte LDS read per candidate test.
+struct alignas(16) BuildCand {
+    float x, y, z;
+    uint32_t e;
+};
 constexpr int LIST_FLAGS = 16;                  // words of the list flags
 constexpr int LIST_FLAGS_RESET = 8;             // the first ones are reset at every call
 constexpr int LDS_IDX_CAP = 2048;               // stencil candidates held (and ordered) in LDS
@@ -1469,15 +1477,15 @@ __device__ __forceinline__ void build_cells(
     constexpr int NWV = NT / WAVE;
     // the candidates' indices are staged, unordered, in the memory of the
     // position tile, which is filled only after they have been ordered
-    __shared__ double4 tile[BUILD_TILE];
-    __shared__ uint8_t ttype[BUILD_TILE];
+    __shared__ BuildCand tile[BUILD_TILE];
+    __shared__ double org[3];  // centre of the home cell
     constexpr int IDX_CAP = LOCAL ? LDS_IDX_CAP_LOCAL : (BATCH ? LDS_IDX_CAP_BATCH : LDS_IDX_CAP);
     __shared__ uint32_t ord[IDX_CAP];
     __shared__ uint32_t hcnt[HOME_CAP];  // running list lengths between chunks
     __shared__ uint32_t seg_base[27];
     __shared__ uint32_t seg_pref[28];
     __shared__ double seg_sh[27][3];
-    static_assert(IDX_CAP * sizeof(uint32_t) <= BUILD_TILE * sizeof(double4), "index staging fits the tile");
+    static_assert(IDX_CAP * sizeof(uint32_t) <= BUILD_TILE * sizeof(BuildCand), "index staging fits the tile");
     uint32_t* cand = reinterpret_cast<uint32_t*>(tile);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (b0 == 0 && tid == 0) flags[3] += 1u;  // rebuilds done on the device
@@ -1491,6 +1499,11 @@ __device__ __forceinline__ void build_cells(
         if (flags[0] == 0u && bt.rflags[4 * rep + 3] == 0u) continue;
         cutc = bt.ibox[rep].w;
     }
+    // FP32 test with a relative margin of 1e-5 on r^2: every pair within
+    // rc + skin passes (the FP32 error of r^2 on cell-relative coordinates is
+    // below 2e-6 relative), a pair up to 5e-6 (rc + skin) farther may pass too;
+    // it lies beyond rc, where the switched pair law is exactly zero
+    const float cut2f = (float)(cutc * (1.0 + 1e-5));
     __syncthreads();  // the previous cell's LDS readers are done
     const uint32_t nh = min(bcount[c], bcap);
     if (nh == 0) {
@@ -1516,6 +1529,11 @@ __device__ __forceinline__ void build_cells(
         seg_sh[tid][0] = sx;
         seg_sh[tid][1] = sy;
         seg_sh[tid][2] = sz;
+        if (tid == 13) {
+            org[0] = -Q.half[0] + ((double)cx + 0.5) * Q.cell[0];
+            org[1] = -Q.half[1] + ((double)cy + 0.5) * Q.cell[1];
+            org[2] = -Q.half[2] + ((double)cz + 0.5) * Q.cell[2];
+        }
     }
     __syncthreads();
     if (tid == 0) {
@@ -1582,12 +1600,13 @@ __device__ __forceinline__ void build_cells(
             const uint32_t idx = c0 + t;
             const int sg = seg_of(idx);
             const uint32_t j = ord[idx];
-            double4 pj = pos4[j];
-            pj.x += seg_sh[sg][0];
-            pj.y += seg_sh[sg][1];
-            pj.z += seg_sh[sg][2];
-            tile[t] = pj;
-            ttype[t] = type[j];
+            const double4 pj = pos4[j];
+            BuildCand q;
+            q.x = (float)((pj.x + seg_sh[sg][0]) - org[0]);
+            q.y = (float)((pj.y + seg_sh[sg][1]) - org[1]);
+            q.z = (float)((pj.z + seg_sh[sg][2]) - org[2]);
+            q.e = ((uint32_t)type[j] << 28) | j;
+            tile[t] = q;
         }
         __syncthreads();
         // the wave's next home atom is fetched while this one is swept, and
@@ -1597,11 +1616,18 @@ __device__ __forceinline__ void build_cells(
         // its position comes from the tile when the segment lies in this
         // chunk, which it does whenever the stencil fits one chunk)
         const bool home_in_tile = hs >= c0 && hs + nh <= c0 + len;
-        auto home_pos = [&](uint32_t ii) { return home_in_tile ? tile[hs + ii - c0] : pos4[ord[hs + ii]]; };
-        double4 pi_n = (uint32_t)w < nh ? home_pos((uint32_t)w) : make_double4(0.0, 0.0, 0.0, 0.0);
+        auto home_pos = [&](uint32_t ii) {
+            if (home_in_tile) {
+                const BuildCand q = tile[hs + ii - c0];
+                return make_float3(q.x, q.y, q.z);
+            }
+            const double4 p = pos4[ord[hs + ii]];  // centre cell: no image shift
+            return make_float3((float)(p.x - org[0]), (float)(p.y - org[1]), (float)(p.z - org[2]));
+        };
+        float3 pi_n = (uint32_t)w < nh ? home_pos((uint32_t)w) : make_float3(0.f, 0.f, 0.f);
         for (uint32_t i = w; i < nh; i += NWV) {
             const uint32_t gi = ord[hs + i];
-            const double4 pi = pi_n;
+            const float3 pi = pi_n;
             if (i + NWV < nh) pi_n = home_pos(i + NWV);
             uint32_t cnt = ch == 0 ? 0u : (nh <= HOME_CAP ? hcnt[i] : nlen[gi]);
             // 256 candidates per step, four per lane (independent distance
@@ -1614,12 +1640,12 @@ __device__ __forceinline__ void build_cells(
                 for (int q = 0; q < 4; ++q) {
                     const uint32_t j = base + (uint32_t)(q * WAVE + lane);
                     const uint32_t jc = j < len ? j : len - 1u;
-                    const double4 pj = tile[jc];
-                    const uint32_t jg = ord[c0 + jc];
-                    e[q] = LOCAL ? ((uint32_t)ttype[jc] << CELL_IDX_BITS) | (c0 + jc) : ((uint32_t)ttype[jc] << 28) | jg;
-                    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                    in[q] = j < len && r2 < cutc && jg != gi;
+                    const BuildCand cj = tile[jc];
+                    const uint32_t jg = cj.e & NBR_IDX_MASK;
+                    e[q] = LOCAL ? ((cj.e >> 28) << CELL_IDX_BITS) | (c0 + jc) : cj.e;
+                    const float dx = pi.x - cj.x, dy = pi.y - cj.y, dz = pi.z - cj.z;
+                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    in[q] = j < len && r2 < cut2f && jg != gi;
                 }
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
