@@ -39,7 +39,7 @@ FLOP_PER_PAIR = 82.0        # LJ + quintic switch + accumulate (70) + Coulomb (1
 # device copy (FETCH_SIZE 131084.5 KiB -> x2.0, WRITE_SIZE 262144 KiB -> x1.0);
 # profiles/k_pair_list__mixture32768_pmc.md holds the runs.
 PAIR_TRAFFIC_KIB = {"fetch": 12433.3, "write": 1560.3, "fetch_scale": 2.0, "write_scale": 1.0}
-DEFAULT_LANES, DEFAULT_SKIN_FRAC = 2, 0.12
+DEFAULT_LANES, DEFAULT_SKIN_FRAC = 2, 0.10
 
 
 def log(msg):

@@ -88,6 +88,7 @@ _SIGNATURES = {
     "gs_wrap": (ctypes.c_int, [_H]),
     "gs_thermostat": (ctypes.c_int, [_H, _D]),
     "gs_kinetic": (ctypes.c_int, [_H, _D]),
+    "gs_speed_stats": (ctypes.c_int, [_H, _D, _D]),
     "gs_step": (ctypes.c_int, [_H, ctypes.c_int64, ctypes.POINTER(Observables)]),
     "gs_radial_distribution": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_int32, _U64, _D, _D, _D]),
     "gs_radial_distribution_between": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,

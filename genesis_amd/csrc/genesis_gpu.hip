@@ -2314,6 +2314,36 @@ __global__ __launch_bounds__(CELL_THREADS) __attribute__((amdgpu_waves_per_eu(5)
 // (Vec3::length, left to right); speeds >= max_speed are overflow, the
 // others fall in bin min((s / bin_width) as usize, nbins - 1).  Integer
 // counts, so the order of the atomic adds does not matter.
+// StatsSystem's per-entity sums (src/systems/aggregate.rs:87-93): kinetic
+// 0.5 m |v|^2 and speed |v| = sqrt(|v|^2), |v|^2 = x x + y y + z z left to
+// right (Vec3::length_squared), one partial of each per workgroup in a fixed
+// order; k_reduce_to finishes both.
+__global__ __launch_bounds__(BLOCK) void k_speed_parts(int n, const double* __restrict__ vx,
+                                                      const double* __restrict__ vy, const double* __restrict__ vz,
+                                                      const uint8_t* __restrict__ type, KParams P,
+                                                      double* __restrict__ kpart, double* __restrict__ spart) {
+    __shared__ double rk[BLOCK / WAVE], rs[BLOCK / WAVE];
+    const int tid = threadIdx.x, k = blockIdx.x * BLOCK + tid;
+    double ke = 0.0, sp = 0.0;
+    if (k < n) {
+        const double a = vx[k], b = vy[k], c = vz[k];
+        const double l2 = a * a + b * b + c * c;
+        ke = 0.5 * P.mass[type[k]] * l2;
+        sp = sqrt(l2);
+    }
+    ke = wave_sum(ke);
+    sp = wave_sum(sp);
+    if ((tid & 63) == 0) {
+        rk[tid >> 6] = ke;
+        rs[tid >> 6] = sp;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        kpart[blockIdx.x] = (rk[0] + rk[1]) + (rk[2] + rk[3]);
+        spart[blockIdx.x] = (rs[0] + rs[1]) + (rs[2] + rs[3]);
+    }
+}
+
 constexpr int VHIST_MAX_BINS = 512;
 __global__ __launch_bounds__(BLOCK) void k_vhist(int n, const double* __restrict__ vx, const double* __restrict__ vy,
                                                  const double* __restrict__ vz, double max_speed, double bin_width,
@@ -3583,6 +3613,29 @@ int gs_kinetic(gs_universe* u, double* kinetic) {
     HIPCHK(hipMemcpyAsync(&o, u->obs, sizeof(o), hipMemcpyDeviceToHost, u->stream));
     HIPCHK(hipStreamSynchronize(u->stream));
     if (kinetic) *kinetic = o.kinetic;
+    return GS_OK;
+}
+
+int gs_speed_stats(gs_universe* u, double* kinetic, double* speed_sum) {
+    int rc = check_state(u);
+    if (rc) return rc;
+    double out[2] = {0.0, 0.0};
+    if (u->n > 0) {
+        const int c = u->cur;
+        const uint32_t nb = nblocks(u->n);
+        double* kp = u->stage;  // 2 nb partials and the two sums fit the 9 n staging words
+        double* sp = u->stage + nb;
+        double* res = u->stage + 2 * (size_t)nb;
+        k_speed_parts<<<nb, BLOCK, 0, u->stream>>>(u->n, u->v[c][0], u->v[c][1], u->v[c][2], u->type[c], u->kp, kp,
+                                                   sp);
+        k_reduce_to<<<1, BLOCK, 0, u->stream>>>(kp, nb, res);
+        k_reduce_to<<<1, BLOCK, 0, u->stream>>>(sp, nb, res + 1);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, res, sizeof(out), hipMemcpyDeviceToHost, u->stream));
+        HIPCHK(hipStreamSynchronize(u->stream));
+    }
+    if (kinetic) *kinetic = out[0];
+    if (speed_sum) *speed_sum = out[1];
     return GS_OK;
 }
 

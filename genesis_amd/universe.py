@@ -116,6 +116,12 @@ class GpuUniverse:
         _lib.check(self.lib.gs_kinetic(self.h, ctypes.byref(k)))
         return k.value
 
+    def speed_stats(self) -> Tuple[float, float]:
+        """(sum 0.5 m |v|^2, sum |v|) over all entities, reduced on the device."""
+        k, sp = ctypes.c_double(), ctypes.c_double()
+        _lib.check(self.lib.gs_speed_stats(self.h, ctypes.byref(k), ctypes.byref(sp)))
+        return k.value, sp.value
+
     def step(self, n_steps: int, observe: bool = True) -> Optional[np.ndarray]:
         """Universe::tick x n_steps on the device; returns (n_steps, 6) observables."""
         if not observe:

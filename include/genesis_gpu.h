@@ -181,6 +181,11 @@ int gs_wrap(gs_universe* u);
 int gs_thermostat(gs_universe* u, double* lambda);
 /* Kinetic energy sum (src/systems/aggregate.rs:86-93). */
 int gs_kinetic(gs_universe* u, double* kinetic);
+/* StatsSystem's reductions over Velocity and Mass (src/systems/aggregate.rs:
+ * 87-93): kinetic = sum 0.5 m |v|^2 and speed_sum = sum |v| over all
+ * entities, each in a fixed order (the reference sums sequentially; the two
+ * agree to rounding).  mean_speed = speed_sum / n. */
+int gs_speed_stats(gs_universe* u, double* kinetic, double* speed_sum);
 
 /* Universe::tick x n_steps (src/universe/mod.rs:141-159) with the schedule of
  * build_schedule (src/universe/mod.rs:355-392): kick, drift, [wrap], forces,

@@ -328,6 +328,20 @@ def velocity_histogram(vx, vy, vz, max_speed, bins):
             "overflow": int(over.sum())}
 
 
+def stats_sums(vx, vy, vz, mass):
+    """StatsSystem's sums over Velocity and Mass (src/systems/aggregate.rs:87-93)
+    in the reference's order, entity after entity: kinetic += (0.5 m) |v|^2
+    and speed_sum += sqrt(|v|^2), |v|^2 = (x^2 + y^2) + z^2
+    (Vec3::length_squared).  np.add.accumulate adds left to right."""
+    vx, vy, vz, mass = (np.asarray(a, dtype=np.float64) for a in (vx, vy, vz, mass))
+    if vx.size == 0:
+        return 0.0, 0.0
+    l2 = (vx * vx + vy * vy) + vz * vz
+    ke = (0.5 * mass) * l2
+    sp = np.sqrt(l2)
+    return float(np.add.accumulate(ke)[-1]), float(np.add.accumulate(sp)[-1])
+
+
 def persistence_ticks(physics, a, b):
     p = physics.to_c()
     return int(lib().or_persistence_ticks(ctypes.byref(p), a, b))
