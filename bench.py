@@ -33,13 +33,13 @@ FP64_PEAK_TFLOPS = 78.6
 STEP_BYTES = 161.0          # x, v, a read+write (144) + species (1) + cell key (8) + charge (8)
 PAIR_BYTES = 57.0           # pair kernel: x, y, z, q, species read (33) + a written (24)
 FLOP_PER_PAIR = 82.0        # LJ + quintic switch + accumulate (70) + Coulomb (12), per ordered pair
-# HBM traffic of one k_pair_list launch of the default setting (2 lanes, skin
-# 0.12 rc) on this workload, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in
+# HBM traffic of one k_pair_list launch of the default setting (4 lanes, skin
+# 0.10 rc) on this workload, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in
 # KiB per dispatch, scaled by what the same counters read for a 256 MiB
 # device copy (FETCH_SIZE 131084.5 KiB -> x2.0, WRITE_SIZE 262144 KiB -> x1.0);
 # profiles/k_pair_list__mixture32768_pmc.md holds the runs.
-PAIR_TRAFFIC_KIB = {"fetch": 12433.3, "write": 1560.3, "fetch_scale": 2.0, "write_scale": 1.0}
-DEFAULT_LANES, DEFAULT_SKIN_FRAC = 2, 0.10
+PAIR_TRAFFIC_KIB = {"fetch": 12218.0, "write": 1584.4, "fetch_scale": 2.0, "write_scale": 1.0}
+DEFAULT_LANES, DEFAULT_SKIN_FRAC = 4, 0.10
 
 
 def log(msg):
