@@ -95,6 +95,7 @@ _SIGNATURES = {
                                                       ctypes.c_int32, _U64, _D, _D, _D]),
     "gs_get_tuning": (ctypes.c_int, [_H, ctypes.POINTER(Tuning)]),
     "gs_set_tuning": (ctypes.c_int, [_H, ctypes.POINTER(Tuning)]),
+    "gs_set_bound_search": (ctypes.c_int, [_H, ctypes.c_int32]),
     "gs_autotune": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.POINTER(Tuning)]),
     "gs_hvp": (ctypes.c_int, [_H, _D, _U8, _D, _D, _D, ctypes.c_double, _D, _D, _D, _D]),
     "gs_adjoint_forward": (ctypes.c_int, [_H, _D, _U8, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,

@@ -1249,7 +1249,8 @@ __global__ __launch_bounds__(BLOCK) void k_cells_entity(KParams P, const uint32_
 //        [3] rebuilds done by the device, [4] a tick needed a rebuild that
 //        its binning had not been requested for (predictive binning),
 //        [5] last-workgroup counter of the pair kernel, [6] a cell bucket
-//        overflowed its capacity;
+//        overflowed its capacity, [7] an atom had more candidate bound
+//        partners than the pair kernel's hit row holds (fused bond search);
 //        kept across calls: [8] some atom is within reach of skin/2 next
 //        tick, [9] bin (and rebuild) this tick, [10] this tick binned for
 //        a build one tick behind (k_list_step), [11] build this tick from
@@ -1778,6 +1779,22 @@ __device__ __forceinline__ void close_list_tick(uint32_t* flags) {
     flags[10] = 0u;
 }
 
+// Bound-pair search fused into the list pass (bond observation on the list
+// path): the list of an atom holds every pair within rc, so with the bond
+// cutoff k_bind max(sigma) <= rc the pair kernel finds the candidates of
+// bound_pairs_with_grid (src/analysis/pairs.rs:62-85) among the entries it
+// evaluates anyway.
+constexpr int BOUND_MAXT = 32;  // bound partners kept per owner (the tracker's episodes)
+constexpr int HIT_CAP = 32;     // screened candidates an atom's LDS hit row holds
+struct BoundArgs {
+    const uint32_t* id = nullptr;        // entity id of each resident atom
+    const double* thr = nullptr;         // k_bind sigma_ab, [a * 10 + b]
+    uint32_t* nowp = nullptr;            // partners by entity, slot-major [q * n + e]
+    uint32_t* nown = nullptr;            // partner count of entity e
+    unsigned long long* cnt = nullptr;   // [4]: an owner with more than BOUND_MAXT partners
+    double cd2 = 0.0, cw2 = 0.0;         // cutoff^2 and the screen's cutoff^2 (1 + 1e-12)
+};
+
 // Per-atom list traversal fused with the second half of the tick: T lanes
 // per atom split its list (entries s, s+T, ...) and sum with shuffles; the
 // minimum image is taken per pair (dx - L rint(dx / L)); pairs of the skin
@@ -1792,7 +1809,7 @@ __device__ __forceinline__ void close_list_tick(uint32_t* flags) {
 // pairs are skipped.  The kernel stores a = F/m and the atom's potential
 // energy (half its pair sum) in pe_out; the kick and the per-replica
 // reductions run in the batch's own kernels.
-template <int LOGT, bool COUL, bool SHARED, bool SINGLE, bool GRAV, bool BATCH = false>
+template <int LOGT, bool COUL, bool SHARED, bool SINGLE, bool GRAV, bool BATCH = false, bool BOUND = false>
 __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, const double4* __restrict__ pos4,
                                                      const uint8_t* __restrict__ type,
                                                      const uint32_t* __restrict__ nbr,
@@ -1806,7 +1823,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
                                                      ThermoArgs th, const double* __restrict__ abond,
                                                      const double* __restrict__ vbpart, uint32_t nvb, uint32_t rpc,
                                                      uint32_t* __restrict__ bcount, uint32_t nbucket,
-                                                     BatchTabs bt = BatchTabs{}, double* __restrict__ pe_out = nullptr) {
+                                                     BatchTabs bt = BatchTabs{}, double* __restrict__ pe_out = nullptr,
+                                                     BoundArgs ba = BoundArgs{}) {
     constexpr int T = 1 << LOGT;
     __shared__ PairCoef sc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];
     __shared__ SpCoef ssp[GS_NSPECIES];
@@ -1815,7 +1833,11 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     __shared__ double wk[BLOCK / WAVE];
     __shared__ double red[2 * BLOCK / WAVE];
     __shared__ bool last;
+    // BOUND: screened bound-pair candidates (list entries) of each atom
+    __shared__ uint32_t hrow[BOUND ? (BLOCK >> LOGT) : 1][BOUND ? HIT_CAP : 1];
+    __shared__ uint32_t hcnt[BOUND ? (BLOCK >> LOGT) : 1];
     const int tid = threadIdx.x;
+    if (BOUND && tid < (BLOCK >> LOGT)) hcnt[tid] = 0u;
     // Atom chunk of this workgroup.  Workgroup b runs on L2 slice b mod 8
     // and, when the grid holds rpc workgroups per CU of a 256-CU device,
     // shares its CU with b + 256, b + 512, ...: those get adjacent chunks
@@ -1907,6 +1929,25 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
         const uint32_t ng = (cnt + 3) >> 2;  // groups of this lane
         const uint4* glast = grp + (size_t)(ng ? ng - 1 : 0) * gstride;
         auto group_ptr = [&](uint32_t k) { return k < ng ? grp + (size_t)k * gstride : glast; };
+        const uint32_t ha = (uint32_t)tid >> LOGT;  // the atom's hit row
+        // BOUND: an entry within the bond cutoff (the screen of
+        // bound_pairs_with_grid on the r^2 the pair law uses) joins the row
+        auto screen = [&](const double4 (&p)[4], uint32_t tt, const uint4& ee) {
+            const uint32_t f4[4] = {ee.x, ee.y, ee.z, ee.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const bool live = ((tt >> (8 * q)) & 0xFFu) != 0u;
+                double dx = pi.x - p[q].x, dy = pi.y - p[q].y, dz = pi.z - p[q].z;
+                dx = fma(-Lx, rint(dx * iLx), dx);
+                dy = fma(-Ly, rint(dy * iLy), dy);
+                dz = fma(-Lz, rint(dz * iLz), dz);
+                const double t = fma(dz, dz, fma(dy, dy, dx * dx));
+                if (live && t < ba.cw2) {
+                    const uint32_t slot = atomicAdd(&hcnt[ha], 1u);
+                    if (slot < (uint32_t)HIT_CAP) hrow[ha][slot] = f4[q];
+                }
+            }
+        };
         auto gather = [&](const uint4& e, uint32_t k, double4 (&p)[4], uint32_t& tt) {
             const uint32_t f4[4] = {e.x, e.y, e.z, e.w};
             tt = 0u;
@@ -1950,10 +1991,12 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
         uint4 eB = load_nt(group_ptr(1));
         double4 pA[4], pB[4];
         uint32_t tA = 0u, tB = 0u;
+        uint4 jA = make_uint4(0u, 0u, 0u, 0u), jB = jA;  // BOUND: a group's entries, kept for its screen
         // group 1's entries are issued before group 0's gathers, the order
         // every later step keeps
         __builtin_amdgcn_sched_barrier(0);
         gather(eA, 0, pA, tA);
+        if (BOUND) jA = eA;
         // no exit between a step's loads and the next step (the compiler
         // would sink the loads past it): a lane with an odd number of
         // groups evaluates one group of four masked pairs at its end
@@ -1962,14 +2005,18 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
             __builtin_amdgcn_sched_barrier(0);
             eA = load_nt(group_ptr(k + 2));
             gather(eB, k + 1, pB, tB);
+            if (BOUND) jB = eB;
             __builtin_amdgcn_sched_barrier(0);
             evaluate(pA, tA);
+            if (BOUND) screen(pA, tA, jA);
             // step B: group k + 1
             __builtin_amdgcn_sched_barrier(0);
             eB = load_nt(group_ptr(k + 3));
             gather(eA, k + 2, pA, tA);
+            if (BOUND) jA = eA;
             __builtin_amdgcn_sched_barrier(0);
             evaluate(pB, tB);
+            if (BOUND) screen(pB, tB, jB);
         }
     }
 #pragma unroll
@@ -1978,6 +2025,57 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
         fy += __shfl_xor(fy, off, WAVE);
         fz += __shfl_xor(fz, off, WAVE);
         if (BATCH) vl += __shfl_xor(vl, off, WAVE);
+    }
+    if (BOUND) {
+        // Exact test of bound_pairs_with_grid on the screened candidates (the
+        // reference's min_image and length): a larger entity id, d^2 <
+        // cutoff^2, d > EPS, d < k_bind sigma_ab.  The T lanes of an atom
+        // test its row, then one lane orders the partners by entity id and
+        // writes them slot-major like the aux-grid search.
+        __syncthreads();
+        const uint32_t ha = (uint32_t)tid >> LOGT;
+        uint32_t e_i = 0u, nh = 0u;
+        bool over = false;
+        if (i < N) {
+            e_i = ba.id[i];
+            const uint32_t hc = hcnt[ha];
+            over = hc > (uint32_t)HIT_CAP;
+            nh = min(hc, (uint32_t)HIT_CAP);
+            for (uint32_t h = s; h < nh; h += T) {
+                const uint32_t f = hrow[ha][h];
+                const uint32_t j = f & NBR_IDX_MASK;
+                const double4 q = pos4[j];
+                const uint32_t em = ba.id[j];
+                const double dx = min_image1(pi0.x - q.x, P.box[0]);
+                const double dy = min_image1(pi0.y - q.y, P.box[1]);
+                const double dz = min_image1(pi0.z - q.z, P.box[2]);
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                const double d = sqrt(d2);
+                const bool ok = em > e_i && d2 < ba.cd2 && d > F64_EPS && d < ba.thr[ti0 * GS_NSPECIES + (f >> 28)];
+                hrow[ha][h] = ok ? em : 0xFFFFFFFFu;
+            }
+        }
+        __syncthreads();
+        if (i < N && s == 0) {
+            if (over) atomicOr(&flags[7], 1u);
+            // insertion in ascending order, compacting the accepted ids to the front
+            uint32_t m = 0u;
+            for (uint32_t h = 0; h < nh; ++h) {
+                const uint32_t v = hrow[ha][h];
+                if (v == 0xFFFFFFFFu) continue;
+                uint32_t w = m;
+                while (w > 0u && hrow[ha][w - 1] > v) {
+                    hrow[ha][w] = hrow[ha][w - 1];
+                    --w;
+                }
+                hrow[ha][w] = v;
+                ++m;
+            }
+            if (m > (uint32_t)BOUND_MAXT) atomicOr(ba.cnt + 4, 1ull);
+            const uint32_t mm = min(m, (uint32_t)BOUND_MAXT);
+            ba.nown[e_i] = mm;
+            for (uint32_t q = 0; q < mm; ++q) ba.nowp[(size_t)q * N + e_i] = hrow[ha][q];
+        }
     }
     if (BATCH) {
         if (own && s == 0) {
@@ -2728,13 +2826,20 @@ struct gs_universe {
     int timing_every = 1;  // time every k-th tick of a call
     std::vector<hipEvent_t> ev;
     gs_timing tm = {0, 0.0, 0.0, 0.0, 0.0, 0};
+    // bound-pair search fused into the list pass (bond observation)
+    bool bound_fused = false;    // this attempt of a gs_step call uses it
+    bool bound_unfused = false;  // a hit row overflowed: the call replays on the aux grid
+    bool bound_aux_only = false;  // gs_set_bound_search(u, 1)
+    double bound_cd2 = 0.0, bound_cw2 = 0.0;
 };
 
 namespace {
 
 // observed bonds (gs_bonds.inc)
 int bond_force_pass(gs_universe* u, bool accumulate);
-int bond_observe_pass(gs_universe* u, uint64_t now);
+int bond_observe_pass(gs_universe* u, uint64_t now, bool fused = false);
+int bond_fused_prepare(gs_universe* u, bool list);
+BoundArgs bond_fused_args(gs_universe* u);
 int bond_call_begin(gs_universe* u);
 int bond_call_restore(gs_universe* u, bool existed);
 int bond_refresh_species(gs_universe* u, const uint8_t* species);
@@ -3082,7 +3187,9 @@ int snapshot(gs_universe* u, bool restore) {
 
 // List pair pass of one tick, fused with the second kick, the kinetic
 // reduction and the Berendsen lambda; obs (device) receives the observables.
-int launch_pair_list(gs_universe* u, gs_observables* obs, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+// bound (may be NULL): the fused bound-pair search, for 2 or 4 lanes per atom.
+int launch_pair_list(gs_universe* u, gs_observables* obs, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
+                     const BoundArgs* bound = nullptr) {
     const KParams& P = u->kp;
     const gs_params& p = u->prm;
     const int c = u->cur;
@@ -3092,21 +3199,27 @@ int launch_pair_list(gs_universe* u, gs_observables* obs, hipEvent_t e0 = nullpt
     const uint32_t rpc = (u->n_cu == 256 && nb % 256u == 0u) ? nb / 256u : 0u;
     const ThermoArgs th{p.enable_thermostat != 0 ? 1 : 0, p.thermostat_temperature, std::fmax(p.thermostat_tau, 1.0),
                         p.thermal_constant};
-#define GS_PL(L, C, S, SG, G)                                                                                 \
-    launch_timed(k_pair_list<L, C, S, SG, G>, dim3(nb), dim3(BLOCK), u->stream, e0, e1, P, N,                       \
+#define GS_PL(L, C, S, SG, G, BD)                                                                             \
+    launch_timed(k_pair_list<L, C, S, SG, G, false, BD>, dim3(nb), dim3(BLOCK), u->stream, e0, e1, P, N,            \
                  (const double4*)u->pos4[c], (const uint8_t*)u->type[c], (const uint32_t*)u->nbr,               \
                  (const uint32_t*)u->nlen, (const PairCoef*)u->coef, u->a[0], u->a[1], u->a[2], u->v[c][0],      \
                  u->v[c][1], u->v[c][2], u->vpart, u->kpart, u->flags, u->scal, obs, th,                          \
                  (const double*)u->ab_live, (const double*)u->vb_live, u->nvb_live, rpc, u->bcount, u->kpl.ncell, \
-                 BatchTabs{}, (double*)nullptr)
-#define GS_PL_T(C, S, SG, G)                   \
-    switch (u->logT) {                         \
-        case 0: GS_PL(0, C, S, SG, G); break;  \
-        case 1: GS_PL(1, C, S, SG, G); break;  \
-        case 2: GS_PL(2, C, S, SG, G); break;  \
-        case 3: GS_PL(3, C, S, SG, G); break;  \
-        default: GS_PL(4, C, S, SG, G); break; \
-    }
+                 BatchTabs{}, (double*)nullptr, bound ? *bound : BoundArgs{})
+#define GS_PL_T(C, S, SG, G)                                  \
+    if (bound) {                                              \
+        if (u->logT == 1)                                     \
+            GS_PL(1, C, S, SG, G, true);                      \
+        else                                                  \
+            GS_PL(2, C, S, SG, G, true);                      \
+    } else                                                    \
+        switch (u->logT) {                                    \
+            case 0: GS_PL(0, C, S, SG, G, false); break;      \
+            case 1: GS_PL(1, C, S, SG, G, false); break;      \
+            case 2: GS_PL(2, C, S, SG, G, false); break;      \
+            case 3: GS_PL(3, C, S, SG, G, false); break;      \
+            default: GS_PL(4, C, S, SG, G, false); break;     \
+        }
     const bool grav = u->prm.enable_gravity != 0;
     if (u->single) {
         GS_PL_T(false, false, true, false);
@@ -3238,12 +3351,15 @@ int run_ticks(gs_universe* u, int64_t n_steps, bool observe, bool list) {
     for (int64_t s = 0; s < n_steps; ++s) {
         const int tmode = (s % u->timing_every) == 0 ? u->timing : 0;
         if (tmode > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 0], u->stream));
+        const bool fused = list && u->bound_fused;
         if (list) {
             if ((rc = list_tick_start(u))) return rc;
             if ((rc = bond_force_pass(u, false))) return rc;
             hipEvent_t e0 = tmode ? u->ev[4 * s + 1] : nullptr, e1 = tmode ? u->ev[4 * s + 2] : nullptr;
             gs_observables* o = observe ? u->obs + s : nullptr;
-            if ((rc = u->pair_mode == 2 ? launch_pair_cell(u, o, e0, e1) : launch_pair_list(u, o, e0, e1)))
+            const BoundArgs ba = fused ? bond_fused_args(u) : BoundArgs{};
+            if ((rc = u->pair_mode == 2 ? launch_pair_cell(u, o, e0, e1)
+                                        : launch_pair_list(u, o, e0, e1, fused ? &ba : nullptr)))
                 return rc;
             if (tmode > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
         } else {
@@ -3262,7 +3378,7 @@ int run_ticks(gs_universe* u, int64_t n_steps, bool observe, bool list) {
         // BondObservationSystem runs after the thermostat (build_schedule,
         // src/universe/mod.rs:383-388); its Bonds feed the next tick.
         u->tick += 1;
-        if (p.enable_bond_observation && (rc = bond_observe_pass(u, (uint64_t)u->tick))) return rc;
+        if (p.enable_bond_observation && (rc = bond_observe_pass(u, (uint64_t)u->tick, fused))) return rc;
     }
     HIPCHK(hipGetLastError());
     return GS_OK;
@@ -3666,8 +3782,10 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
         if ((rc = snapshot(u, false))) return rc;
         if ((rc = bond_call_begin(u))) return rc;
     }
+    u->bound_unfused = false;
     for (int attempt = 0;; ++attempt) {
         if (list) HIPCHK(hipMemsetAsync(u->flags, 0, LIST_FLAGS_RESET * sizeof(uint32_t), u->stream));
+        if ((rc = bond_fused_prepare(u, list))) return rc;
         if ((rc = run_ticks(u, n_steps, obs != nullptr, list))) return rc;
         const int c = u->cur;
         k_apply_lambda<<<nblocks(u->n), BLOCK, 0, u->stream>>>(u->n, u->v[c][0], u->v[c][1], u->v[c][2], u->scal);
@@ -3676,13 +3794,14 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
         uint32_t fl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (list) HIPCHK(hipMemcpyAsync(fl, u->flags, sizeof(fl), hipMemcpyDeviceToHost, u->stream));
         HIPCHK(hipStreamSynchronize(u->stream));
-        if (list && (fl[1] || fl[2] || fl[4] || fl[6])) {
+        if (list && (fl[1] || fl[2] || fl[4] || fl[6] || fl[7])) {
             // A list or a cell bucket overflowed its capacity, a stencil
-            // held more candidates than the build stages in LDS, or a tick
-            // needed a rebuild its predictive binning had not foreseen:
-            // replay the call from the snapshot with larger lists or
-            // buckets, on the per-tick cell tiles, or with a binning launch
-            // every tick.
+            // held more candidates than the build stages in LDS, a tick
+            // needed a rebuild its predictive binning had not foreseen, or
+            // an atom had more bound-pair candidates than the pair kernel's
+            // hit row holds: replay the call from the snapshot with larger
+            // lists or buckets, on the per-tick cell tiles, with a binning
+            // launch every tick, or with the aux-grid bound-pair search.
             if (attempt > 8) return fail(GS_E_OVERFLOW, "neighbour list could not be made valid");
             u->list_replays += 1;
             u->cur = cur0;
@@ -3691,6 +3810,7 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
             if ((rc = bond_call_restore(u, bonds0))) return rc;
             u->list_valid = false;
             if (fl[4]) u->prebin = false;
+            if (fl[7]) u->bound_unfused = true;  // the aux-grid bound-pair search for this call
             if (fl[2]) {
                 u->pair_mode = 0;
                 list = false;
@@ -3764,6 +3884,13 @@ int gs_set_tuning(gs_universe* u, const gs_tuning* t) {
     }
     u->pair_mode = t->pair_mode;
     u->list_valid = false;
+    return GS_OK;
+}
+
+int gs_set_bound_search(gs_universe* u, int32_t aux_grid_only) {
+    if (!u) return fail(GS_E_INVALID, "universe handle is NULL");
+    if (aux_grid_only != 0 && aux_grid_only != 1) return fail(GS_E_INVALID, "aux_grid_only must be 0 or 1");
+    u->bound_aux_only = aux_grid_only != 0;
     return GS_OK;
 }
 

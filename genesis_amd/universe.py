@@ -146,6 +146,12 @@ class GpuUniverse:
         t = _lib.Tuning(pair_mode, lanes_per_atom, cur["skin"] if skin is None else skin, sort_every, 0, 0, 0.0)
         _lib.check(self.lib.gs_set_tuning(self.h, ctypes.byref(t)))
 
+    def set_bound_search(self, aux_grid_only: bool) -> None:
+        """Where gs_step finds bond observation's bound pairs: False (default)
+        lets the Verlet-list pass do it when it can, True always uses the aux
+        grid; the results are identical."""
+        _lib.check(self.lib.gs_set_bound_search(self.h, 1 if aux_grid_only else 0))
+
     def autotune(self, trial_steps: int = 40) -> dict:
         t = _lib.Tuning()
         _lib.check(self.lib.gs_autotune(self.h, trial_steps, ctypes.byref(t)))

@@ -197,6 +197,15 @@ int gs_get_tuning(gs_universe* u, gs_tuning* out);
 /* Validates and applies; GS_E_INVALID if the setting is impossible for this
  * box (list mode needs >= 3 cells of edge rc + skin per axis). */
 int gs_set_tuning(gs_universe* u, const gs_tuning* t);
+/* No counterpart in the reference (where gs_step finds the bound pairs of
+ * bond observation).  0 (default): on the Verlet-list path with 2 or 4 lanes
+ * per atom and k_bind max(sigma) <= rc, the pair kernel screens its list
+ * entries against the bind cutoff and applies bound_pairs_with_grid's exact
+ * test to the hits (src/analysis/pairs.rs:62-85); otherwise, and in a call
+ * replayed because an atom had more candidates than that search holds, the
+ * aux-grid search runs.  1: always the aux-grid search.  Results are
+ * identical either way. */
+int gs_set_bound_search(gs_universe* u, int32_t aux_grid_only);
 /* Project autotuner: times trial_steps ticks of every candidate setting on
  * the resident state, restores the state after each trial and keeps the
  * fastest.  best (may be NULL) receives it.  The trials observe bonds too;

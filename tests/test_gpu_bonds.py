@@ -232,3 +232,64 @@ def test_replayed_call_restores_the_bond_state(GU):
     np.testing.assert_allclose(got.x, ref.x, rtol=0, atol=1e-8)
     assert [s["tick"], s["bonded_pairs"], s["bonded_entities"], s["bonds_formed"]] == list(bs_ref[-1][:4])
     assert np.array_equal(ptr, rptr) and np.array_equal(idx, ridx)
+
+
+@pytest.mark.parametrize("lanes", [2, 4])
+def test_fused_bound_search_matches_the_aux_grid_search(GU, lanes):
+    # The list pass screens its entries against the bind cutoff and applies
+    # the exact test to the hits; the aux-grid search scans the grid.  With
+    # the bond interaction on, any difference in the bound pairs would move
+    # the trajectory, so everything must agree bit for bit.
+    ph, st = melt(12)
+    out = []
+    for aux in (True, False):
+        with GU(ph, st) as u:
+            u.set_tuning(1, lanes)
+            u.set_bound_search(aux)
+            obs = np.concatenate([u.step(70), u.step(80)])
+            out.append((obs, u.download(), u.bond_stats(), u.bonds(), u.tuning()))
+    (o1, s1, b1, (p1, i1), t1), (o2, s2, b2, (p2, i2), t2) = out
+    say(f"fused vs aux-grid bound search, {lanes} lanes: bonded pairs {b1['bonded_pairs']}")
+    assert b1["bonded_pairs"] > 0 and t2["list_replays"] == 0
+    assert np.array_equal(o1, o2)
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(getattr(s1, f), getattr(s2, f)), f
+    for k in ("tick", "bonded_pairs", "bonded_entities", "bonds_formed", "lifetime_sum_ticks"):
+        assert b1[k] == b2[k], k
+    assert np.array_equal(b1["species_matrix"], b2["species_matrix"])
+    assert np.array_equal(p1, p2) and np.array_equal(i1, i2)
+
+
+def test_dense_cluster_overflows_the_fused_search_and_replays(GU):
+    # 40 atoms inside one bind cutoff: each has more candidates than the
+    # list pass's hit row holds, so the call is replayed with the aux-grid
+    # search and must equal a run that used the aux grid throughout.  At
+    # rest and with a tiny dt nothing moves, so the overflow is the only
+    # reason for a replay.
+    base, _ = melt(8)
+    ph = Physics(**{**base.__dict__, "box": (40.0, 40.0, 40.0), "dt": 1e-7})
+    rng = np.random.default_rng(7)
+    nc, ng = 40, 400
+    cl = rng.uniform(-1.8, 1.8, (nc, 3))
+    gas = rng.uniform(-20.0, 20.0, (ng, 3))
+    gas = gas[np.linalg.norm(gas, axis=1) > 8.0]
+    p = np.concatenate([cl, gas])
+    n = len(p)
+    species = np.where(np.arange(n) % 2 == 0, 5, 4).astype(np.uint8)  # Na, O
+    charge = np.where(species == 5, 1.0, -1.0)
+    z = np.zeros(n)
+    st = State(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), z, z, z, species, charge)
+    out = []
+    for aux in (True, False):
+        with GU(ph, st) as u:
+            u.set_tuning(1, 4)
+            u.set_bound_search(aux)
+            obs = u.step(2)
+            out.append((obs, u.download(), u.bond_stats(), u.tuning()))
+    (o1, s1, b1, t1), (o2, s2, b2, t2) = out
+    say(f"dense cluster n={n}: replays aux {t1['list_replays']}, fused {t2['list_replays']}")
+    assert t2["pair_mode"] == 1 and t1["list_replays"] == 0 and t2["list_replays"] == 1
+    assert np.array_equal(o1, o2)
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(getattr(s1, f), getattr(s2, f)), f
+    assert b1["tick"] == b2["tick"] and np.array_equal(b1["species_matrix"], b2["species_matrix"])
