@@ -1249,8 +1249,9 @@ __global__ __launch_bounds__(BLOCK) void k_cells_entity(KParams P, const uint32_
 //        [3] rebuilds done by the device, [4] a tick needed a rebuild that
 //        its binning had not been requested for (predictive binning),
 //        [5] last-workgroup counter of the pair kernel, [6] a cell bucket
-//        overflowed its capacity, [7] an atom had more candidate bound
-//        partners than the pair kernel's hit row holds (fused bond search);
+//        overflowed its capacity, [7] a lane of the pair kernel screened
+//        more bound-pair candidates than its LDS column holds (fused bond
+//        search);
 //        kept across calls: [8] some atom is within reach of skin/2 next
 //        tick, [9] bin (and rebuild) this tick, [10] this tick binned for
 //        a build one tick behind (k_list_step), [11] build this tick from
@@ -1785,7 +1786,7 @@ __device__ __forceinline__ void close_list_tick(uint32_t* flags) {
 // bound_pairs_with_grid (src/analysis/pairs.rs:62-85) among the entries it
 // evaluates anyway.
 constexpr int BOUND_MAXT = 32;  // bound partners kept per owner (the tracker's episodes)
-constexpr int HIT_CAP = 32;     // screened candidates an atom's LDS hit row holds
+constexpr int HIT_LANE = 16;    // screened candidates a lane's LDS column holds
 struct BoundArgs {
     const uint32_t* id = nullptr;        // entity id of each resident atom
     const double* thr = nullptr;         // k_bind sigma_ab, [a * 10 + b]
@@ -1793,6 +1794,7 @@ struct BoundArgs {
     uint32_t* nown = nullptr;            // partner count of entity e
     unsigned long long* cnt = nullptr;   // [4]: an owner with more than BOUND_MAXT partners
     double cd2 = 0.0, cw2 = 0.0;         // cutoff^2 and the screen's cutoff^2 (1 + 1e-12)
+    double kbind = 0.0;                  // k_bind: the screen also takes d < k_bind sigma_ab
 };
 
 // Per-atom list traversal fused with the second half of the tick: T lanes
@@ -1833,11 +1835,12 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     __shared__ double wk[BLOCK / WAVE];
     __shared__ double red[2 * BLOCK / WAVE];
     __shared__ bool last;
-    // BOUND: screened bound-pair candidates (list entries) of each atom
-    __shared__ uint32_t hrow[BOUND ? (BLOCK >> LOGT) : 1][BOUND ? HIT_CAP : 1];
-    __shared__ uint32_t hcnt[BOUND ? (BLOCK >> LOGT) : 1];
+    // BOUND: each lane's screened bound-pair candidates (list entries), a
+    // column per lane, then its accepted partners' entity ids
+    __shared__ uint32_t hl[BOUND ? HIT_LANE : 1][BOUND ? BLOCK : 1];
+    __shared__ uint32_t hn[BOUND ? BLOCK : 1];
     const int tid = threadIdx.x;
-    if (BOUND && tid < (BLOCK >> LOGT)) hcnt[tid] = 0u;
+    uint32_t nhit = 0u;  // BOUND: screened candidates of this lane
     // Atom chunk of this workgroup.  Workgroup b runs on L2 slice b mod 8
     // and, when the grid holds rpc workgroups per CU of a 256-CU device,
     // shares its CU with b + 256, b + 512, ...: those get adjacent chunks
@@ -1929,9 +1932,9 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
         const uint32_t ng = (cnt + 3) >> 2;  // groups of this lane
         const uint4* glast = grp + (size_t)(ng ? ng - 1 : 0) * gstride;
         auto group_ptr = [&](uint32_t k) { return k < ng ? grp + (size_t)k * gstride : glast; };
-        const uint32_t ha = (uint32_t)tid >> LOGT;  // the atom's hit row
         // BOUND: an entry within the bond cutoff (the screen of
-        // bound_pairs_with_grid on the r^2 the pair law uses) joins the row
+        // bound_pairs_with_grid on the r^2 the pair law uses) joins the
+        // lane's column
         auto screen = [&](const double4 (&p)[4], uint32_t tt, const uint4& ee) {
             const uint32_t f4[4] = {ee.x, ee.y, ee.z, ee.w};
 #pragma unroll
@@ -1942,9 +1945,17 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
                 dy = fma(-Ly, rint(dy * iLy), dy);
                 dz = fma(-Lz, rint(dz * iLz), dz);
                 const double t = fma(dz, dz, fma(dy, dy, dx * dx));
-                if (live && t < ba.cw2) {
-                    const uint32_t slot = atomicAdd(&hcnt[ha], 1u);
-                    if (slot < (uint32_t)HIT_CAP) hrow[ha][slot] = f4[q];
+                bool hit = live && t < ba.cw2;
+                if (!SINGLE) {
+                    // the pair's own threshold, from the mixing rule's sigma_ij
+                    // (1e-10 wider, so the exact test decides every pair it keeps)
+                    const uint32_t tq = (tt >> (8 * q)) & 0xFFu;
+                    const double th = ba.kbind * (0.5 * (sig_i + ssp[tq ? tq - 1u : 0u].sigma));
+                    hit = hit && t < th * th * (1.0 + 1e-10);
+                }
+                if (hit) {
+                    if (nhit < (uint32_t)HIT_LANE) hl[nhit][tid] = f4[q];
+                    ++nhit;
                 }
             }
         };
@@ -2029,20 +2040,17 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     if (BOUND) {
         // Exact test of bound_pairs_with_grid on the screened candidates (the
         // reference's min_image and length): a larger entity id, d^2 <
-        // cutoff^2, d > EPS, d < k_bind sigma_ab.  The T lanes of an atom
-        // test its row, then one lane orders the partners by entity id and
-        // writes them slot-major like the aux-grid search.
-        __syncthreads();
-        const uint32_t ha = (uint32_t)tid >> LOGT;
-        uint32_t e_i = 0u, nh = 0u;
-        bool over = false;
+        // cutoff^2, d > EPS, d < k_bind sigma_ab.  Each lane tests its own
+        // column and keeps the accepted ids in it; then every accepted id
+        // takes its rank among the atom's accepted ids (all distinct) and is
+        // written to that slot, slot-major as the aux-grid search writes
+        // them, so the partners come out in ascending id order.
+        uint32_t e_i = 0u, nacc = 0u;
         if (i < N) {
             e_i = ba.id[i];
-            const uint32_t hc = hcnt[ha];
-            over = hc > (uint32_t)HIT_CAP;
-            nh = min(hc, (uint32_t)HIT_CAP);
-            for (uint32_t h = s; h < nh; h += T) {
-                const uint32_t f = hrow[ha][h];
+            const uint32_t nh = min(nhit, (uint32_t)HIT_LANE);
+            for (uint32_t h = 0; h < nh; ++h) {
+                const uint32_t f = hl[h][tid];
                 const uint32_t j = f & NBR_IDX_MASK;
                 const double4 q = pos4[j];
                 const uint32_t em = ba.id[j];
@@ -2051,30 +2059,33 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
                 const double dz = min_image1(pi0.z - q.z, P.box[2]);
                 const double d2 = dx * dx + dy * dy + dz * dz;
                 const double d = sqrt(d2);
-                const bool ok = em > e_i && d2 < ba.cd2 && d > F64_EPS && d < ba.thr[ti0 * GS_NSPECIES + (f >> 28)];
-                hrow[ha][h] = ok ? em : 0xFFFFFFFFu;
-            }
-        }
-        __syncthreads();
-        if (i < N && s == 0) {
-            if (over) atomicOr(&flags[7], 1u);
-            // insertion in ascending order, compacting the accepted ids to the front
-            uint32_t m = 0u;
-            for (uint32_t h = 0; h < nh; ++h) {
-                const uint32_t v = hrow[ha][h];
-                if (v == 0xFFFFFFFFu) continue;
-                uint32_t w = m;
-                while (w > 0u && hrow[ha][w - 1] > v) {
-                    hrow[ha][w] = hrow[ha][w - 1];
-                    --w;
+                if (em > e_i && d2 < ba.cd2 && d > F64_EPS && d < ba.thr[ti0 * GS_NSPECIES + (f >> 28)]) {
+                    hl[nacc][tid] = em;
+                    ++nacc;
                 }
-                hrow[ha][w] = v;
-                ++m;
             }
-            if (m > (uint32_t)BOUND_MAXT) atomicOr(ba.cnt + 4, 1ull);
-            const uint32_t mm = min(m, (uint32_t)BOUND_MAXT);
-            ba.nown[e_i] = mm;
-            for (uint32_t q = 0; q < mm; ++q) ba.nowp[(size_t)q * N + e_i] = hrow[ha][q];
+            if (nhit > (uint32_t)HIT_LANE) atomicOr(&flags[7], 1u);
+        }
+        hn[tid] = nacc;
+        __syncthreads();
+        if (i < N) {
+            const int first = tid & ~(T - 1);  // the atom's first lane
+            uint32_t m = 0u;
+#pragma unroll
+            for (int l = 0; l < T; ++l) m += hn[first + l];
+            for (uint32_t k = 0; k < nacc; ++k) {
+                const uint32_t v = hl[k][tid];
+                uint32_t r = 0u;
+                for (int l = 0; l < T; ++l) {
+                    const uint32_t c = hn[first + l];
+                    for (uint32_t kk = 0; kk < c; ++kk) r += hl[kk][first + l] < v ? 1u : 0u;
+                }
+                if (r < (uint32_t)BOUND_MAXT) ba.nowp[(size_t)r * N + e_i] = v;
+            }
+            if (s == 0) {
+                ba.nown[e_i] = min(m, (uint32_t)BOUND_MAXT);
+                if (m > (uint32_t)BOUND_MAXT) atomicOr(ba.cnt + 4, 1ull);
+            }
         }
     }
     if (BATCH) {
@@ -2828,7 +2839,7 @@ struct gs_universe {
     gs_timing tm = {0, 0.0, 0.0, 0.0, 0.0, 0};
     // bound-pair search fused into the list pass (bond observation)
     bool bound_fused = false;    // this attempt of a gs_step call uses it
-    bool bound_unfused = false;  // a hit row overflowed: the call replays on the aux grid
+    bool bound_unfused = false;  // a hit column overflowed: the call replays on the aux grid
     bool bound_aux_only = false;  // gs_set_bound_search(u, 1)
     double bound_cd2 = 0.0, bound_cw2 = 0.0;
 };
@@ -3798,8 +3809,8 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
             // A list or a cell bucket overflowed its capacity, a stencil
             // held more candidates than the build stages in LDS, a tick
             // needed a rebuild its predictive binning had not foreseen, or
-            // an atom had more bound-pair candidates than the pair kernel's
-            // hit row holds: replay the call from the snapshot with larger
+            // a lane of the pair kernel screened more bound-pair candidates
+            // than its LDS column holds: replay the call from the snapshot with larger
             // lists or buckets, on the per-tick cell tiles, with a binning
             // launch every tick, or with the aux-grid bound-pair search.
             if (attempt > 8) return fail(GS_E_OVERFLOW, "neighbour list could not be made valid");

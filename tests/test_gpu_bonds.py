@@ -261,15 +261,16 @@ def test_fused_bound_search_matches_the_aux_grid_search(GU, lanes):
 
 
 def test_dense_cluster_overflows_the_fused_search_and_replays(GU):
-    # 40 atoms inside one bind cutoff: each has more candidates than the
-    # list pass's hit row holds, so the call is replayed with the aux-grid
-    # search and must equal a run that used the aux grid throughout.  At
+    # 48 atoms inside one bind cutoff: with 2 lanes per atom a lane screens
+    # more candidates than its LDS column holds, so the call is replayed with
+    # the aux-grid search and must equal a run that used the aux grid
+    # throughout.  At
     # rest and with a tiny dt nothing moves, so the overflow is the only
     # reason for a replay.
     base, _ = melt(8)
     ph = Physics(**{**base.__dict__, "box": (40.0, 40.0, 40.0), "dt": 1e-7})
     rng = np.random.default_rng(7)
-    nc, ng = 40, 400
+    nc, ng = 48, 400
     cl = rng.uniform(-1.8, 1.8, (nc, 3))
     gas = rng.uniform(-20.0, 20.0, (ng, 3))
     gas = gas[np.linalg.norm(gas, axis=1) > 8.0]
@@ -282,7 +283,7 @@ def test_dense_cluster_overflows_the_fused_search_and_replays(GU):
     out = []
     for aux in (True, False):
         with GU(ph, st) as u:
-            u.set_tuning(1, 4)
+            u.set_tuning(1, 2)
             u.set_bound_search(aux)
             obs = u.step(2)
             out.append((obs, u.download(), u.bond_stats(), u.tuning()))
