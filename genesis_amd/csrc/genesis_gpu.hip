@@ -2049,19 +2049,32 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
         if (i < N) {
             e_i = ba.id[i];
             const uint32_t nh = min(nhit, (uint32_t)HIT_LANE);
-            for (uint32_t h = 0; h < nh; ++h) {
-                const uint32_t f = hl[h][tid];
-                const uint32_t j = f & NBR_IDX_MASK;
-                const double4 q = pos4[j];
-                const uint32_t em = ba.id[j];
-                const double dx = min_image1(pi0.x - q.x, P.box[0]);
-                const double dy = min_image1(pi0.y - q.y, P.box[1]);
-                const double dz = min_image1(pi0.z - q.z, P.box[2]);
-                const double d2 = dx * dx + dy * dy + dz * dz;
-                const double d = sqrt(d2);
-                if (em > e_i && d2 < ba.cd2 && d > F64_EPS && d < ba.thr[ti0 * GS_NSPECIES + (f >> 28)]) {
-                    hl[nacc][tid] = em;
-                    ++nacc;
+            // four hits at a time: their gathers are in flight together (an
+            // accepted id is written at or before its own slot, never ahead
+            // of the batch being read)
+            for (uint32_t h0 = 0; h0 < nh; h0 += 4) {
+                uint32_t f[4], em[4];
+                double4 q[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) f[k] = hl[min(h0 + (uint32_t)k, nh - 1u)][tid];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t j = f[k] & NBR_IDX_MASK;
+                    q[k] = pos4[j];
+                    em[k] = ba.id[j];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const double dx = min_image1(pi0.x - q[k].x, P.box[0]);
+                    const double dy = min_image1(pi0.y - q[k].y, P.box[1]);
+                    const double dz = min_image1(pi0.z - q[k].z, P.box[2]);
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    const double d = sqrt(d2);
+                    if (h0 + (uint32_t)k < nh && em[k] > e_i && d2 < ba.cd2 && d > F64_EPS &&
+                        d < ba.thr[ti0 * GS_NSPECIES + (f[k] >> 28)]) {
+                        hl[nacc][tid] = em[k];
+                        ++nacc;
+                    }
                 }
             }
             if (nhit > (uint32_t)HIT_LANE) atomicOr(&flags[7], 1u);
