@@ -1,6 +1,6 @@
 """HIP path vs the oracle on identical seeded inputs, through the C ABI.
 
-Tolerances (written here, as the bar the north star sets):
+Tolerances (the bar this path is held to, written here):
   * per-atom acceleration: |a_gpu - a_cpu|_inf <= 1e-10 max(|a_cpu|_inf, S_i/m_i)
     with S_i = sum_j |f_ij| from the oracle;
   * total potential / kinetic energy and per-atom PE: 1e-10 relative;
