@@ -2867,6 +2867,7 @@ BoundArgs bond_fused_args(gs_universe* u);
 int bond_call_begin(gs_universe* u);
 int bond_call_restore(gs_universe* u, bool existed);
 int bond_refresh_species(gs_universe* u, const uint8_t* species);
+int bond_overflow_fetch(gs_universe* u, unsigned long long* host_word);
 
 int validate_params(const gs_params* p, int64_t n) {
     if (!p) return fail(GS_E_INVALID, "params is NULL");
@@ -3374,22 +3375,23 @@ int run_ticks(gs_universe* u, int64_t n_steps, bool observe, bool list) {
     const bool thermo = p.enable_thermostat != 0;
     for (int64_t s = 0; s < n_steps; ++s) {
         const int tmode = (s % u->timing_every) == 0 ? u->timing : 0;
-        if (tmode > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 0], u->stream));
+        const int64_t te = 4 * (s / u->timing_every);  // this timed tick's four events
+        if (tmode > 1) HIPCHK(hipEventRecord(u->ev[te + 0], u->stream));
         const bool fused = list && u->bound_fused;
         if (list) {
             if ((rc = list_tick_start(u))) return rc;
             if ((rc = bond_force_pass(u, false))) return rc;
-            hipEvent_t e0 = tmode ? u->ev[4 * s + 1] : nullptr, e1 = tmode ? u->ev[4 * s + 2] : nullptr;
+            hipEvent_t e0 = tmode ? u->ev[te + 1] : nullptr, e1 = tmode ? u->ev[te + 2] : nullptr;
             gs_observables* o = observe ? u->obs + s : nullptr;
             const BoundArgs ba = fused ? bond_fused_args(u) : BoundArgs{};
             if ((rc = u->pair_mode == 2 ? launch_pair_cell(u, o, e0, e1)
                                         : launch_pair_list(u, o, e0, e1, fused ? &ba : nullptr)))
                 return rc;
-            if (tmode > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
+            if (tmode > 1) HIPCHK(hipEventRecord(u->ev[te + 3], u->stream));
         } else {
             if ((rc = rebuild(u, true, u->kp))) return rc;
             u->list_valid = false;
-            if ((rc = launch_pair<false>(u, tmode ? u->ev[4 * s + 1] : nullptr, tmode ? u->ev[4 * s + 2] : nullptr)))
+            if ((rc = launch_pair<false>(u, tmode ? u->ev[te + 1] : nullptr, tmode ? u->ev[te + 2] : nullptr)))
                 return rc;
             if ((rc = bond_force_pass(u, true))) return rc;
             const int c = u->cur;
@@ -3397,7 +3399,7 @@ int run_ticks(gs_universe* u, int64_t n_steps, bool observe, bool list) {
                 u->kp, u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->type[c], u->kpart,
                 u->counter, u->vpart, u->nvpart, u->vb_live, u->nvb_live, u->scal, observe ? u->obs + s : nullptr,
                 thermo ? 1 : 0, p.thermostat_temperature, std::fmax(p.thermostat_tau, 1.0), p.thermal_constant);
-            if (tmode > 1) HIPCHK(hipEventRecord(u->ev[4 * s + 3], u->stream));
+            if (tmode > 1) HIPCHK(hipEventRecord(u->ev[te + 3], u->stream));
         }
         // BondObservationSystem runs after the thermostat (build_schedule,
         // src/universe/mod.rs:383-388); its Bonds feed the next tick.
@@ -3792,7 +3794,8 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
         u->obs_cap = n_steps;
     }
     bool list = (u->pair_mode == 1 || u->pair_mode == 2) && u->list_ok && !u->small_box;
-    const size_t nev = u->timing ? (size_t)(4 * n_steps) : 0;
+    // four events per timed tick: ticks 0, every, 2 every, ... of the call
+    const size_t nev = u->timing ? (size_t)(4 * ((n_steps + u->timing_every - 1) / u->timing_every)) : 0;
     while (u->ev.size() < nev) {
         hipEvent_t e;
         HIPCHK(hipEventCreate(&e));
@@ -3817,6 +3820,10 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
         HIPCHK(hipGetLastError());
         uint32_t fl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (list) HIPCHK(hipMemcpyAsync(fl, u->flags, sizeof(fl), hipMemcpyDeviceToHost, u->stream));
+        // the bond state's sticky overflow word: an entity with more than 32
+        // bound partners, tracked episodes or bonds
+        unsigned long long bond_ovf = 0;
+        if ((rc = bond_overflow_fetch(u, &bond_ovf))) return rc;
         HIPCHK(hipStreamSynchronize(u->stream));
         if (list && (fl[1] || fl[2] || fl[4] || fl[6] || fl[7])) {
             // A list or a cell bucket overflowed its capacity, a stencil
@@ -3847,6 +3854,21 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
         }
         if (list) u->list_builds += fl[3];
         u->prebin = true;  // a replay without predictive binning holds for one call
+        if (bond_ovf) {
+            // Partners past the capacity were dropped: the observed bonds and
+            // the forces they feed are incomplete.  On the list path the call
+            // returns to its start (state, bonds and tick); on the cell tiles
+            // the state is left at the call's end.
+            if (list) {
+                u->cur = cur0;
+                u->tick = tick0;
+                if ((rc = snapshot(u, true))) return rc;
+                if ((rc = bond_call_restore(u, bonds0))) return rc;
+                u->list_valid = false;
+                HIPCHK(hipStreamSynchronize(u->stream));
+            }
+            return fail(GS_E_OVERFLOW, "more than 32 bound partners, tracked episodes or bonds for one entity");
+        }
         break;
     }
     if (obs)
@@ -3855,12 +3877,13 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
     if (u->timing) {
         int64_t timed = 0;
         for (int64_t s = 0; s < n_steps; s += u->timing_every) {
+            const int64_t te = 4 * (s / u->timing_every);
             float a = 0.f, b = 0.f, c2 = 0.f;
             ++timed;
-            HIPCHK(hipEventElapsedTime(&b, u->ev[4 * s + 1], u->ev[4 * s + 2]));
+            HIPCHK(hipEventElapsedTime(&b, u->ev[te + 1], u->ev[te + 2]));
             if (u->timing > 1) {
-                HIPCHK(hipEventElapsedTime(&a, u->ev[4 * s + 0], u->ev[4 * s + 1]));
-                HIPCHK(hipEventElapsedTime(&c2, u->ev[4 * s + 2], u->ev[4 * s + 3]));
+                HIPCHK(hipEventElapsedTime(&a, u->ev[te + 0], u->ev[te + 1]));
+                HIPCHK(hipEventElapsedTime(&c2, u->ev[te + 2], u->ev[te + 3]));
             }
             u->tm.rebuild_ms += a;
             u->tm.pair_ms += b;
@@ -3936,31 +3959,59 @@ int gs_autotune(gs_universe* u, int32_t trial_steps, gs_tuning* best) {
         for (double f : {0.04, 0.08, 0.12, 0.18})
             for (int T : {0, 2, 4, 8, 16})
                 cand.push_back(gs_tuning{T ? 1 : 2, T, f * u->kp.rc, 0, 0, 0, 0.0});
+    gs_tuning orig;
+    if ((rc = gs_get_tuning(u, &orig))) return rc;
     const int was_timing = u->timing;
     u->timing = 0;
     gs_tuning win{0, 1, 0.0, 0, 0, 0, 1e300};
+    bool any = false;
+    int last_err = GS_OK;
+    std::string last_msg;
     for (gs_tuning c : cand) {
         if (gs_set_tuning(u, &c) != GS_OK) continue;
-        if ((rc = gs_step(u, trial_steps / 2, nullptr))) return rc;  // warm up: first lists, list capacity
-        hipEvent_t e0, e1;
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipEventRecord(e0, u->stream));
-        if ((rc = gs_step(u, trial_steps, nullptr))) return rc;
-        HIPCHK(hipEventRecord(e1, u->stream));
-        HIPCHK(hipEventSynchronize(e1));
+        // a trial that fails (a capacity the candidate cannot meet) is skipped;
+        // the state is restored either way
+        int trc = gs_step(u, trial_steps / 2, nullptr);  // warm up: first lists, list capacity
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        c.ms_per_tick = ms / trial_steps;
-        c.sort_every = u->sort_every;
-        // a list setting that fell back to the cell tiles is not a candidate
-        if (u->pair_mode == c.pair_mode && c.ms_per_tick < win.ms_per_tick) win = c;
+        if (trc == GS_OK) {
+            hipEvent_t e0, e1;
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            HIPCHK(hipEventRecord(e0, u->stream));
+            trc = gs_step(u, trial_steps, nullptr);
+            HIPCHK(hipEventRecord(e1, u->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+        if (trc == GS_E_HIP) {
+            u->timing = was_timing;
+            return trc;
+        }
+        if (trc == GS_OK) {
+            any = true;
+            c.ms_per_tick = ms / trial_steps;
+            c.sort_every = u->sort_every;
+            // a list setting that fell back to the cell tiles is not a candidate
+            if (u->pair_mode == c.pair_mode && c.ms_per_tick < win.ms_per_tick) win = c;
+        } else {
+            last_err = trc;
+            last_msg = g_err;
+            if (u->bonds && (rc = gs_bond_reset(u))) return rc;
+        }
         if ((rc = gs_upload_state(u, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], sp.data(), q.data())))
             return rc;
     }
     u->timing = was_timing;
+    if (!any) {
+        // every trial failed: the universe keeps its state and tuning
+        orig.list_builds = orig.list_replays = 0;
+        if ((rc = gs_set_tuning(u, &orig))) return rc;
+        if (u->bonds && (rc = gs_bond_reset(u))) return rc;
+        u->tick = 0;
+        return fail(last_err, ("autotune: every candidate failed; last: " + last_msg).c_str());
+    }
     if ((rc = gs_set_tuning(u, &win))) return rc;
     // the trials observed bonds on copies of the starting state: start the
     // observation afresh

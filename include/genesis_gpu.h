@@ -190,7 +190,12 @@ int gs_speed_stats(gs_universe* u, double* kinetic, double* speed_sum);
 /* Universe::tick x n_steps (src/universe/mod.rs:141-159) with the schedule of
  * build_schedule (src/universe/mod.rs:355-392): kick, drift, [wrap], forces,
  * kick, [thermostat].  Fused and resident: nothing crosses to the host but the
- * observables.  obs may be NULL, else it receives n_steps records. */
+ * observables.  obs may be NULL, else it receives n_steps records.
+ * With bond observation, an entity with more than 32 bound partners, tracked
+ * episodes or bonds (the reference has no cap) makes the call return
+ * GS_E_OVERFLOW; on the Verlet-list path the universe is then back at the
+ * call's start, on the cell tiles it holds the call's end.  The overflow is
+ * sticky until gs_bond_reset. */
 int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs);
 
 int gs_get_tuning(gs_universe* u, gs_tuning* out);
@@ -208,9 +213,12 @@ int gs_set_tuning(gs_universe* u, const gs_tuning* t);
 int gs_set_bound_search(gs_universe* u, int32_t aux_grid_only);
 /* Project autotuner: times trial_steps ticks of every candidate setting on
  * the resident state, restores the state after each trial and keeps the
- * fastest.  best (may be NULL) receives it.  The trials observe bonds too;
- * the bond observation state (tracker, adjacency, tick counter) is reset
- * afterwards, as by gs_bond_reset. */
+ * fastest.  best (may be NULL) receives it.  A trial that fails (other than
+ * with GS_E_HIP) is skipped; if every trial fails, the state, the tuning and
+ * the timing mode are left as they were and the last trial's status is
+ * returned.  The trials observe bonds too; the bond observation state
+ * (tracker, adjacency, tick counter) is reset afterwards, as by
+ * gs_bond_reset. */
 int gs_autotune(gs_universe* u, int32_t trial_steps, gs_tuning* best);
 
 /* Event timing of gs_step, reset on every call: on = 0 off, 1 the pair
@@ -342,7 +350,9 @@ int gs_adjoint_backward(gs_universe* u, double* dloss_dtheta);
  * atom, skin 0.12 rc, re-sort every 100 ticks) for every replica with >= 3
  * list cells of edge rc + skin per axis, or the per-tick cell tiles
  * (pair_mode 0) for every replica with >= 3 cells per axis.  The other
- * replicas use a brute-force pair kernel (grid_semantics as in gs_params). */
+ * replicas use a brute-force pair kernel (grid_semantics as in gs_params).
+ * Bond observation and the bond interaction are not part of the batched
+ * path: gs_batch_create rejects params with either enabled (GS_E_INVALID). */
 typedef struct gs_batch gs_batch;
 int gs_batch_create(int32_t n_replicas, const gs_params* params, const int64_t* n_atoms, gs_batch** out);
 int gs_batch_destroy(gs_batch* b);

@@ -143,3 +143,15 @@ def test_batch_list_overflow_replays():
         ok, worst = force_tolerance_ok(got[r].ax, got[r].ay, got[r].az, ref_f, m, 1e-10)
         assert ok, worst
         assert obs[0, r, 1] == pytest.approx(obs_ref[0, 1], rel=1e-10)
+
+
+def test_batch_rejects_bond_observation_and_interaction():
+    from genesis_amd._lib import GS_E_INVALID, GsError
+    from genesis_amd.batch import GpuBatch
+    ws = replica_sweep(n_rep=2, lo=600, hi=700, steps=1)
+    for flag in ("enable_bond_observation", "enable_bond_interaction"):
+        setattr(ws[1].physics, flag, True)
+        with pytest.raises(GsError) as e:
+            GpuBatch([w.physics for w in ws], [w.state for w in ws])
+        assert e.value.code == GS_E_INVALID
+        setattr(ws[1].physics, flag, False)

@@ -144,17 +144,22 @@ def test_bond_force_term_matches_oracle(GU):
     assert V == pytest.approx(ref["V"], rel=REL) and Vb == pytest.approx(ref["Vb"], rel=REL) and Vb != 0.0
 
 
-@pytest.mark.parametrize("n_side,mode", [(8, 0), (12, 0), (12, 1)])
-def test_bonded_trajectory_matches_oracle(GU, n_side, mode):
+@pytest.mark.parametrize("n_side,mode,lanes,aux", [(8, 0, 0, False), (12, 0, 0, False), (12, 1, 4, False),
+                                                    (12, 1, 2, False), (12, 1, 2, True)])
+def test_bonded_trajectory_matches_oracle(GU, n_side, mode, lanes, aux):
     # observation and interaction on, thresholds of ~116 ticks: bonds appear
-    # late in the run and their springs act from the next tick on
+    # late in the run and their springs act from the next tick on.  The list
+    # path at 2 and 4 lanes per atom finds the bound pairs inside the pair
+    # pass (the ionic melt's bench setting is 2 lanes); aux=True forces the
+    # aux-grid search instead.
     ph, st = melt(n_side)
     nsteps = 150
     ref, obs_ref, bs_ref, (rptr, ridx) = O.run_observed(ph, st, nsteps, nthreads=8)
-    say(f"bonded trajectory n={st.n} dims={ph.grid_dims()} mode={mode}")
+    say(f"bonded trajectory n={st.n} dims={ph.grid_dims()} mode={mode} lanes={lanes} aux={aux}")
     with GU(ph, st) as u:
         if mode == 1:
-            u.set_tuning(1, 4)
+            u.set_tuning(1, lanes)
+        u.set_bound_search(aux)
         obs = u.step(nsteps)
         got = u.download()
         s = u.bond_stats()
@@ -260,37 +265,175 @@ def test_fused_bound_search_matches_the_aux_grid_search(GU, lanes):
     assert np.array_equal(p1, p2) and np.array_equal(i1, i2)
 
 
-def test_dense_cluster_overflows_the_fused_search_and_replays(GU):
-    # 48 atoms inside one bind cutoff: with 2 lanes per atom a lane screens
-    # more candidates than its LDS column holds, so the call is replayed with
-    # the aux-grid search and must equal a run that used the aux grid
-    # throughout.  At
-    # rest and with a tiny dt nothing moves, so the overflow is the only
+def _bound_partner_counts(ph, p, species):
+    """Bound partners of every atom by brute force (d < k_bind sigma_ab,
+    src/analysis/pairs.rs:132-174): all partners, and those of larger id
+    (the ones an atom owns)."""
+    sig = np.asarray(ph.sigma)
+    d = p[:, None, :] - p[None, :, :]
+    d -= np.asarray(ph.box) * np.round(d / np.asarray(ph.box))
+    r = np.sqrt((d * d).sum(-1))
+    thr = ph.bond_k_bind * 0.5 * (sig[species][:, None] + sig[species][None, :])
+    b = (r < thr) & (r > 0)
+    return b.sum(1), np.triu(b, 1).sum(1)
+
+
+def _shell_cluster(ph, n_shell=40, radius=2.3, seed=7):
+    # n_shell atoms on a sphere around one centre atom that has the largest id
+    # (it owns none of its pairs), plus a dilute gas far away
+    rng = np.random.default_rng(seed)
+    k = np.arange(n_shell) + 0.5
+    th, phi = np.arccos(1.0 - 2.0 * k / n_shell), np.pi * (1.0 + 5.0 ** 0.5) * k
+    shell = radius * np.stack([np.sin(th) * np.cos(phi), np.sin(th) * np.sin(phi), np.cos(th)], 1)
+    gas = rng.uniform(-20.0, 20.0, (400, 3))
+    gas = gas[np.linalg.norm(gas, axis=1) > 8.0]
+    p = np.concatenate([shell, gas, np.zeros((1, 3))])
+    n = len(p)
+    species = np.where(np.arange(n) % 2 == 0, 5, 4).astype(np.uint8)  # Na, O
+    species[-1] = 5
+    charge = np.where(species == 5, 1.0, -1.0)
+    z = np.zeros(n)
+    return p, State(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), z, z.copy(), z.copy(), species, charge)
+
+
+def test_dense_shell_overflows_the_fused_search_and_replays(GU):
+    # An atom with 40 bound partners: with 2 lanes per atom one lane screens
+    # more candidates than its 16-entry LDS column holds, so the call is
+    # replayed with the aux-grid search.  No atom owns more than 32 partners
+    # (the centre has the largest id), so nothing is truncated: the result
+    # must equal a run that used the aux grid throughout and the oracle's.
+    # At rest and with a tiny dt nothing moves, so the overflow is the only
     # reason for a replay.
     base, _ = melt(8)
     ph = Physics(**{**base.__dict__, "box": (40.0, 40.0, 40.0), "dt": 1e-7})
-    rng = np.random.default_rng(7)
-    nc, ng = 48, 400
-    cl = rng.uniform(-1.8, 1.8, (nc, 3))
-    gas = rng.uniform(-20.0, 20.0, (ng, 3))
-    gas = gas[np.linalg.norm(gas, axis=1) > 8.0]
-    p = np.concatenate([cl, gas])
-    n = len(p)
-    species = np.where(np.arange(n) % 2 == 0, 5, 4).astype(np.uint8)  # Na, O
-    charge = np.where(species == 5, 1.0, -1.0)
-    z = np.zeros(n)
-    st = State(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), z, z, z, species, charge)
+    p, st = _shell_cluster(ph)
+    allp, owned = _bound_partner_counts(ph, p, st.species)
+    assert allp.max() >= 40 and owned.max() <= 32
+    nsteps = 2
+    ref, obs_ref, bs_ref, (rptr, ridx) = O.run_observed(ph, st, nsteps, nthreads=8)
     out = []
     for aux in (True, False):
         with GU(ph, st) as u:
             u.set_tuning(1, 2)
             u.set_bound_search(aux)
-            obs = u.step(2)
-            out.append((obs, u.download(), u.bond_stats(), u.tuning()))
-    (o1, s1, b1, t1), (o2, s2, b2, t2) = out
-    say(f"dense cluster n={n}: replays aux {t1['list_replays']}, fused {t2['list_replays']}")
+            obs = u.step(nsteps)
+            out.append((obs, u.download(), u.bond_stats(), u.tuning(), u.bound_pairs()))
+    (o1, s1, b1, t1, bp1), (o2, s2, b2, t2, bp2) = out
+    say(f"dense shell n={st.n}: replays aux {t1['list_replays']}, fused {t2['list_replays']}")
     assert t2["pair_mode"] == 1 and t1["list_replays"] == 0 and t2["list_replays"] == 1
     assert np.array_equal(o1, o2)
     for f in ("x", "y", "z", "vx", "vy", "vz"):
         assert np.array_equal(getattr(s1, f), getattr(s2, f)), f
     assert b1["tick"] == b2["tick"] and np.array_equal(b1["species_matrix"], b2["species_matrix"])
+    # against the oracle
+    np.testing.assert_allclose(o2[:, 5], obs_ref[:, 5], rtol=1e-9)
+    np.testing.assert_allclose(s2.x, ref.x, rtol=0, atol=1e-12)
+    assert [b2["tick"], b2["bonded_pairs"], b2["bonded_entities"], b2["bonds_formed"]] == list(bs_ref[-1][:4])
+    refp = O.bound_pairs(ph, s2.x, s2.y, s2.z, st.species)
+    assert np.array_equal(bp2, refp) and len(refp) >= 40 + 200
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_more_than_32_owned_partners_is_an_overflow_error(GU, mode):
+    # 48 atoms inside one bind cutoff: the smallest ids own more than 32
+    # bound partners, past the tracker's per-entity capacity.  The reference
+    # has no cap; here gs_step must report GS_E_OVERFLOW rather than drop
+    # partners, and on the list path the call returns to its start.
+    from genesis_amd._lib import GS_E_OVERFLOW, GsError
+    base, _ = melt(8)
+    ph = Physics(**{**base.__dict__, "box": (40.0, 40.0, 40.0), "dt": 1e-7})
+    rng = np.random.default_rng(7)
+    cl = rng.uniform(-1.8, 1.8, (48, 3))
+    gas = rng.uniform(-20.0, 20.0, (400, 3))
+    gas = gas[np.linalg.norm(gas, axis=1) > 8.0]
+    p = np.concatenate([cl, gas])
+    n = len(p)
+    species = np.where(np.arange(n) % 2 == 0, 5, 4).astype(np.uint8)
+    z = np.zeros(n)
+    st = State(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), z, z.copy(), z.copy(), species,
+               np.where(species == 5, 1.0, -1.0))
+    _, owned = _bound_partner_counts(ph, p, species)
+    assert owned.max() > 32
+    say(f"owned-partner overflow, mode {mode}, max owned {owned.max()}")
+    with GU(ph, st) as u:
+        if mode == 1:
+            u.set_tuning(1, 2)
+        with pytest.raises(GsError) as e:
+            u.step(2)
+        assert e.value.code == GS_E_OVERFLOW
+        got = u.download()
+        if mode == 1:
+            assert np.array_equal(got.x, st.x) and u.bond_stats()["tick"] == 0
+        # the same state fails again; a fresh universe without the cluster
+        # steps normally
+        with pytest.raises(GsError):
+            u.step(1)
+        u.bond_reset()
+        sub = State(st.x[48:].copy(), st.y[48:].copy(), st.z[48:].copy(), z[48:].copy(), z[48:].copy(),
+                    z[48:].copy(), species[48:].copy(), st.charge[48:].copy())
+    with GU(ph, sub) as u2:
+        if mode == 1:
+            u2.set_tuning(1, 2)
+        u2.step(2)
+
+
+def test_bond_force_with_a_bind_cutoff_inside_the_well(GU):
+    # k_bind = 1.1 < 2^(1/6): the bind cutoff lies inside r_eq, where
+    # smooth_cutoff(d, r_eq, rc) is (1, 0) (src/physics/forces.rs:302-305),
+    # so every bonded pair closer than 1.1 sigma_ab feels the full harmonic
+    # spring (src/systems/forces.rs:305-317).
+    ph, st = melt(12, enable_bond_interaction=True, bond_k_bind=1.1)
+    pairs = O.bound_pairs(ph, st.x, st.y, st.z, st.species)
+    keep = [(a, b) for a, b in pairs if a < 1500 and b < 1500]
+    adj = [[] for _ in range(st.n)]
+    for a, b in keep:
+        adj[a].append(b)
+        adj[b].append(a)
+    ptr = np.zeros(st.n + 1, dtype=np.int32)
+    ptr[1:] = np.cumsum([len(x) for x in adj])
+    idx = np.array([m for x in adj for m in sorted(x)], dtype=np.int32)
+    say(f"bond force, k_bind 1.1: {len(keep)} bonds, n={st.n}")
+    assert len(keep) >= 20
+    ref = O.forces(ph, st, bonds_csr=(ptr, idx))
+    with GU(ph, st) as u:
+        u.upload_bonds(ptr, idx)
+        V, Vb = u.forces()
+        got = u.download()
+    m = np.asarray(ph.mass)[st.species]
+    ok, worst = force_tolerance_ok(got.ax, got.ay, got.az, ref, m, REL)
+    assert ok, worst
+    assert np.isfinite(Vb) and Vb > 0.0
+    assert V == pytest.approx(ref["V"], rel=REL) and Vb == pytest.approx(ref["Vb"], rel=REL)
+
+
+def test_autotune_with_every_trial_failing_leaves_the_universe_as_it_was(GU):
+    # The 48-atom cluster overflows the bond tracker in every candidate
+    # setting: autotune must report the overflow and leave the state, the
+    # tuning and the timing mode untouched.
+    from genesis_amd._lib import GS_E_OVERFLOW, GsError
+    base, _ = melt(8)
+    ph = Physics(**{**base.__dict__, "box": (40.0, 40.0, 40.0), "dt": 1e-7})
+    rng = np.random.default_rng(7)
+    cl = rng.uniform(-1.8, 1.8, (48, 3))
+    gas = rng.uniform(-20.0, 20.0, (400, 3))
+    gas = gas[np.linalg.norm(gas, axis=1) > 8.0]
+    p = np.concatenate([cl, gas])
+    n = len(p)
+    species = np.where(np.arange(n) % 2 == 0, 5, 4).astype(np.uint8)
+    z = np.zeros(n)
+    st = State(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), z, z.copy(), z.copy(), species,
+               np.where(species == 5, 1.0, -1.0))
+    say("autotune with failing trials")
+    with GU(ph, st) as u:
+        u.set_tuning(1, 8, skin=0.3)
+        u.timing("pair", every=3)
+        before = u.tuning()
+        with pytest.raises(GsError) as e:
+            u.autotune(8)
+        assert e.value.code == GS_E_OVERFLOW
+        after = u.tuning()
+        got = u.download()
+        for k in ("pair_mode", "lanes_per_atom", "skin"):
+            assert after[k] == before[k], k
+        assert np.array_equal(got.x, st.x) and np.array_equal(got.vx, st.vx)
+        assert u.bond_stats()["tick"] == 0

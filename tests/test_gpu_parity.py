@@ -510,3 +510,73 @@ def test_velocity_histogram_bit_exact(GU, nbins, max_speed):
         assert h["bin_width"] == ref["bin_width"] and h["samples"] == ref["samples"] == st.n
         assert h["overflow"] == ref["overflow"]
         assert np.array_equal(h["bins"], ref["bins"])
+
+
+def _one_tick_forces_vs_oracle(GU, ph, st, mode, lanes, label, skin=None):
+    # Tick 1 starts with a = 0, so the GPU's positions after it equal the
+    # oracle's bit for bit and the forces of the tick compare directly.
+    say(f"{label}: one tick, pair_mode {mode} lanes {lanes}, n={st.n} dims={ph.grid_dims()}")
+    ref, obs_ref = O.run(ph, st, 1)
+    with GU(ph, st) as u:
+        u.set_tuning(mode, lanes, skin=skin)
+        obs = u.step(1)
+        got = u.download()
+        tun = u.tuning()
+    assert tun["pair_mode"] == mode
+    for a in ("x", "y", "z"):
+        assert np.array_equal(getattr(got, a), getattr(ref, a)), a
+    ref_f = O.forces(ph, State(ref.x, ref.y, ref.z, ref.vx, ref.vy, ref.vz, st.species, st.charge))
+    m = np.asarray(ph.mass)[st.species]
+    ok, worst = force_tolerance_ok(got.ax, got.ay, got.az, ref_f, m, REL)
+    say(f"{label}: worst conditioned force error {worst:.3e}")
+    assert ok, worst
+    assert obs[0, 1] == pytest.approx(obs_ref[0, 1], rel=REL)
+
+
+def _close_pairs(ph, st, cut):
+    p = np.stack([st.x, st.y, st.z], 1)
+    d = p[:, None, :] - p[None, :, :]
+    d -= np.asarray(ph.box) * np.round(d / np.asarray(ph.box))
+    r = np.sqrt((d * d).sum(-1))
+    iu = np.triu_indices(st.n, 1)
+    return r[iu], iu
+
+
+@pytest.fixture(scope="module")
+def charged_gas():
+    # Na/O/Si/Fe gas with no minimum separation: many charged pairs closer
+    # than the Coulomb clamp (0.5) and inside the hardened LJ core
+    # (0.5 sigma_ij), with electrostatics on (src/systems/forces.rs:282-289,
+    # src/physics/forces.rs:170-173).
+    ph = Physics(box=(30.0, 30.0, 30.0), enable_electrostatics=True)
+    st = random_gas(61, 3000, ph.box, species_pool=(4, 5, 6, 9), min_sep=0.0)
+    r, (ia, ib) = _close_pairs(ph, st, 0.5)
+    sig = np.asarray(DEFAULT_SIGMA)
+    core = r < 0.5 * 0.5 * (sig[st.species[ia]] + sig[st.species[ib]])
+    charged = (st.charge[ia] != 0) & (st.charge[ib] != 0)
+    assert ((r < 0.5) & charged).sum() >= 20 and (core & charged).sum() >= 100
+    return ph, st
+
+
+def test_forces_charged_gas_inside_clamp_and_core(GU, charged_gas):
+    ph, st = charged_gas
+    check_forces(GU, ph, st, "charged-gas-clamp-core")
+
+
+@pytest.mark.parametrize("mode,lanes", [(0, 1), (1, 2), (1, 4), (2, 0)])
+def test_charged_gas_inside_clamp_and_core_on_every_pair_path(GU, charged_gas, mode, lanes):
+    ph, st = charged_gas
+    _one_tick_forces_vs_oracle(GU, ph, st, mode, lanes, "charged-gas", skin=0.12 * ph.force_rc())
+
+
+@pytest.mark.parametrize("mode,lanes", [(0, 1), (1, 4), (2, 0)])
+def test_coulomb_switch_straddling_the_clamp(GU, mode, lanes):
+    # sigma 0.2 for every species and coulomb_cutoff 2.7: the Coulomb switch
+    # runs on [0.486, 0.54), across the 0.5 clamp, so pairs in [0.486, 0.5)
+    # take the clamped law with the switch at d, and the LJ core (0.1) and
+    # cutoff (0.5) sit inside the Coulomb range.
+    ph = Physics(box=(6.0, 6.0, 6.0), enable_electrostatics=True, coulomb_cutoff=2.7, sigma=(0.2,) * 10)
+    st = random_gas(62, 2000, ph.box, species_pool=(4, 5, 6, 9), min_sep=0.0)
+    r, _ = _close_pairs(ph, st, 0.54)
+    assert ((r > 0.486) & (r < 0.5)).sum() >= 20 and (r < 0.1).sum() >= 2
+    _one_tick_forces_vs_oracle(GU, ph, st, mode, lanes, "switch-across-clamp", skin=0.12 * ph.force_rc())

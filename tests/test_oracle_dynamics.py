@@ -7,6 +7,7 @@ import math
 import numpy as np
 import pytest
 
+from genesis_amd.inputs import State
 from genesis_amd.params import Physics
 from oracle import oracle as O
 from tests.helpers import reference_like
@@ -77,3 +78,38 @@ def test_integrator_steps_are_the_reference_expressions():
     st.x += 13.7
     O.wrap(st, ph.box)
     assert (st.x >= -5.0).all() and (st.x <= 5.0).all()
+
+
+def test_wrap_is_a_half_box_translation_modulo_the_box():
+    # BoundarySystem maps v to v.rem_euclid(2h) - h (src/systems/boundary.rs:30),
+    # which is mod(v, L) - L/2: every coordinate moves by half a box (mod L),
+    # so the origin goes to -L/2.  A wrap into [-L/2, L/2) without that
+    # translation would map 0 to 0 and fail here.
+    L = np.array([10.0, 7.5, 3.0])
+    vals = np.array([0.0, -0.0, 1.0, -1.0, 4.999, -4.999, 5.0, -5.0, 10.0, -10.0, 20.0, -20.0, 30.0, 12.5, -12.5,
+                     -1e-300, 1e-300, 7.25, -7.25, 1e6 + 0.5, -1e6 - 0.5, 2.5, -2.5])
+    n = len(vals)
+    z = np.zeros(n)
+    st = State(vals.copy(), vals.copy(), vals.copy(), z, z, z, np.zeros(n, dtype=np.uint8), z)
+    O.wrap(st, tuple(L))
+    for got, Lk in ((st.x, L[0]), (st.y, L[1]), (st.z, L[2])):
+        expect = np.mod(vals, Lk) - 0.5 * Lk
+        np.testing.assert_array_equal(got, expect)
+    assert st.x[0] == -5.0 and st.x[2] == -4.0 and st.x[3] == 4.0 and st.x[8] == -5.0
+    # -1e-300 mod L rounds to L itself: the reference's result is then +L/2
+    assert st.x[15] == 5.0
+
+
+def test_wrap_moves_the_whole_universe_rigidly():
+    # relative positions (minimum image) are unchanged by the wrap, absolute
+    # ones move by L/2 modulo L
+    ph = Physics(box=(24.0, 24.0, 24.0))
+    st = reference_like(5, 24.0, 100.0, seed=9)
+    x0, y0 = st.x.copy(), st.y.copy()
+    O.wrap(st, ph.box)
+    d_before = x0[:, None] - x0[None, :]
+    d_after = st.x[:, None] - st.x[None, :]
+    img = lambda d: d - 24.0 * np.round(d / 24.0)
+    np.testing.assert_allclose(img(d_after), img(d_before), rtol=0, atol=1e-12)
+    shift = np.mod(st.y - y0, 24.0)
+    np.testing.assert_allclose(np.minimum(np.abs(shift - 12.0), np.abs(shift + 12.0)), 0.0, atol=1e-12)

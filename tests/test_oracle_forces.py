@@ -95,3 +95,60 @@ def test_plain_lj_workload_geometry():
     assert w.state.n == 4096
     out = O.forces(w.physics, w.state)
     assert out["V"] < 0.0
+
+
+NA, OX = SPECIES_INDEX["Na"], SPECIES_INDEX["O"]
+
+
+def _coulomb_pair(ph, d):
+    """Na (+1) at the origin, O (-1) at +d on x: the scalar force along the
+    normal from O to Na (positive = repulsive) and V."""
+    st = State(np.array([0.0, d]), np.zeros(2), np.zeros(2), np.zeros(2), np.zeros(2), np.zeros(2),
+               np.array([NA, OX], dtype=np.uint8), np.array([1.0, -1.0]))
+    out = O.forces(ph, st)
+    f0 = -out["ax"][0] * ph.mass[NA]
+    f1 = out["ax"][1] * ph.mass[OX]
+    assert f0 == pytest.approx(f1, rel=1e-15, abs=1e-300)  # Newton's third law
+    return f0, out["V"]
+
+
+def test_coulomb_clamp_is_half_a_length_unit_absolute():
+    # src/systems/forces.rs:282-289: r = max(d, 0.5) in absolute units, so for
+    # k_e = 1 and q = +-1 the pair feels -1/0.25 = -4 and V = -1/0.5 = -2 at
+    # every d <= 0.5 (the switch is 1 this far inside the cutoff), and the
+    # plain law above.  LJ is off (epsilon = 0).  A clamp at 0.5 sigma_ij
+    # (0.9125 for Na-O) would give -1/0.9125^2 at d = 0.51.
+    ph = Physics(box=(40.0, 40.0, 40.0), enable_electrostatics=True, coulomb_constant=1.0,
+                 epsilon_k=(0.0,) * 10)
+    for d in (0.2, 0.49, 0.5):
+        f, v = _coulomb_pair(ph, d)
+        assert v == -2.0
+        assert f == pytest.approx(-4.0, rel=1e-15)
+    f, v = _coulomb_pair(ph, 0.51)
+    assert v == pytest.approx(-1.0 / 0.51, rel=1e-15)
+    assert f == pytest.approx(-1.0 / (0.51 * 0.51), rel=1e-15)
+
+
+def test_coulomb_switch_is_evaluated_at_the_unclamped_distance():
+    # A table of sigma 0.2 and coulomb_cutoff 2.7 puts the Coulomb switch on
+    # [0.486, 0.54), straddling the clamp.  With V(d) = (q/r) sw(d) and
+    # F = (q/r^2) sw(d) - (q/r) sw'(d), r = max(d, 0.5):
+    #   d < 0.486:        F = -4, V = -2 (switch 1 at d; at r = 0.5 it is not)
+    #   0.486 < d < 0.5:  F + V'(d) = V / 0.5 (the clamp breaks F = -V')
+    #   0.5 < d < 0.54:   F = -V'(d) (the switched law is conservative)
+    # V' by central differences.
+    ph = Physics(box=(3.0, 3.0, 3.0), enable_electrostatics=True, coulomb_constant=1.0, coulomb_cutoff=2.7,
+                 sigma=(0.2,) * 10, epsilon_k=(0.0,) * 10)
+    _, _, cut = O.lj_table(ph)
+    assert cut["coulomb_rc"] == pytest.approx(0.54) and cut["coulomb_ron"] == pytest.approx(0.486)
+    for d in (0.2, 0.3, 0.45, 0.48):
+        f, v = _coulomb_pair(ph, d)
+        assert (f, v) == (pytest.approx(-4.0, rel=1e-15), -2.0)
+    h = 1e-7
+    for d in (0.489, 0.492, 0.495, 0.498, 0.505, 0.51, 0.52, 0.53, 0.535):
+        f, v = _coulomb_pair(ph, d)
+        dv = (_coulomb_pair(ph, d + h)[1] - _coulomb_pair(ph, d - h)[1]) / (2 * h)
+        assert -2.0 < v < 0.0
+        expect = v / 0.5 if d < 0.5 else 0.0
+        assert f + dv == pytest.approx(expect, rel=1e-6, abs=1e-6), d
+    assert _coulomb_pair(ph, 0.541) == (0.0, 0.0)
