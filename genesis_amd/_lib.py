@@ -123,6 +123,7 @@ _SIGNATURES = {
     "gs_velocity_histogram": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_int32, _U64,
                                              ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), _D]),
     "gs_bond_reset": (ctypes.c_int, [_H]),
+    "gs_radial_distribution_noalloc": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_int32, ctypes.c_void_p, _D, _D]),
 }
 
 _lib = None

@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <set>
 #include <string>
 #include <vector>
@@ -3412,6 +3413,10 @@ int run_ticks(gs_universe* u, int64_t n_steps, bool observe, bool list) {
 
 }  // namespace
 
+// gs_noalloc.inc: address of the allocation-free entry point's scratch area on
+// the current device, set up once per device.
+static int noalloc_scratch_setup(int dev);
+
 // =============================================================================
 // C ABI
 // =============================================================================
@@ -3444,6 +3449,10 @@ int gs_create(const gs_params* params, int64_t n_atoms, gs_universe** out) {
     u->list_ok = !u->small_box && list_geometry(u, 0.06 * u->kp.rc);
     if (!u->list_ok) u->pair_mode = 0;
     (void)hipGetDevice(&u->dev);
+    if ((rc = noalloc_scratch_setup(u->dev))) {
+        delete u;
+        return rc;
+    }
     {
         int ncu = 0;
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, u->dev) == hipSuccess) u->n_cu = ncu;
@@ -4289,3 +4298,4 @@ int gs_clusters(gs_universe* u, double bond_factor, uint32_t* labels, gs_cluster
 #include "gs_adjoint.inc"
 #include "gs_batch.inc"
 #include "gs_bonds.inc"
+#include "gs_noalloc.inc"

@@ -245,6 +245,23 @@ int gs_velocity_histogram(gs_universe* u, double max_speed, int32_t nbins, uint6
 int gs_radial_distribution(gs_universe* u, double r_max, int32_t nbins,
                            uint64_t* counts, double* g,
                            double* r_max_out, double* dr_out);
+/* No counterpart in the reference: an allocation-free variant of
+ * gs_radial_distribution for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same subsample, grid and pair counts (bit-identical to
+ * gs_radial_distribution's counts); the counts are written to counts_dev, a
+ * DEVICE array of nbins u64 (nbins in [1, 65536], otherwise GS_E_INVALID),
+ * on the handle's stream, and the call does not wait for them: they are
+ * ready once the stream is (any later call on the handle that returns data,
+ * or a device synchronisation).  The subsample (at most 8192 points) and its
+ * grid (at most 36864 cells) live in one fixed scratch area of the library
+ * per GPU, under 1 MiB, held until the process ends; the call makes no
+ * hipMalloc/hipFree and no host allocation.  Calls on one GPU share that
+ * area: calls that overlap on one GPU (two threads, two handles' streams)
+ * race.  r_max_out / dr_out (host, may be NULL) as in gs_radial_distribution;
+ * g is the caller's normalisation of the counts. */
+int gs_radial_distribution_noalloc(gs_universe* u, double r_max, int32_t nbins, uint64_t* counts_dev,
+                                   double* r_max_out, double* dr_out);
 /* radial_distribution_between (src/analysis/mod.rs:143-201). */
 int gs_radial_distribution_between(gs_universe* u, int32_t species_a, int32_t species_b,
                                    double r_max, int32_t nbins,
