@@ -1,18 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X force-and-integration path.
 
-Workload: the 10-element mixture (32,768 atoms on a 32^3 lattice, FP64,
-Lorentz-Berthelot LJ + truncated Coulomb sharing rc = 2.5 max sigma,
-Berendsen NVT at 60 K with tau = 100 fs, dt = 0.5 fs, g(r) with 200 bins
-to L/4 sampled every 100 ticks).  A step is one full tick of the reference
-schedule (kick, drift, wrap, cell-list rebuild, forces, kick, Berendsen) over
-the resident state; inputs are synthetic (seeded) and resident in HBM before
-the timed region.
+Headline workload: the 1,048,576-atom Na/O ionic melt with bonds (rock salt
+on a 128 x 128 x 64 lattice at rho sigma_NaO^3 = 0.7, FP64, LJ + truncated
+Coulomb at rc = 2.5 max sigma, Maxwell-Boltzmann 300 K quenched by Berendsen
+to 80 K with tau = 100 fs, dt = 0.5 fs, bond observation every tick with the
+harmonic bond interaction on, g(r) with 256 bins to L_min/4 every 100 ticks,
+the Bonds graph's components every 500 ticks).  The quench runs untimed until
+persistent bonds exist (the persistence threshold is about 11,600 ticks), so
+the timed window is in the bonded regime; it ends on a tick that is a
+multiple of 500, so it always holds its g(r) and bond-component samples.  A
+step is one full tick of the reference schedule (kick, drift, wrap, list
+maintenance, forces with the bond term, kick, Berendsen, bond observation)
+over the resident state; inputs are synthetic (seeded) and resident in HBM
+before the timed region; gs_step returns only after its stream has drained.
 
-Prints one JSON line.  ``roofline`` is the pair kernel's FP64 roofline (its
-binding resource; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s),
-``roofline_hbm`` the same kernel against HBM, ``cpu_baseline`` the oracle's
-restatement of the reference CPU algorithm timed on this host.
+Prints one JSON line.  ``roofline`` is the dominant kernel's FP64 roofline
+(its binding resource: the FP64 vector pipe; the MI355X FP64 vector peak is
+78.6 TFLOP/s), ``roofline_hbm`` the same kernel against HBM,
+``cpu_baseline`` the oracle's restatement of the reference CPU algorithm
+timed on this host on a one-tick sample of the same workload, with all the
+host cores this job may use and with one thread.  Extra legs: the 32,768-atom
+mixture, the 64-replica temperature sweep, the 8,192-atom adjoint.
 """
 from __future__ import annotations
 
@@ -27,178 +36,242 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "atom-steps/s (force+Verlet), batched vs plain, at stated force tol; % HBM BW"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 # Algorithmic work per atom-step (compulsory HBM bytes of one tick, and the pair law's flops):
 STEP_BYTES = 161.0          # x, v, a read+write (144) + species (1) + cell key (8) + charge (8)
-PAIR_BYTES = 57.0           # pair kernel: x, y, z, q, species read (33) + a written (24)
+# list pair kernel, per atom: x, y, z, q, species read (33) + a written (24) + v read and written
+# by the fused second kick (48)
+PAIR_BYTES = 105.0
 FLOP_PER_PAIR = 82.0        # LJ + quintic switch + accumulate (70) + Coulomb (12), per ordered pair
-# HBM traffic of one k_pair_list launch of the default setting (4 lanes, skin
-# 0.10 rc) on this workload, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in
-# KiB per dispatch, scaled by what the same counters read for a 256 MiB
-# device copy (FETCH_SIZE 131084.5 KiB -> x2.0, WRITE_SIZE 262144 KiB -> x1.0);
-# profiles/k_pair_list__mixture32768_pmc.md holds the runs.
-PAIR_TRAFFIC_KIB = {"fetch": 12218.0, "write": 1584.4, "fetch_scale": 2.0, "write_scale": 1.0}
-DEFAULT_LANES, DEFAULT_SKIN_FRAC = 4, 0.10
+CPU_SHARE = 16              # host cores one GPU's job may use on the GPU box
+# HBM traffic per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (KiB per dispatch) of `bench.py --leg ionic` at the default setting, scaled by
+# what the same counters read for a 256 MiB device copy (`bench.py --copy-probe`).  Only
+# reported for the setting it was measured at.
+PAIR_TRAFFIC = {"setting": {"lanes": 2, "skin_frac": 0.04}, "fetch_kib": None, "write_kib": None,
+                "fetch_scale": 2.0, "write_scale": 1.0}
+DEFAULT_LANES, DEFAULT_SKIN_FRAC = 2, 0.04
+GR_EVERY, COMP_EVERY = 100, 500
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--pair-mode", type=int, default=1, help="-1 autotune, 0 cell tiles, 1 Verlet list")
-    ap.add_argument("--lanes", type=int, default=DEFAULT_LANES)
-    ap.add_argument("--skin", type=float, default=None,
-                    help=f"list skin (length units); default {DEFAULT_SKIN_FRAC} rc")
-    ap.add_argument("--sort-every", type=int, default=0, help="list mode: ticks between full re-sorts")
-    ap.add_argument("--no-extras", action="store_true",
-                    help="skip the batched-replica and adjoint legs")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="only run a 256 MiB device copy (HBM counter calibration) and exit")
-    ap.add_argument("--leg", choices=("batched", "adjoint", "ionic"), default=None,
-                    help="run only that leg (ticks from --steps) and print its JSON record, for profiling")
-    args = ap.parse_args()
-    if args.copy_probe:
-        return copy_probe()
-    if args.leg:
-        if args.leg == "batched":
-            rec = batched_leg(ticks=args.steps, segment=min(100, args.steps))
-        elif args.leg == "adjoint":
-            rec = adjoint_leg(steps=args.steps, ckpt=min(50, args.steps))
-        else:
-            rec = ionic_leg(ticks=args.steps, gr_every=min(100, args.steps))
-        print(json.dumps(rec), flush=True)
-        return 0
-    if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on one GPU only: the path would shard by spatial "
-                                   "domain with halo exchange, but no multi-GPU figure is produced; run with --gpus 1"}))
-        return 2
+class Sampler:
+    """Steps a universe in chunks that end on the sampling ticks: g(r) on every
+    tick that is a multiple of gr_every, the Bonds graph's components on every
+    multiple of comp_every (labels by entity id, copied to the host)."""
 
-    from genesis_amd.inputs import mixture
-    from genesis_amd.universe import GpuUniverse
+    def __init__(self, u, tick, gr_rmax, gr_bins, gr_every=GR_EVERY, comp_every=COMP_EVERY):
+        self.u, self.tick = u, tick
+        self.gr_rmax, self.gr_bins = gr_rmax, gr_bins
+        self.gr_every, self.comp_every = gr_every, comp_every
+        self.n_gr = self.n_comp = 0
 
-    w = mixture()
-    ph, st = w.physics, w.state
-    n = st.n
-    gr_every, gr_bins, gr_rmax = w.extra["gr_every"], w.extra["gr_bins"], w.extra["gr_rmax"]
-    log(f"workload {w.name}: n={n} box={ph.box[0]:.4f} dims={ph.grid_dims()} steps={args.steps} "
-        f"warmup={args.warmup}")
-
-    u = GpuUniverse(ph, st)
-    skin = DEFAULT_SKIN_FRAC * ph.force_rc() if args.skin is None else args.skin
-    default_setting = args.pair_mode == 1 and args.lanes == DEFAULT_LANES and args.skin is None
-    if args.pair_mode < 0:
-        tuning = u.autotune(40)
-        log(f"autotuned: {tuning}")
-    else:
-        u.set_tuning(args.pair_mode, args.lanes, skin=skin, sort_every=args.sort_every)
-        tuning = u.tuning()
-        log(f"tuning: {tuning}")
-
-    def run(k, observe=False):
+    def run(self, k):
         done = 0
         while done < k:
-            chunk = min(gr_every, k - done)
-            u.step(chunk, observe=observe)
+            chunk = min(self.gr_every - self.tick % self.gr_every, k - done)
+            self.u.step(chunk, observe=False)
+            self.tick += chunk
             done += chunk
-            u.radial_distribution(gr_rmax, gr_bins)
-    if args.warmup:
-        run(args.warmup)
-    builds0, replays0 = u.tuning()["list_builds"], u.tuning()["list_replays"]
-    # timed region: on every 10th tick the pair kernel is launched with start
-    # and stop HIP events bound to its dispatch (its average duration feeds
-    # the roofline); sampling keeps the events' cost out of the rate
+            if self.tick % self.gr_every == 0:
+                self.u.radial_distribution(self.gr_rmax, self.gr_bins)
+                self.n_gr += 1
+            if self.comp_every and self.tick % self.comp_every == 0:
+                self.u.bond_labels()
+                self.n_comp += 1
+
+
+def count_pairs(s, ph) -> int:
+    """Ordered pairs within the force cutoff of a state (periodic k-d tree count)."""
+    from scipy.spatial import cKDTree
+    box = np.asarray(ph.box, dtype=np.float64)
+    p = np.mod(np.stack([s.x, s.y, s.z], 1) + 0.5 * box, box)
+    p[p >= box] = 0.0
+    t = cKDTree(p, boxsize=box)
+    c = t.query_ball_point(p, ph.force_rc(), return_length=True, workers=min(CPU_SHARE, os.cpu_count() or 1))
+    return int(c.sum()) - s.n
+
+
+def pair_roofline(kernel, pair_ms, npairs, n, traffic=None):
+    tflops = FLOP_PER_PAIR * npairs / (pair_ms * 1e-3) / 1e12
+    gbs = PAIR_BYTES * n / (pair_ms * 1e-3) / 1e9
+    fp64 = {"bound": "fp64-valu", "pipe": "FP64 vector ALU (pair forces are not a dense contraction: no MFMA)",
+            "kernel": kernel, "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tflops / FP64_PEAK_TFLOPS, "traffic": traffic,
+            "per_launch": {"ordered_pairs_in_range": npairs, "flop_per_pair": FLOP_PER_PAIR, "avg_ms": pair_ms,
+                           "algorithmic_flop": FLOP_PER_PAIR * npairs}}
+    hbm = {"bound": "hbm", "kernel": kernel, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": gbs / HBM_PEAK_GBS, "bytes_per_atom": PAIR_BYTES, "traffic": traffic}
+    return fp64, hbm
+
+
+def ionic_headline(steps: int, warmup: int, lanes: int, skin_frac: float, setup_max: int = 20000):
+    from genesis_amd.inputs import ionic_melt
+    from genesis_amd.universe import GpuUniverse
+    w = ionic_melt()
+    ph, st = w.physics, w.state
+    n = st.n
+    skin = skin_frac * ph.force_rc()
+    log(f"workload {w.name}: n={n} box={tuple(round(b, 3) for b in ph.box)} dims={ph.grid_dims()} "
+        f"steps={steps} warmup={warmup} lanes={lanes} skin={skin:.4f}")
+    u = GpuUniverse(ph, st)
+    u.set_tuning(1, lanes, skin=skin, sort_every=100)
+    # Setup (untimed): the quench until persistent bonds exist.
+    t0 = time.perf_counter()
+    smp = Sampler(u, 0, w.extra["gr_rmax"], w.extra["gr_bins"])
+    bonded = 0
+    while smp.tick < setup_max:
+        smp.run(500)
+        bonded = u.bond_stats()["bonded_pairs"]
+        if bonded > 0:
+            break
+    if bonded == 0:
+        raise RuntimeError(f"no persistent bonds after {smp.tick} ticks")
+    # the window [first, last] ends on a multiple of 500
+    smp.run((-(smp.tick + warmup + steps)) % COMP_EVERY)
+    setup_ticks, setup_s = smp.tick, time.perf_counter() - t0
+    log(f"setup: {setup_ticks} ticks in {setup_s:.1f} s, {bonded} bonded pairs at the first bonded sample")
+    smp.run(warmup)
+    b0 = u.bond_stats()
+    tun0 = u.tuning()
+    gr0, comp0 = smp.n_gr, smp.n_comp
+    first = smp.tick + 1
     u.timing(enable="pair", every=10)
     t0 = time.perf_counter()
-    run(args.steps)
+    smp.run(steps)
     t1 = time.perf_counter()
     tm = u.timing()
-    builds = u.tuning()["list_builds"] - builds0
-    replays = u.tuning()["list_replays"] - replays0
-    elapsed = t1 - t0
-    value = n * args.steps / elapsed
-    ms_per_step = 1e3 * elapsed / args.steps
-    # phase breakdown from a separate, untimed pass with four events per tick
-    u.timing(enable="phases")
-    run(min(args.steps, 500))
-    ph_tm = u.timing()
     u.timing(enable=False)
-    phases = {k: ph_tm[k] / max(ph_tm["steps"], 1) for k in ("rebuild_ms", "pair_ms", "kick_ms")}
-    log(f"{value:.4e} atom-steps/s, {ms_per_step:.4f} ms/step; pair kernel {tm['pair_ms'] / tm['steps']:.4f} ms; "
-        f"phases per step: rebuild {phases['rebuild_ms']:.4f} ms, pair {phases['pair_ms']:.4f} ms, "
-        f"kick {phases['kick_ms']:.4f} ms; list builds {builds}, replays {replays}")
-    kernel = {1: f"k_pair_list<{tuning['lanes_per_atom']} lanes> (pair forces + second kick + Berendsen)",
-              2: "k_pair_cell (cell-staged lists; pair forces + second kick + Berendsen)"}.get(
-                  tuning["pair_mode"], "k_pair")
-
-    # pair count of the final state (ordered pairs within rc)
-    fin = u.download()
+    elapsed = t1 - t0
+    tun = u.tuning()
+    b1 = u.bond_stats()
+    value = n * steps / elapsed
     pair_ms = tm["pair_ms"] / max(tm["steps"], 1)
-    # in-range ordered pairs per atom from a neighbour count on the final state
+    log(f"{value:.4e} atom-steps/s, {1e3 * elapsed / steps:.4f} ms/tick over ticks {first}..{smp.tick}; "
+        f"pair kernel {pair_ms:.4f} ms; bonded pairs {b0['bonded_pairs']} -> {b1['bonded_pairs']}; "
+        f"g(r) samples {smp.n_gr - gr0}, component samples {smp.n_comp - comp0}")
+    # untimed: phases, fixed per-call costs, pair count of the final state
+    u.timing(enable="phases")
+    t0 = time.perf_counter()
+    u.step(200, observe=False)
+    t_tick = (time.perf_counter() - t0) / 200
+    pt = u.timing()
+    u.timing(enable=False)
+    phases = {k: pt[k] / max(pt["steps"], 1) for k in ("rebuild_ms", "pair_ms", "kick_ms")}
+    phases["observe_and_rest_ms"] = 1e3 * t_tick - sum(phases.values())
+    fixed = fixed_costs(u, w.extra["gr_rmax"], w.extra["gr_bins"], 1e3 * t_tick)
+    fin = u.download()
+    bfin = u.bond_stats()
+    u.close()
     npairs = count_pairs(fin, ph)
-    flops_per_launch = FLOP_PER_PAIR * npairs
-    tflops = flops_per_launch / (pair_ms * 1e-3) / 1e12
-    gbs = PAIR_BYTES * n / (pair_ms * 1e-3) / 1e9
+    kernel = f"k_pair_list<{lanes} lanes, bound-pair search> (pair forces + bond term + second kick + Berendsen)"
+    traffic = None
+    if PAIR_TRAFFIC["fetch_kib"] is not None and (lanes, skin_frac) == (PAIR_TRAFFIC["setting"]["lanes"],
+                                                                       PAIR_TRAFFIC["setting"]["skin_frac"]):
+        traffic = 1024.0 * (PAIR_TRAFFIC["fetch_kib"] * PAIR_TRAFFIC["fetch_scale"] +
+                            PAIR_TRAFFIC["write_kib"] * PAIR_TRAFFIC["write_scale"])
+    roof, roof_hbm = pair_roofline(kernel, pair_ms, npairs, n, traffic)
+    deg = 2.0 * bfin["bonded_pairs"] / n
+    step_bytes = STEP_BYTES + 4.0 * (1.0 + deg)
     out = {
-        "metric": "atom-steps/s (force+Verlet), batched vs plain, at stated force tol; % HBM BW",
+        "metric": METRIC,
         "value": value,
         "unit": "atom-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": 1e3 * elapsed / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded lattice + Maxwell-Boltzmann, no dataset)",
-        "config": {"workload": w.name, "atoms": n, "species": 10, "box": ph.box[0], "rc": ph.force_rc(),
-                   "cells": list(ph.grid_dims()), "dt_fs": 0.5, "thermostat": "berendsen 60 K tau 200 ticks",
-                   "coulomb": True, "gr": {"bins": gr_bins, "r_max": gr_rmax, "every": gr_every},
+        "data": "synthetic (seeded rock-salt lattice + Maxwell-Boltzmann, no dataset)",
+        "config": {"workload": w.name, "atoms": n, "lattice": "128x128x64 rock salt, Na +1 / O -1",
+                   "box": list(ph.box), "rc": ph.force_rc(), "cells": list(ph.grid_dims()), "dt_fs": 0.5,
+                   "thermostat": "berendsen, 300 K start -> 80 K, tau 200 ticks", "coulomb": True,
+                   "bonds": {"k_bind": ph.bond_k_bind, "min_periods": ph.bond_min_periods, "debounce_ticks": 2,
+                             "observation": "every tick", "interaction": True},
+                   "gr": {"bins": w.extra["gr_bins"], "r_max": w.extra["gr_rmax"], "every": GR_EVERY},
+                   "bond_components_every": COMP_EVERY,
+                   "window": {"first_tick": first, "last_tick": smp.tick, "untimed_setup_ticks": setup_ticks,
+                              "bonded_pairs_at_start": b0["bonded_pairs"], "bonded_pairs_at_end": b1["bonded_pairs"],
+                              "bonded_entities_at_end": b1["bonded_entities"],
+                              "gr_samples_timed": smp.n_gr - gr0, "component_samples_timed": smp.n_comp - comp0},
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "pipe": "fp64 VALU (pair forces are not a dense contraction; the MI355X "
-                                              "FP64 vector peak equals its FP64 matrix peak)",
-                     "kernel": kernel, "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": tflops / FP64_PEAK_TFLOPS,
-                     "traffic": (1024.0 * (PAIR_TRAFFIC_KIB["fetch"] * PAIR_TRAFFIC_KIB["fetch_scale"] +
-                                           PAIR_TRAFFIC_KIB["write"] * PAIR_TRAFFIC_KIB["write_scale"])
-                                 if default_setting else None),
-                     "per_launch": {"ordered_pairs": npairs, "flop_per_pair": FLOP_PER_PAIR,
-                                    "avg_ms": pair_ms}},
-        "roofline_hbm": {"bound": "hbm", "kernel": kernel, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": gbs / HBM_PEAK_GBS, "bytes_per_atom": PAIR_BYTES},
-        "step_roofline": {"hbm_frac": value * STEP_BYTES / (HBM_PEAK_GBS * 1e9),
+        "roofline": roof,
+        "roofline_hbm": roof_hbm,
+        "step_roofline": {"hbm_frac": value * step_bytes / (HBM_PEAK_GBS * 1e9),
                           "fp64_frac": value * (FLOP_PER_PAIR * npairs / n) / (FP64_PEAK_TFLOPS * 1e12),
-                          "bytes_per_atom_step": STEP_BYTES, "flop_per_atom_step": FLOP_PER_PAIR * npairs / n},
+                          "bytes_per_atom_step": step_bytes, "flop_per_atom_step": FLOP_PER_PAIR * npairs / n},
         "phases_ms_per_step": phases,
-        "tuning": {k: tuning[k] for k in ("pair_mode", "lanes_per_atom", "skin", "sort_every", "ms_per_tick")},
-        "list_builds_in_timed_region": builds,
-        "list_replays_in_timed_region": replays,
+        "fixed_costs_ms": fixed,
+        "tuning": {"pair_mode": tun["pair_mode"], "lanes_per_atom": tun["lanes_per_atom"], "skin": tun["skin"],
+                   "sort_every": tun["sort_every"]},
+        "list_builds_in_timed_region": tun["list_builds"] - tun0["list_builds"],
+        "list_replays_in_timed_region": tun["list_replays"] - tun0["list_replays"],
     }
-    u.close()
-    if args.pair_mode >= 0 and not args.no_extras:
-        # the project autotuner on the same workload, for comparison with the
-        # fixed setting timed above
-        with GpuUniverse(ph, st) as au:
-            t = au.autotune(40)
-        out["autotune"] = {k: t[k] for k in ("pair_mode", "lanes_per_atom", "skin", "ms_per_tick")}
-        log(f"autotune pick: {out['autotune']}")
-    if not args.no_extras:
-        out["batched"] = batched_leg()
-        out["adjoint"] = adjoint_leg()
-        out["ionic_melt"] = ionic_leg()
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
-        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
-    print(json.dumps(out), flush=True)
-    return 0
+    return out, w, fin
+
+
+def fixed_costs(u, gr_rmax, gr_bins, tick_ms) -> dict:
+    """Per-call costs outside the ticks: a g(r) sample, a bond-components
+    sample, and the fixed cost of a gs_step call (snapshot, flag reads,
+    synchronisation) from 20 one-tick calls against one 20-tick call."""
+    t0 = time.perf_counter()
+    for _ in range(3):
+        u.radial_distribution(gr_rmax, gr_bins)
+    gr = (time.perf_counter() - t0) / 3
+    t0 = time.perf_counter()
+    for _ in range(3):
+        u.bond_labels()
+    comp = (time.perf_counter() - t0) / 3
+    t0 = time.perf_counter()
+    u.step(20, observe=False)
+    t20 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(20):
+        u.step(1, observe=False)
+    t1 = time.perf_counter() - t0
+    return {"gr_sample": 1e3 * gr, "bond_components_sample": 1e3 * comp,
+            "step_call_overhead": 1e3 * (t1 - t20) / 19, "tick": tick_ms}
+
+
+def mixture_leg(steps: int = 2000, warmup: int = 100) -> dict:
+    """The 32,768-atom 10-element mixture (Coulomb, Berendsen 60 K, dt 0.5 fs,
+    g(r) every 100 ticks) at the list setting of 4 lanes per atom, skin 0.10 rc,
+    with the autotuner's pick beside it."""
+    from genesis_amd.inputs import mixture
+    from genesis_amd.universe import GpuUniverse
+    w = mixture()
+    ph, st = w.physics, w.state
+    n = st.n
+    with GpuUniverse(ph, st) as u:
+        u.set_tuning(1, 4, skin=0.10 * ph.force_rc())
+        smp = Sampler(u, 0, w.extra["gr_rmax"], w.extra["gr_bins"], comp_every=0)
+        smp.run(warmup)
+        u.timing(enable="pair", every=10)
+        t0 = time.perf_counter()
+        smp.run(steps)
+        el = time.perf_counter() - t0
+        tm = u.timing()
+        u.timing(enable=False)
+        fin = u.download()
+    with GpuUniverse(ph, st) as au:
+        t = au.autotune(40)
+    pair_ms = tm["pair_ms"] / max(tm["steps"], 1)
+    npairs = count_pairs(fin, ph)
+    roof, roof_hbm = pair_roofline("k_pair_list<4 lanes>", pair_ms, npairs, n)
+    v = n * steps / el
+    log(f"mixture: {v:.4e} atom-steps/s ({1e3 * el / steps:.4f} ms/tick), pair kernel {pair_ms:.4f} ms, "
+        f"{roof['achieved']:.2f} TFLOP/s; autotune pick {t}")
+    return {"workload": w.name, "atoms": n, "ticks": steps, "atom_steps_per_s": v, "ms_per_tick": 1e3 * el / steps,
+            "roofline": roof, "roofline_hbm": roof_hbm,
+            "autotune": {k: t[k] for k in ("pair_mode", "lanes_per_atom", "skin", "ms_per_tick")}}
 
 
 def batched_leg(ticks: int = 200, segment: int = 100) -> dict:
@@ -244,8 +317,8 @@ def adjoint_leg(steps: int = 1000, ckpt: int = 50) -> dict:
     """Adjoint workload: 8,192 atoms of the mixture, NVE, dt = 0.5 fs, 1000
     ticks, checkpoints every 50; loss = mean V over the last 200 ticks +
     smoothed g(r) deviation; gradient w.r.t. the 30 species parameters.
-    Reports the reverse pass per tick relative to the forward tick."""
-    import numpy as np
+    Reports the reverse pass per tick against the adjoint's own forward tick
+    and against a plain gs_step tick (Verlet-list path) of the same state."""
     from genesis_amd.adjoint import theta_of
     from genesis_amd.inputs import adjoint_mixture
     from genesis_amd.universe import GpuUniverse
@@ -265,55 +338,74 @@ def adjoint_leg(steps: int = 1000, ckpt: int = 50) -> dict:
         t0 = time.perf_counter()
         grad = u.adjoint_backward()
         tb = time.perf_counter() - t0
+    with GpuUniverse(ph, st) as u:
+        u.step(50, observe=False)
+        t0 = time.perf_counter()
+        u.step(steps, observe=False)
+        tplain = time.perf_counter() - t0
+        tun = u.tuning()
     log(f"adjoint: forward {1e3 * tf / steps:.4f} ms/tick, reverse {1e3 * tb / steps:.4f} ms/tick "
-        f"({tb / tf:.2f}x)")
+        f"({tb / tf:.2f}x); plain gs_step {1e3 * tplain / steps:.4f} ms/tick (reverse {tb / tplain:.2f}x)")
     return {"workload": f"adjoint mixture, {st.n} atoms, {steps} ticks, checkpoint every {ckpt}",
             "forward_ms_per_tick": 1e3 * tf / steps, "reverse_ms_per_tick": 1e3 * tb / steps,
-            "reverse_over_forward": tb / tf, "loss": loss, "grad_norm": float(np.linalg.norm(grad))}
+            "reverse_over_forward": tb / tf, "plain_step_ms_per_tick": 1e3 * tplain / steps,
+            "reverse_over_plain_step": tb / tplain,
+            "plain_step_tuning": {k: tun[k] for k in ("pair_mode", "lanes_per_atom", "skin")},
+            "loss": loss, "grad_norm": float(np.linalg.norm(grad))}
 
 
-def ionic_leg(ticks: int = 200, gr_every: int = 100) -> dict:
-    """The 1,048,576-atom Na/O rock-salt melt (128 x 128 x 64, Coulomb,
-    Berendsen 300 K -> 80 K, dt = 0.5 fs) with bond observation and the bond
-    interaction on every tick, g(r) (256 bins) every 100 ticks, and the
-    components of the Bonds graph once at the end (timed apart)."""
-    from genesis_amd.inputs import ionic_melt
-    from genesis_amd.structure import BondStructureSystem
-    from genesis_amd.universe import GpuUniverse
-    w = ionic_melt()
-    ph, st = w.physics, w.state
-    n = st.n
-    with GpuUniverse(ph, st) as u:
-        tune = u.autotune(20)
-        u.step(20, observe=False)
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, state) -> dict:
+    """The oracle's restatement of the reference CPU algorithm (serial grid +
+    half-shell pair list, threaded fold with per-thread buffers merged in
+    order, serial kinetic sum, the bond tracker) timed on this host on a
+    bounded sample of the headline workload: one full tick with bond
+    observation from the state the GPU reached, with all the cores this job
+    may use and with one thread.  Beside it, the force pass alone on the
+    32,768-atom mixture at 1, 8 and all cores: the reference's published claim
+    is '~3x on 8 cores' for the parallel force pass (README.md:55-56)."""
+    from genesis_amd.inputs import mixture
+    from oracle import oracle as O
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    cores = max(1, min(avail, CPU_SHARE))
+    legs = {}
+    for th in (cores, 1):
         t0 = time.perf_counter()
-        done = 0
-        while done < ticks:
-            k = min(gr_every, ticks - done)
-            u.step(k, observe=False)
-            done += k
-            u.radial_distribution(w.extra["gr_rmax"], w.extra["gr_bins"])
-        te = time.perf_counter() - t0
+        O.run_observed(w.physics, state, 1, nthreads=th)
+        legs[th] = time.perf_counter() - t0
+        log(f"cpu baseline: one tick of {state.n} atoms with {th} threads: {legs[th]:.2f} s")
+    m = mixture()
+    fp = {}
+    for th in sorted({1, min(8, cores), cores}):
+        O.forces(m.physics, m.state, nthreads=th, extras=False)
         t0 = time.perf_counter()
-        comps = u.bond_labels()
-        tc = time.perf_counter() - t0
-        # the bond-graph stoichiometry lens (components, formulas, binding
-        # energies, lifecycle against the previous sample), as sampled every
-        # 500 ticks in this workload
-        lens = BondStructureSystem(1)
-        t0 = time.perf_counter()
-        chem = lens.run(u, ticks, st.species)
-        ts = time.perf_counter() - t0
-        bs = u.bond_stats()
-    v = n * ticks / te
-    log(f"ionic melt: {v:.4e} atom-steps/s ({1e3 * te / ticks:.3f} ms/tick), {n} atoms, tuning {tune}, "
-        f"bound episodes tracked, bonded pairs {bs['bonded_pairs']}")
-    return {"workload": f"{w.name}: Na/O rock-salt melt, bonds observed every tick, interaction on, "
-                        f"g(r) every {gr_every}",
-            "atoms": n, "ticks": ticks, "atom_steps_per_s": v, "ms_per_tick": 1e3 * te / ticks,
-            "bond_components_ms": 1e3 * tc, "bond_structure_ms": 1e3 * ts,
-            "aggregates_at_end": chem["aggregates"], "bonded_pairs_at_end": bs["bonded_pairs"],
-            "tuning": {k: tune[k] for k in ("pair_mode", "lanes_per_atom", "skin")}}
+        for _ in range(3):
+            O.forces(m.physics, m.state, nthreads=th, extras=False)
+        fp[th] = (time.perf_counter() - t0) / 3
+    log(f"cpu force pass, 32768 atoms: " + ", ".join(f"{k} threads {v * 1e3:.1f} ms" for k, v in fp.items()))
+    return {"value": state.n / legs[cores], "unit": "atom-steps/s", "cores": cores, "kind": "port",
+            "sample": f"1 full tick (forces, bond observation, Berendsen) of the {state.n}-atom ionic melt from "
+                      f"the GPU's bonded state after the timed window (tracker episodes start empty), "
+                      f"C++ restatement of the reference CPU algorithm, {legs[cores]:.1f} s with {cores} threads",
+            "one_thread": {"value": state.n / legs[1], "seconds": legs[1]},
+            "speedup_all_over_one": legs[1] / legs[cores],
+            "cpu_model": cpu_model(), "host_cpus_visible": os.cpu_count(),
+            "force_pass_mixture32768_ms": {str(k): 1e3 * v for k, v in fp.items()},
+            "force_pass_speedup_8_over_1": fp[1] / fp[min(8, cores)],
+            "reference_claim": "~3x parallel force pass on 8 cores (README.md:55-56)"}
 
 
 def copy_probe() -> int:
@@ -330,60 +422,51 @@ def copy_probe() -> int:
     return 0
 
 
-def count_pairs(s, ph) -> int:
-    """Ordered pairs within the force cutoff (cell-binned numpy count)."""
-    rc = ph.force_rc()
-    box = np.asarray(ph.box)
-    p = np.stack([s.x, s.y, s.z], 1)
-    p = p - box * np.floor(p / box + 0.5)
-    dims = np.maximum(np.floor(box / rc).astype(int), 1)
-    cell = np.floor((p + 0.5 * box) / (box / dims)).astype(int) % dims
-    key = cell[:, 0] + dims[0] * (cell[:, 1] + dims[1] * cell[:, 2])
-    order = np.argsort(key, kind="stable")
-    ks = key[order]
-    starts = np.searchsorted(ks, np.arange(dims.prod() + 1))
-    total = 0
-    for c in range(int(dims.prod())):
-        a = order[starts[c]:starts[c + 1]]
-        if a.size == 0:
-            continue
-        cx, cy, cz = c % dims[0], (c // dims[0]) % dims[1], c // (dims[0] * dims[1])
-        neigh = []
-        for dz in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dx in (-1, 0, 1):
-                    nc = ((cx + dx) % dims[0]) + dims[0] * (((cy + dy) % dims[1]) + dims[1] * ((cz + dz) % dims[2]))
-                    neigh.append(order[starts[nc]:starts[nc + 1]])
-        b = np.unique(np.concatenate(neigh))
-        d = p[a][:, None, :] - p[b][None, :, :]
-        d -= box * np.round(d / box)
-        r2 = (d * d).sum(-1)
-        total += int(((r2 < rc * rc) & (r2 > 0)).sum())
-    return total
-
-
-def cpu_baseline(w, seconds: float) -> dict:
-    """The oracle's restatement of the reference CPU algorithm (serial grid +
-    half-shell pair list, threaded fold with per-thread buffers merged in
-    order, serial kinetic sum) timed on this host on a bounded sample of the
-    same workload: as many full ticks as fit in ~``seconds``."""
-    from oracle import oracle as O
-    cores = max(1, min(os.cpu_count() or 1, 16))
-    st = w.state
-    O.run(w.physics, st, 1, nthreads=cores, observe=False)  # warm caches
-    steps, t_used = 0, 0.0
-    s = st
-    while t_used < seconds and steps < 400:
-        k = 1 if steps < 2 else max(1, min(20, int((seconds - t_used) / max(t_used / steps, 1e-3))))
-        t0 = time.perf_counter()
-        s, _ = O.run(w.physics, s, k, nthreads=cores, observe=False)
-        t_used += time.perf_counter() - t0
-        steps += k
-    v = st.n * steps / t_used
-    log(f"cpu baseline: {v:.4e} atom-steps/s over {steps} ticks ({t_used:.1f} s, {cores} threads)")
-    return {"value": v, "unit": "atom-steps/s", "cores": cores, "kind": "port",
-            "sample": f"{steps} full ticks of the {st.n}-atom mixture workload ({t_used:.1f} s), "
-                      f"C++ restatement of the reference rayon force pass with {cores} threads"}
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lanes", type=int, default=DEFAULT_LANES)
+    ap.add_argument("--skin-frac", type=float, default=DEFAULT_SKIN_FRAC, help="list skin as a fraction of rc")
+    ap.add_argument("--no-extras", action="store_true", help="skip the mixture, batched-replica and adjoint legs")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 256 MiB device copy (HBM counter calibration) and exit")
+    ap.add_argument("--leg", choices=("ionic", "mixture", "batched", "adjoint"), default=None,
+                    help="run only that leg and print its JSON record, for profiling")
+    args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU only: the path would shard by spatial "
+                                   "domain with halo exchange, but no multi-GPU figure is produced; run with --gpus 1"}))
+        return 2
+    if args.leg == "mixture":
+        print(json.dumps(mixture_leg(steps=args.steps, warmup=args.warmup)), flush=True)
+        return 0
+    if args.leg == "batched":
+        print(json.dumps(batched_leg(ticks=args.steps, segment=min(100, args.steps))), flush=True)
+        return 0
+    if args.leg == "adjoint":
+        print(json.dumps(adjoint_leg(steps=args.steps, ckpt=min(50, args.steps))), flush=True)
+        return 0
+    out, w, fin = ionic_headline(args.steps, args.warmup, args.lanes, args.skin_frac)
+    if args.leg == "ionic":
+        print(json.dumps(out), flush=True)
+        return 0
+    if not args.no_extras:
+        out["mixture"] = mixture_leg()
+        out["batched"] = batched_leg()
+        out["adjoint"] = adjoint_leg()
+    if not args.no_cpu_baseline:
+        st = w.state.copy()
+        st.x, st.y, st.z, st.vx, st.vy, st.vz = fin.x, fin.y, fin.z, fin.vx, fin.vy, fin.vz
+        st.ax, st.ay, st.az = fin.ax, fin.ay, fin.az
+        out["cpu_baseline"] = cpu_baseline(w, st)
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 if __name__ == "__main__":
