@@ -110,7 +110,8 @@ def pair_roofline(kernel, pair_ms, npairs, n, traffic=None):
     return fp64, hbm
 
 
-def ionic_headline(steps: int, warmup: int, lanes: int, skin_frac: float, setup_max: int = 20000):
+def ionic_headline(steps: int, warmup: int, lanes: int, skin_frac: float, setup_max: int = 20000,
+                   aux_bound_search: bool = False):
     from genesis_amd.inputs import ionic_melt
     from genesis_amd.universe import GpuUniverse
     w = ionic_melt()
@@ -121,6 +122,7 @@ def ionic_headline(steps: int, warmup: int, lanes: int, skin_frac: float, setup_
         f"steps={steps} warmup={warmup} lanes={lanes} skin={skin:.4f}")
     u = GpuUniverse(ph, st)
     u.set_tuning(1, lanes, skin=skin, sort_every=100)
+    u.set_bound_search(aux_bound_search)
     # Setup (untimed): the quench until persistent bonds exist.
     t0 = time.perf_counter()
     smp = Sampler(u, 0, w.extra["gr_rmax"], w.extra["gr_bins"])
@@ -169,7 +171,8 @@ def ionic_headline(steps: int, warmup: int, lanes: int, skin_frac: float, setup_
     bfin = u.bond_stats()
     u.close()
     npairs = count_pairs(fin, ph)
-    kernel = f"k_pair_list<{lanes} lanes, bound-pair search> (pair forces + bond term + second kick + Berendsen)"
+    kernel = (f"k_pair_list<{lanes} lanes{'' if aux_bound_search else ', bound-pair search'}> "
+              f"(pair forces + bond term + second kick + Berendsen)")
     traffic = None
     if PAIR_TRAFFIC["fetch_kib"] is not None and (lanes, skin_frac) == (PAIR_TRAFFIC["setting"]["lanes"],
                                                                        PAIR_TRAFFIC["setting"]["skin_frac"]):
@@ -431,6 +434,8 @@ def main() -> int:
     ap.add_argument("--lanes", type=int, default=DEFAULT_LANES)
     ap.add_argument("--skin-frac", type=float, default=DEFAULT_SKIN_FRAC, help="list skin as a fraction of rc")
     ap.add_argument("--no-extras", action="store_true", help="skip the mixture, batched-replica and adjoint legs")
+    ap.add_argument("--aux-bound-search", action="store_true",
+                    help="bond observation's bound pairs from the aux-grid search instead of the list pass")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device copy (HBM counter calibration) and exit")
     ap.add_argument("--leg", choices=("ionic", "mixture", "batched", "adjoint"), default=None,
@@ -451,7 +456,8 @@ def main() -> int:
     if args.leg == "adjoint":
         print(json.dumps(adjoint_leg(steps=args.steps, ckpt=min(50, args.steps))), flush=True)
         return 0
-    out, w, fin = ionic_headline(args.steps, args.warmup, args.lanes, args.skin_frac)
+    out, w, fin = ionic_headline(args.steps, args.warmup, args.lanes, args.skin_frac,
+                                 aux_bound_search=args.aux_bound_search)
     if args.leg == "ionic":
         print(json.dumps(out), flush=True)
         return 0

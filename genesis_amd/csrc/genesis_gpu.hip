@@ -442,15 +442,27 @@ __device__ __forceinline__ void pair_law_sp(const KParams& P, double sig_i, doub
 // operand's result for several issue slots, so the four independent chains
 // are interleaved to fill them.  p: the four neighbour positions; tt: their
 // species + 1 in successive bytes, 0 for a slot past the end of the list.
-template <bool COUL, bool SHARED, bool GRAV>
+// Bound-pair screen of the list pass: k_bind^2 narrowed and widened by
+// 1e-10 relative.
+struct BoundScreen {
+    double kb2_lo = 0.0, kb2_hi = 0.0;  // k_bind^2 (1 -+ 1e-10)
+};
+
+// BOUND: bits 0-3 of hitm mark the pairs that are bound pairs for certain
+// (r^2 at least 1e-10 relative inside (k_bind sigma_ij)^2, so no rounding of
+// the reference's own expressions can move them out), bits 4-7 the pairs
+// within that margin of the threshold, which the exact test decides.
+template <bool COUL, bool SHARED, bool GRAV, bool BOUND = false>
 __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double Ly, double Lz, double iLx, double iLy,
                                            double iLz, const double4& pi, double kqi, double sig_i, double a4_i,
                                            const SpCoef* ssp, const double* irc_row, const PairCoef* row,
                                            const double4 (&p)[4], uint32_t tt, double& fx, double& fy, double& fz,
-                                           double& vl) {
+                                           double& vl, const BoundScreen& bs = BoundScreen{},
+                                           uint32_t* hitm = nullptr) {
     bool live[4], ok[4], core[4];
     int tj[4];
     double dx[4], dy[4], dz[4], r2[4], y[4], e[4], inv_d[4], d[4], sig[4], eps4[4], irc[4], r[4], inv_r[4];
+    double traw[4];
     double s6[4], s12[4], vlj[4], flj[4], sw[4], dsw[4], fmag[4], vv[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -475,6 +487,7 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
         const double t = fma(dz[q], dz[q], fma(dy[q], dy[q], dx[q] * dx[q]));
         ok[q] = live[q] && t > P.eps2;
         r2[q] = ok[q] ? t : 1.0;
+        if (BOUND) traw[q] = t;
     }
     // rsqrt_refined, four at a time
 #pragma unroll
@@ -490,6 +503,20 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
         irc[q] = irc_row[tj[q]];
         sig[q] = 0.5 * (sig_i + cj.sigma);
         eps4[q] = a4_i * cj.a4;
+    }
+    if (BOUND) {
+        uint32_t m = 0u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            // (k_bind sigma_ab <= k_bind max sigma: inside the pair's own
+            // threshold is inside the cutoff)
+            const double t = traw[q], th2 = sig[q] * sig[q];
+            const bool sure = live[q] && t > 1e-24 && t < th2 * bs.kb2_lo;
+            const bool maybe = live[q] && t < th2 * bs.kb2_hi;
+            m |= (sure ? 1u : 0u) << q;
+            m |= (maybe && !sure ? 1u : 0u) << (4 + q);
+        }
+        *hitm = m;
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -1789,13 +1816,9 @@ __device__ __forceinline__ void close_list_tick(uint32_t* flags) {
 constexpr int BOUND_MAXT = 32;  // bound partners kept per owner (the tracker's episodes)
 constexpr int HIT_LANE = 16;    // screened candidates a lane's LDS column holds
 struct BoundArgs {
-    const uint32_t* id = nullptr;        // entity id of each resident atom
-    const double* thr = nullptr;         // k_bind sigma_ab, [a * 10 + b]
-    uint32_t* nowp = nullptr;            // partners by entity, slot-major [q * n + e]
-    uint32_t* nown = nullptr;            // partner count of entity e
-    unsigned long long* cnt = nullptr;   // [4]: an owner with more than BOUND_MAXT partners
-    double cd2 = 0.0, cw2 = 0.0;         // cutoff^2 and the screen's cutoff^2 (1 + 1e-12)
-    double kbind = 0.0;                  // k_bind: the screen also takes d < k_bind sigma_ab
+    uint32_t* hits = nullptr;   // [h * (N << LOGT) + lane]: the lane's h-th screened list entry
+    uint32_t* hinfo = nullptr;  // [lane]: screened entries (clamped at 255) | band mask << 16
+    BoundScreen scr;            // the list pass's screen around k_bind sigma_ab
 };
 
 // Per-atom list traversal fused with the second half of the tick: T lanes
@@ -1812,8 +1835,11 @@ struct BoundArgs {
 // pairs are skipped.  The kernel stores a = F/m and the atom's potential
 // energy (half its pair sum) in pe_out; the kick and the per-replica
 // reductions run in the batch's own kernels.
+#ifndef GS_LIST_WAVES
+#define GS_LIST_WAVES 2  // minimum waves per SIMD the list pair kernel is compiled for
+#endif
 template <int LOGT, bool COUL, bool SHARED, bool SINGLE, bool GRAV, bool BATCH = false, bool BOUND = false>
-__global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, const double4* __restrict__ pos4,
+__global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, uint32_t N, const double4* __restrict__ pos4,
                                                      const uint8_t* __restrict__ type,
                                                      const uint32_t* __restrict__ nbr,
                                                      const uint32_t* __restrict__ nlen,
@@ -1838,10 +1864,12 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
     __shared__ bool last;
     // BOUND: each lane's screened bound-pair candidates (list entries), a
     // column per lane, then its accepted partners' entity ids
-    __shared__ uint32_t hl[BOUND ? HIT_LANE : 1][BOUND ? BLOCK : 1];
-    __shared__ uint32_t hn[BOUND ? BLOCK : 1];
+    // (slot HIT_LANE is the spare that the branch-free recording writes
+    // once the column is full; it is never read)
+    __shared__ uint32_t hl[BOUND ? HIT_LANE + 1 : 1][BOUND ? BLOCK : 1];
     const int tid = threadIdx.x;
-    uint32_t nhit = 0u;  // BOUND: screened candidates of this lane
+    uint32_t nhit = 0u;   // BOUND: screened candidates of this lane
+    uint32_t bandm = 0u;  // BOUND: column entries within 1e-10 of a threshold
     // Atom chunk of this workgroup.  Workgroup b runs on L2 slice b mod 8
     // and, when the grid holds rpc workgroups per CU of a 256-CU device,
     // shares its CU with b + 256, b + 512, ...: those get adjacent chunks
@@ -1933,31 +1961,22 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
         const uint32_t ng = (cnt + 3) >> 2;  // groups of this lane
         const uint4* glast = grp + (size_t)(ng ? ng - 1 : 0) * gstride;
         auto group_ptr = [&](uint32_t k) { return k < ng ? grp + (size_t)k * gstride : glast; };
-        // BOUND: an entry within the bond cutoff (the screen of
-        // bound_pairs_with_grid on the r^2 the pair law uses) joins the
-        // lane's column
-        auto screen = [&](const double4 (&p)[4], uint32_t tt, const uint4& ee) {
+        // BOUND: the pairs the pair law's screen marks (hitm: bound for
+        // certain, or within 1e-10 of a threshold) join the lane's column;
+        // bandm marks the column entries the exact test must decide
+        // (branch-free: every entry is stored at the next free slot, which
+        // only a hit keeps; once the column's HIT_LANE slots are taken the
+        // entries go to the spare slot, and a column with more hits than
+        // slots is never read: the host replays the call)
+        auto record = [&](uint32_t hitm, const uint4& ee) {
             const uint32_t f4[4] = {ee.x, ee.y, ee.z, ee.w};
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const bool live = ((tt >> (8 * q)) & 0xFFu) != 0u;
-                double dx = pi.x - p[q].x, dy = pi.y - p[q].y, dz = pi.z - p[q].z;
-                dx = fma(-Lx, rint(dx * iLx), dx);
-                dy = fma(-Ly, rint(dy * iLy), dy);
-                dz = fma(-Lz, rint(dz * iLz), dz);
-                const double t = fma(dz, dz, fma(dy, dy, dx * dx));
-                bool hit = live && t < ba.cw2;
-                if (!SINGLE) {
-                    // the pair's own threshold, from the mixing rule's sigma_ij
-                    // (1e-10 wider, so the exact test decides every pair it keeps)
-                    const uint32_t tq = (tt >> (8 * q)) & 0xFFu;
-                    const double th = ba.kbind * (0.5 * (sig_i + ssp[tq ? tq - 1u : 0u].sigma));
-                    hit = hit && t < th * th * (1.0 + 1e-10);
-                }
-                if (hit) {
-                    if (nhit < (uint32_t)HIT_LANE) hl[nhit][tid] = f4[q];
-                    ++nhit;
-                }
+                const uint32_t slot = min(nhit, (uint32_t)HIT_LANE);
+                const uint32_t h = (hitm >> q) & 0x11u;
+                hl[slot][tid] = f4[q];
+                bandm |= ((h >> 4) & 1u) << slot;
+                nhit += h != 0u ? 1u : 0u;
             }
         };
         auto gather = [&](const uint4& e, uint32_t k, double4 (&p)[4], uint32_t& tt) {
@@ -1970,7 +1989,8 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
                 tt |= (live ? (f4[q] >> 28) + 1u : 0u) << (8 * q);  // 0: past the end
             }
         };
-        auto evaluate = [&](const double4 (&p)[4], uint32_t tt) {
+        auto evaluate = [&](const double4 (&p)[4], uint32_t tt) -> uint32_t {
+            uint32_t hitm = 0u;
             if (SINGLE) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
@@ -1981,6 +2001,13 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
                     dy = fma(-Ly, rint(dy * iLy), dy);
                     dz = fma(-Lz, rint(dz * iLz), dz);
                     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                    if (BOUND) {
+                        const double th2 = row[0].sigma * row[0].sigma;
+                        const bool sure = live && r2 > 1e-24 && r2 < th2 * ba.scr.kb2_lo;
+                        const bool maybe = live && r2 < th2 * ba.scr.kb2_hi;
+                        hitm |= (sure ? 1u : 0u) << q;
+                        hitm |= (maybe && !sure ? 1u : 0u) << (4 + q);
+                    }
                     const bool ok = live && r2 > P.eps2;
                     const double r2s = ok ? r2 : 1.0;
                     const double inv_d = rsqrt_refined(r2s);
@@ -1994,10 +2021,11 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
                     vl += ok ? vv : 0.0;
                 }
             } else {
-                pair_eval4<COUL, SHARED, GRAV>(P, Lx, Ly, Lz, iLx, iLy, iLz, pi, kqi, sig_i, a4_i, ssp,
-                                               sirc + ti * GS_NSPECIES, row, p, tt, fx, fy,
-                                               fz, vl);
+                pair_eval4<COUL, SHARED, GRAV, BOUND>(P, Lx, Ly, Lz, iLx, iLy, iLz, pi, kqi, sig_i, a4_i, ssp,
+                                                      sirc + ti * GS_NSPECIES, row, p, tt, fx, fy, fz, vl, ba.scr,
+                                                      &hitm);
             }
+            return hitm;
         };
         uint4 eA = load_nt(group_ptr(0));
         uint4 eB = load_nt(group_ptr(1));
@@ -2019,16 +2047,20 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
             gather(eB, k + 1, pB, tB);
             if (BOUND) jB = eB;
             __builtin_amdgcn_sched_barrier(0);
-            evaluate(pA, tA);
-            if (BOUND) screen(pA, tA, jA);
+            {
+                const uint32_t hm = evaluate(pA, tA);
+                if (BOUND) record(hm, jA);
+            }
             // step B: group k + 1
             __builtin_amdgcn_sched_barrier(0);
             eB = load_nt(group_ptr(k + 3));
             gather(eA, k + 2, pA, tA);
             if (BOUND) jA = eA;
             __builtin_amdgcn_sched_barrier(0);
-            evaluate(pB, tB);
-            if (BOUND) screen(pB, tB, jB);
+            {
+                const uint32_t hm = evaluate(pB, tB);
+                if (BOUND) record(hm, jB);
+            }
         }
     }
 #pragma unroll
@@ -2038,69 +2070,16 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
         fz += __shfl_xor(fz, off, WAVE);
         if (BATCH) vl += __shfl_xor(vl, off, WAVE);
     }
-    if (BOUND) {
-        // Exact test of bound_pairs_with_grid on the screened candidates (the
-        // reference's min_image and length): a larger entity id, d^2 <
-        // cutoff^2, d > EPS, d < k_bind sigma_ab.  Each lane tests its own
-        // column and keeps the accepted ids in it; then every accepted id
-        // takes its rank among the atom's accepted ids (all distinct) and is
-        // written to that slot, slot-major as the aux-grid search writes
-        // them, so the partners come out in ascending id order.
-        uint32_t e_i = 0u, nacc = 0u;
-        if (i < N) {
-            e_i = ba.id[i];
-            const uint32_t nh = min(nhit, (uint32_t)HIT_LANE);
-            // four hits at a time: their gathers are in flight together (an
-            // accepted id is written at or before its own slot, never ahead
-            // of the batch being read)
-            for (uint32_t h0 = 0; h0 < nh; h0 += 4) {
-                uint32_t f[4], em[4];
-                double4 q[4];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) f[k] = hl[min(h0 + (uint32_t)k, nh - 1u)][tid];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const uint32_t j = f[k] & NBR_IDX_MASK;
-                    q[k] = pos4[j];
-                    em[k] = ba.id[j];
-                }
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const double dx = min_image1(pi0.x - q[k].x, P.box[0]);
-                    const double dy = min_image1(pi0.y - q[k].y, P.box[1]);
-                    const double dz = min_image1(pi0.z - q[k].z, P.box[2]);
-                    const double d2 = dx * dx + dy * dy + dz * dz;
-                    const double d = sqrt(d2);
-                    if (h0 + (uint32_t)k < nh && em[k] > e_i && d2 < ba.cd2 && d > F64_EPS &&
-                        d < ba.thr[ti0 * GS_NSPECIES + (f[k] >> 28)]) {
-                        hl[nacc][tid] = em[k];
-                        ++nacc;
-                    }
-                }
-            }
-            if (nhit > (uint32_t)HIT_LANE) atomicOr(&flags[7], 1u);
-        }
-        hn[tid] = nacc;
-        __syncthreads();
-        if (i < N) {
-            const int first = tid & ~(T - 1);  // the atom's first lane
-            uint32_t m = 0u;
-#pragma unroll
-            for (int l = 0; l < T; ++l) m += hn[first + l];
-            for (uint32_t k = 0; k < nacc; ++k) {
-                const uint32_t v = hl[k][tid];
-                uint32_t r = 0u;
-                for (int l = 0; l < T; ++l) {
-                    const uint32_t c = hn[first + l];
-                    for (uint32_t kk = 0; kk < c; ++kk) r += hl[kk][first + l] < v ? 1u : 0u;
-                }
-                if (r < (uint32_t)BOUND_MAXT) ba.nowp[(size_t)r * N + e_i] = v;
-            }
-            if (s == 0) {
-                ba.nown[e_i] = min(m, (uint32_t)BOUND_MAXT);
-                if (m > (uint32_t)BOUND_MAXT) atomicOr(ba.cnt + 4, 1ull);
-            }
-        }
+    if (BOUND && i < N) {
+        // the lane's column goes to global memory (k_bound_from_hits applies
+        // the owner test and the exact test and orders the partners); a
+        // column that overflowed is not stored, the call is replayed with the
+        // aux-grid search
+        const size_t NT = (size_t)N << LOGT;
+        const uint32_t nh = nhit > (uint32_t)HIT_LANE ? 0u : nhit;
+        for (uint32_t h = 0; h < nh; ++h) ba.hits[(size_t)h * NT + g] = hl[h][tid];
+        ba.hinfo[g] = (bandm << 16) | min(nhit, 255u);
+        if (nhit > (uint32_t)HIT_LANE) atomicOr(&flags[7], 1u);
     }
     if (BATCH) {
         if (own && s == 0) {
@@ -2155,6 +2134,85 @@ __global__ __launch_bounds__(BLOCK) void k_pair_list(KParams P, uint32_t N, cons
         close_tick(K, V, Vb, (int)N, th, lam_out, obs);
         close_list_tick(flags);
     }
+}
+
+// Bound pairs of the fused search, one thread per resident atom i: the
+// screened entries of its T lanes (k_pair_list<.., BOUND>) pass the owner
+// test (a larger entity id) and, when the screen put them within 1e-10 of the
+// threshold, bound_pairs_with_grid's exact test (the reference's min_image
+// with its divisions, IEEE sqrt, d^2 < cutoff^2, d > EPS, d < k_bind
+// sigma_ab, src/analysis/pairs.rs:62-85 and :132-174).  The accepted
+// partners are ordered in the thread's LDS row and written slot-major by
+// entity, as the aux-grid search writes them.
+template <int LOGT>
+__global__ __launch_bounds__(BLOCK) void k_bound_from_hits(KParams P, uint32_t N, const double4* __restrict__ pos4,
+                                                           const uint8_t* __restrict__ type,
+                                                           const uint32_t* __restrict__ id,
+                                                           const uint32_t* __restrict__ hits,
+                                                           const uint32_t* __restrict__ hinfo,
+                                                           const double* __restrict__ thr, double cd2,
+                                                           uint32_t* __restrict__ nowp, uint32_t* __restrict__ nown,
+                                                           unsigned long long* __restrict__ cnt) {
+    constexpr int T = 1 << LOGT;
+    __shared__ uint32_t rows[BOUND_MAXT][BLOCK];
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= N) return;
+    const size_t NT = (size_t)N << LOGT;
+    const uint32_t e = id[i];
+    const int ti = type[i];
+    uint32_t m = 0u;
+    bool pos_loaded = false;
+    double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
+    uint32_t info[T];
+#pragma unroll
+    for (int s = 0; s < T; ++s) info[s] = hinfo[(i << LOGT) + (uint32_t)s];
+#pragma unroll
+    for (int s = 0; s < T; ++s) {
+        const uint32_t g = (i << LOGT) + (uint32_t)s;
+        const uint32_t c = info[s] & 0xFFu, band = info[s] >> 16;
+        const uint32_t nh = c > (uint32_t)HIT_LANE ? 0u : c;  // an overflowed column is replayed
+        // four hits at a time: their entry loads, then their id gathers, are
+        // in flight together
+        for (uint32_t h0 = 0; h0 < nh; h0 += 4) {
+            uint32_t f[4], em[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) f[k] = hits[(size_t)min(h0 + (uint32_t)k, nh - 1u) * NT + g];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) em[k] = id[f[k] & NBR_IDX_MASK];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t h = h0 + (uint32_t)k;
+                bool acc = h < nh && em[k] > e;
+                if (acc && ((band >> h) & 1u)) {
+                    if (!pos_loaded) {
+                        pi = pos4[i];
+                        pos_loaded = true;
+                    }
+                    const double4 q = pos4[f[k] & NBR_IDX_MASK];
+                    const double dx = min_image1(pi.x - q.x, P.box[0]);
+                    const double dy = min_image1(pi.y - q.y, P.box[1]);
+                    const double dz = min_image1(pi.z - q.z, P.box[2]);
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    const double d = sqrt(d2);
+                    acc = d2 < cd2 && d > F64_EPS && d < thr[ti * GS_NSPECIES + (f[k] >> 28)];
+                }
+                if (!acc) continue;
+                if (m < (uint32_t)BOUND_MAXT) {
+                    uint32_t w = m;  // insertion in ascending order
+                    while (w > 0 && rows[w - 1][threadIdx.x] > em[k]) {
+                        rows[w][threadIdx.x] = rows[w - 1][threadIdx.x];
+                        --w;
+                    }
+                    rows[w][threadIdx.x] = em[k];
+                }
+                ++m;
+            }
+        }
+    }
+    const uint32_t mm = min(m, (uint32_t)BOUND_MAXT);
+    for (uint32_t r = 0; r < mm; ++r) nowp[(size_t)r * N + e] = rows[r][threadIdx.x];
+    nown[e] = mm;
+    if (m > (uint32_t)BOUND_MAXT) atomicOr(cnt + 4, 1ull);
 }
 
 // Exclusive prefix sum of one value per thread over a workgroup of NT
@@ -2855,7 +2913,8 @@ struct gs_universe {
     bool bound_fused = false;    // this attempt of a gs_step call uses it
     bool bound_unfused = false;  // a hit column overflowed: the call replays on the aux grid
     bool bound_aux_only = false;  // gs_set_bound_search(u, 1)
-    double bound_cd2 = 0.0, bound_cw2 = 0.0;
+    double bound_cd2 = 0.0;      // cutoff^2 of the bound-pair search
+    BoundScreen bound_scr;       // the fused search's screen thresholds
 };
 
 namespace {
