@@ -448,11 +448,14 @@ struct BoundScreen {
     double kb2_lo = 0.0, kb2_hi = 0.0;  // k_bind^2 (1 -+ 1e-10)
 };
 
+// Q0, NQ: the pairs q0 .. q0 + nq - 1 of the group (a group may be evaluated
+// as two halves, which holds fewer values live at once; the sums take the
+// pairs in the same order either way).
 // BOUND: bits 0-3 of hitm mark the pairs that are bound pairs for certain
 // (r^2 at least 1e-10 relative inside (k_bind sigma_ij)^2, so no rounding of
 // the reference's own expressions can move them out), bits 4-7 the pairs
 // within that margin of the threshold, which the exact test decides.
-template <bool COUL, bool SHARED, bool GRAV, bool BOUND = false>
+template <bool COUL, bool SHARED, bool GRAV, bool BOUND = false, int Q0 = 0, int NQ = 4>
 __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double Ly, double Lz, double iLx, double iLy,
                                            double iLz, const double4& pi, double kqi, double sig_i, double a4_i,
                                            const SpCoef* ssp, const double* irc_row, const PairCoef* row,
@@ -465,25 +468,25 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
     double traw[4];
     double s6[4], s12[4], vlj[4], flj[4], sw[4], dsw[4], fmag[4], vv[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = Q0; q < Q0 + NQ; ++q) {
         const uint32_t tq = (tt >> (8 * q)) & 0xFFu;
         live[q] = tq != 0u;
         tj[q] = live[q] ? (int)tq - 1 : 0;
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = Q0; q < Q0 + NQ; ++q) {
         dx[q] = pi.x - p[q].x;
         dy[q] = pi.y - p[q].y;
         dz[q] = pi.z - p[q].z;
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = Q0; q < Q0 + NQ; ++q) {
         dx[q] = fma(-Lx, rint(dx[q] * iLx), dx[q]);
         dy[q] = fma(-Ly, rint(dy[q] * iLy), dy[q]);
         dz[q] = fma(-Lz, rint(dz[q] * iLz), dz[q]);
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = Q0; q < Q0 + NQ; ++q) {
         const double t = fma(dz[q], dz[q], fma(dy[q], dy[q], dx[q] * dx[q]));
         ok[q] = live[q] && t > P.eps2;
         r2[q] = ok[q] ? t : 1.0;
@@ -491,13 +494,13 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
     }
     // rsqrt_refined, four at a time
 #pragma unroll
-    for (int q = 0; q < 4; ++q) y[q] = __builtin_amdgcn_rsq(r2[q]);
+    for (int q = Q0; q < Q0 + NQ; ++q) y[q] = __builtin_amdgcn_rsq(r2[q]);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) e[q] = fma(-r2[q] * y[q], y[q], 1.0);
+    for (int q = Q0; q < Q0 + NQ; ++q) e[q] = fma(-r2[q] * y[q], y[q], 1.0);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) inv_d[q] = fma(y[q] * e[q], fma(0.375, e[q], 0.5), y[q]);
+    for (int q = Q0; q < Q0 + NQ; ++q) inv_d[q] = fma(y[q] * e[q], fma(0.375, e[q], 0.5), y[q]);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = Q0; q < Q0 + NQ; ++q) {
         d[q] = r2[q] * inv_d[q];
         const SpCoef cj = ssp[tj[q]];
         irc[q] = irc_row[tj[q]];
@@ -507,7 +510,7 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
     if (BOUND) {
         uint32_t m = 0u;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = Q0; q < Q0 + NQ; ++q) {
             // (k_bind sigma_ab <= k_bind max sigma: inside the pair's own
             // threshold is inside the cutoff)
             const double t = traw[q], th2 = sig[q] * sig[q];
@@ -516,32 +519,32 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
             m |= (sure ? 1u : 0u) << q;
             m |= (maybe && !sure ? 1u : 0u) << (4 + q);
         }
-        *hitm = m;
+        *hitm |= m;
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = Q0; q < Q0 + NQ; ++q) {
         const double rcore = 0.5 * sig[q];
         core[q] = d[q] < rcore;
         r[q] = core[q] ? rcore : d[q];
         inv_r[q] = core[q] ? irc[q] : inv_d[q];
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = Q0; q < Q0 + NQ; ++q) {
         const double s = sig[q] * inv_r[q];
         const double s2 = s * s;
         s6[q] = s2 * s2 * s2;
         s12[q] = s6[q] * s6[q];
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = Q0; q < Q0 + NQ; ++q) {
         vlj[q] = eps4[q] * (s12[q] - s6[q]);
         flj[q] = (6.0 * eps4[q] * inv_r[q]) * fma(2.0, s12[q], -s6[q]);
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) switch_q(r[q], P.lj_ron, P.lj_inv_w, sw[q], dsw[q]);
+    for (int q = Q0; q < Q0 + NQ; ++q) switch_q(r[q], P.lj_ron, P.lj_inv_w, sw[q], dsw[q]);
     if (COUL && SHARED) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = Q0; q < Q0 + NQ; ++q) {
             const bool cc = d[q] < 0.5;
             const double inv_c = cc ? 2.0 : inv_d[q];
             const double vc = (kqi * p[q].w) * inv_c;
@@ -552,13 +555,13 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
         }
     } else {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = Q0; q < Q0 + NQ; ++q) {
             fmag[q] = fma(flj[q], sw[q], -vlj[q] * dsw[q]);
             vv[q] = vlj[q] * sw[q];
         }
         if (COUL) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = Q0; q < Q0 + NQ; ++q) {
                 const bool cc = d[q] < 0.5;
                 const double inv_c = cc ? 2.0 : inv_d[q];
                 const double vc = (kqi * p[q].w) * inv_c;
@@ -572,7 +575,7 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
     }
     if (GRAV) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = Q0; q < Q0 + NQ; ++q) {
             const bool cc = d[q] < 0.5;
             const double inv_g = cc ? 2.0 : inv_d[q];
             const double vg = -(P.gconst * row[tj[q]].mm) * inv_g;
@@ -584,7 +587,7 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
         }
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = Q0; q < Q0 + NQ; ++q) {
         const double fs = ok[q] ? fmag[q] * inv_d[q] : 0.0;
         fx = fma(fs, dx[q], fx);
         fy = fma(fs, dy[q], fy);
@@ -1838,6 +1841,9 @@ struct BoundArgs {
 #ifndef GS_LIST_WAVES
 #define GS_LIST_WAVES 2  // minimum waves per SIMD the list pair kernel is compiled for
 #endif
+#ifndef GS_PAIR_SPLIT
+#define GS_PAIR_SPLIT 1  // a group's four pairs are evaluated as two halves (0: all four at once)
+#endif
 template <int LOGT, bool COUL, bool SHARED, bool SINGLE, bool GRAV, bool BATCH = false, bool BOUND = false>
 __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, uint32_t N, const double4* __restrict__ pos4,
                                                      const uint8_t* __restrict__ type,
@@ -2021,9 +2027,18 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
                     vl += ok ? vv : 0.0;
                 }
             } else {
+#if GS_PAIR_SPLIT
+                pair_eval4<COUL, SHARED, GRAV, BOUND, 0, 2>(P, Lx, Ly, Lz, iLx, iLy, iLz, pi, kqi, sig_i, a4_i, ssp,
+                                                            sirc + ti * GS_NSPECIES, row, p, tt, fx, fy, fz, vl,
+                                                            ba.scr, &hitm);
+                pair_eval4<COUL, SHARED, GRAV, BOUND, 2, 2>(P, Lx, Ly, Lz, iLx, iLy, iLz, pi, kqi, sig_i, a4_i, ssp,
+                                                            sirc + ti * GS_NSPECIES, row, p, tt, fx, fy, fz, vl,
+                                                            ba.scr, &hitm);
+#else
                 pair_eval4<COUL, SHARED, GRAV, BOUND>(P, Lx, Ly, Lz, iLx, iLy, iLz, pi, kqi, sig_i, a4_i, ssp,
                                                       sirc + ti * GS_NSPECIES, row, p, tt, fx, fy, fz, vl, ba.scr,
                                                       &hitm);
+#endif
             }
             return hitm;
         };
