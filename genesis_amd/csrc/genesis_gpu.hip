@@ -2911,10 +2911,9 @@ struct gs_universe {
     double* adj_acc = nullptr;
     double* adj_gatom = nullptr;
     double* adj_spart = nullptr;  // species-reduction partials, 32 per block
-    double* adj_rows = nullptr;
-    size_t adj_rows_cap = 0;
+    unsigned long long* adj_hist = nullptr;  // smoothed g(r): fixed-point bin sums
+    size_t adj_nbins_cap = 0;
     double* adj_ck = nullptr;
-    double* adj_g = nullptr;
     double* adj_l[3] = {nullptr, nullptr, nullptr};
     double* adj_o[3] = {nullptr, nullptr, nullptr};
     double* adj_lv[3] = {nullptr, nullptr, nullptr};

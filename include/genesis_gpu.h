@@ -333,10 +333,13 @@ int gs_bond_reset(gs_universe* u);
  * recomputed segment reproduces the forward bit for bit).  LJ + Coulomb with
  * the Coulomb cutoff equal to the LJ cutoff; gravity and bonds off. */
 typedef struct gs_gr_spec {
-    int32_t nbins;    /* Gaussian-smoothed g(r): bins on [0, r_max)  */
+    int32_t nbins;    /* Gaussian-smoothed g(r): bins on [0, r_max), 1..8192 */
     int32_t pad;
     double r_max;
-    double sigma_b;   /* Gaussian width (same length unit)          */
+    double sigma_b;   /* Gaussian width (same length unit); the kernel is cut
+                         where it is below 1.3e-14 of its peak: a pair feeds
+                         the bins whose centre lies within 8 sigma_b of its
+                         distance                                          */
 } gs_gr_spec;
 
 /* One adjoint pass at the resident positions: with u = lam / m,

@@ -15,9 +15,13 @@
  *
  * Trajectory: velocity Verlet with a_0 = a(x_0), wrap after each drift.
  * Loss: mean V over ticks [win_from, T] + sum_k (g_k(x_T) - g0_k)^2 with a
- * Gaussian-smoothed g(r).  The gradient is the discrete adjoint of the
- * kick-drift-kick map; the two kicks at the same positions are merged, so
- * each tick costs one Hessian-vector product and one parameter VJP.
+ * Gaussian-smoothed g(r): g_k = sum_{i<j} c_k K(d_ij - r_k), K the normal
+ * density of width sigma_b cut where it falls below 1.3e-14 of its peak
+ * (bin centres within 8 sigma_b of d_ij).  Pairs are enumerated on a cell
+ * grid (all pairs when an axis has fewer than 3 cells).  The gradient is the
+ * discrete adjoint of the kick-drift-kick map; the two kicks at the same
+ * positions are merged, so each tick costs one Hessian-vector product and one
+ * parameter VJP.
  *
  * Pinned by: finite differences of every per-pair derivative, of the HVP
  * (forces at x +- h w), of the parameter VJP, and central differences of
@@ -152,22 +156,89 @@ struct Model {
     }
 };
 
-// Accelerations and V (all pairs, O(n^2): the adjoint tests use small n).
+// Every unordered pair (i < j) closer than `cutoff` (minimum image): a cell
+// grid of edge >= cutoff when every axis has at least 3 cells, else all
+// pairs.  The set is exactly the pairs the all-pairs loop would keep; only
+// the order differs (sums then differ in rounding only).
+struct PairList {
+    std::vector<int> a, b;
+    std::vector<double> d;
+    std::vector<P3> dv;
+};
+
+void pair_list(const Model& M, const std::vector<P3>& x, double cutoff, bool keep_zero, PairList& out) {
+    out.a.clear();
+    out.b.clear();
+    out.d.clear();
+    out.dv.clear();
+    int dims[3];
+    for (int k = 0; k < 3; ++k) dims[k] = std::max(1, (int)std::floor(M.box[k] / cutoff));
+    auto consider = [&](int i, int j) {
+        P3 dv = M.mi(x[i], x[j]);
+        double d = std::sqrt(dv.x * dv.x + dv.y * dv.y + dv.z * dv.z);
+        if (d >= cutoff) return;
+        if (!keep_zero && d <= 2.220446049250313e-16) return;
+        out.a.push_back(std::min(i, j));
+        out.b.push_back(std::max(i, j));
+        if (i < j) {
+            out.d.push_back(d);
+            out.dv.push_back(dv);
+        } else {
+            out.d.push_back(d);
+            out.dv.push_back(P3{-dv.x, -dv.y, -dv.z});
+        }
+    };
+    if (dims[0] < 3 || dims[1] < 3 || dims[2] < 3) {
+        for (int i = 0; i < M.n; ++i)
+            for (int j = i + 1; j < M.n; ++j) consider(i, j);
+        return;
+    }
+    const int ncell = dims[0] * dims[1] * dims[2];
+    std::vector<std::vector<int>> cell(ncell);
+    auto cidx = [&](const P3& p, int c[3]) {
+        const double v[3] = {p.x, p.y, p.z};
+        for (int k = 0; k < 3; ++k) {
+            double u = std::fmod(v[k] + 0.5 * M.box[k], M.box[k]);
+            if (u < 0.0) u += M.box[k];
+            c[k] = std::min(dims[k] - 1, (int)std::floor(u / (M.box[k] / dims[k])));
+        }
+    };
+    for (int i = 0; i < M.n; ++i) {
+        int c[3];
+        cidx(x[i], c);
+        cell[c[0] + dims[0] * (c[1] + dims[1] * c[2])].push_back(i);
+    }
+    for (int i = 0; i < M.n; ++i) {
+        int c[3];
+        cidx(x[i], c);
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int ix = (c[0] + dx + dims[0]) % dims[0], iy = (c[1] + dy + dims[1]) % dims[1],
+                              iz = (c[2] + dz + dims[2]) % dims[2];
+                    for (int j : cell[ix + dims[0] * (iy + dims[1] * iz)])
+                        if (j > i) consider(i, j);
+                }
+    }
+}
+
+// Accelerations and V over the pairs within rc.
 double accel(const Model& M, const std::vector<P3>& x, std::vector<P3>& a) {
     a.assign(M.n, P3{0, 0, 0});
     double V = 0.0;
-    for (int i = 0; i < M.n; ++i)
-        for (int j = i + 1; j < M.n; ++j) {
-            P3 dv = M.mi(x[i], x[j]);
-            double d = std::sqrt(dv.x * dv.x + dv.y * dv.y + dv.z * dv.z);
-            if (d >= M.rc || d <= 2.220446049250313e-16) continue;
-            PairD o;
-            M.pd(i, j, d, o);
-            V += o.phi;
-            double f = -o.d1 / d;  // along the unnormalised displacement
-            a[i].x += f * dv.x; a[i].y += f * dv.y; a[i].z += f * dv.z;
-            a[j].x -= f * dv.x; a[j].y -= f * dv.y; a[j].z -= f * dv.z;
-        }
+    PairList pl;
+    pair_list(M, x, M.rc, false, pl);
+    for (size_t k = 0; k < pl.a.size(); ++k) {
+        const int i = pl.a[k], j = pl.b[k];
+        const P3 dv = pl.dv[k];
+        const double d = pl.d[k];
+        PairD o;
+        M.pd(i, j, d, o);
+        V += o.phi;
+        double f = -o.d1 / d;  // along the unnormalised displacement
+        a[i].x += f * dv.x; a[i].y += f * dv.y; a[i].z += f * dv.z;
+        a[j].x -= f * dv.x; a[j].y -= f * dv.y; a[j].z -= f * dv.z;
+    }
     for (int i = 0; i < M.n; ++i) {
         double im = 1.0 / M.mass[M.type[i]];
         a[i].x *= im; a[i].y *= im; a[i].z *= im;
@@ -183,41 +254,44 @@ void adjoint_pass(const Model& M, const std::vector<P3>& x, const std::vector<P3
                   std::vector<P3>& out_x, double* gth) {
     out_x.assign(M.n, P3{0, 0, 0});
     double grc = 0.0;
-    for (int i = 0; i < M.n; ++i)
-        for (int j = 0; j < M.n; ++j) {
-            if (i == j) continue;
-            P3 dv = M.mi(x[i], x[j]);
-            double d = std::sqrt(dv.x * dv.x + dv.y * dv.y + dv.z * dv.z);
-            if (d >= M.rc || d <= 2.220446049250313e-16) continue;
-            PairD o;
-            M.pd(i, j, d, o);
-            const int ti = M.type[i], tj = M.type[j];
-            P3 r{dv.x / d, dv.y / d, dv.z / d};
-            const double imi = 1.0 / M.mass[ti], imj = 1.0 / M.mass[tj];
-            P3 du{lam[i].x * imi - lam[j].x * imj, lam[i].y * imi - lam[j].y * imj, lam[i].z * imi - lam[j].z * imj};
-            const double rdu = r.x * du.x + r.y * du.y + r.z * du.z;
-            // K du with K = f' rr^T + (f/d)(I - rr^T), f = -d1, f' = -d2
-            const double fprime = -o.d2, fd = -o.d1 / d;
-            out_x[i].x += fprime * rdu * r.x + fd * (du.x - rdu * r.x);
-            out_x[i].y += fprime * rdu * r.y + fd * (du.y - rdu * r.y);
-            out_x[i].z += fprime * rdu * r.z + fd * (du.z - rdu * r.z);
-            // per ordered pair, credited to species ti
-            const double Xs = rdu * o.f_s + vcoef * o.phi_s;
-            const double Xe = rdu * o.f_e + vcoef * o.phi_e;
-            const double Xq = rdu * o.f_q + vcoef * o.phi_q;
-            const double Xr = rdu * o.f_rc + vcoef * o.phi_rc;
-            gth[ti] += 0.5 * Xs;  // d sigma_ij / d sigma_ti = 1/2 (each ordered pair credits its first atom)
-            gth[10 + ti] += 0.5 * Xe * M.eps_ij[ti][tj] / M.epsK[ti];
-            gth[20 + ti] += Xq * M.k_e * (M.keep[i] ? 1.0 : 0.0) * M.q[j];
-            grc += 0.5 * Xr;
-            if (vcoef != 0.0) {
-                // loss term vcoef * V: dV/dx_i = -F_i
-                const double f = -o.d1;
-                out_x[i].x -= vcoef * f * r.x;
-                out_x[i].y -= vcoef * f * r.y;
-                out_x[i].z -= vcoef * f * r.z;
-            }
+    PairList pl;
+    pair_list(M, x, M.rc, false, pl);
+    for (size_t k = 0; k < 2 * pl.a.size(); ++k) {
+        // every ordered pair: (a, b) then (b, a)
+        const size_t m = k >> 1;
+        const bool rev = (k & 1) != 0;
+        const int i = rev ? pl.b[m] : pl.a[m], j = rev ? pl.a[m] : pl.b[m];
+        const P3 dv = rev ? P3{-pl.dv[m].x, -pl.dv[m].y, -pl.dv[m].z} : pl.dv[m];
+        const double d = pl.d[m];
+        PairD o;
+        M.pd(i, j, d, o);
+        const int ti = M.type[i], tj = M.type[j];
+        P3 r{dv.x / d, dv.y / d, dv.z / d};
+        const double imi = 1.0 / M.mass[ti], imj = 1.0 / M.mass[tj];
+        P3 du{lam[i].x * imi - lam[j].x * imj, lam[i].y * imi - lam[j].y * imj, lam[i].z * imi - lam[j].z * imj};
+        const double rdu = r.x * du.x + r.y * du.y + r.z * du.z;
+        // K du with K = f' rr^T + (f/d)(I - rr^T), f = -d1, f' = -d2
+        const double fprime = -o.d2, fd = -o.d1 / d;
+        out_x[i].x += fprime * rdu * r.x + fd * (du.x - rdu * r.x);
+        out_x[i].y += fprime * rdu * r.y + fd * (du.y - rdu * r.y);
+        out_x[i].z += fprime * rdu * r.z + fd * (du.z - rdu * r.z);
+        // per ordered pair, credited to species ti
+        const double Xs = rdu * o.f_s + vcoef * o.phi_s;
+        const double Xe = rdu * o.f_e + vcoef * o.phi_e;
+        const double Xq = rdu * o.f_q + vcoef * o.phi_q;
+        const double Xr = rdu * o.f_rc + vcoef * o.phi_rc;
+        gth[ti] += 0.5 * Xs;  // d sigma_ij / d sigma_ti = 1/2 (each ordered pair credits its first atom)
+        gth[10 + ti] += 0.5 * Xe * M.eps_ij[ti][tj] / M.epsK[ti];
+        gth[20 + ti] += Xq * M.k_e * (M.keep[i] ? 1.0 : 0.0) * M.q[j];
+        grc += 0.5 * Xr;
+        if (vcoef != 0.0) {
+            // loss term vcoef * V: dV/dx_i = -F_i
+            const double f = -o.d1;
+            out_x[i].x -= vcoef * f * r.x;
+            out_x[i].y -= vcoef * f * r.y;
+            out_x[i].z -= vcoef * f * r.z;
         }
+    }
     gth[M.argmax] += grc * M.cut_factor;
 }
 
@@ -241,35 +315,47 @@ struct SmoothGr {
         }
     }
     double K(double u) const { return std::exp(-u * u / (2.0 * sb * sb)) / (std::sqrt(2.0 * M_PI) * sb); }
+    // The kernel's support: bins whose centre is within SGR_WIN sigma_b of d
+    // (beyond it the Gaussian is below 1.3e-14 of its peak).
+    static constexpr double SGR_WIN = 8.0;
+    void window(double d, int& k0, int& k1) const {
+        const double w = SGR_WIN * sb;
+        k0 = std::max(0, (int)std::ceil((d - w) / dr - 0.5));
+        k1 = std::min(nb - 1, (int)std::floor((d + w) / dr - 0.5));
+    }
+    double lim() const { return rmax + SGR_WIN * sb; }
     void eval(const Model& M, const std::vector<P3>& x, std::vector<double>& g) const {
         g.assign(nb, 0.0);
-        const double lim = rmax + 5.0 * sb;
-        for (int i = 0; i < M.n; ++i)
-            for (int j = i + 1; j < M.n; ++j) {
-                P3 dv = M.mi(x[i], x[j]);
-                double d = std::sqrt(dv.x * dv.x + dv.y * dv.y + dv.z * dv.z);
-                if (d >= lim) continue;
-                for (int k = 0; k < nb; ++k) g[k] += c[k] * K(d - (k + 0.5) * dr);
-            }
+        PairList pl;
+        pair_list(M, x, lim(), true, pl);
+        for (size_t m = 0; m < pl.a.size(); ++m) {
+            const double d = pl.d[m];
+            int k0, k1;
+            window(d, k0, k1);
+            for (int k = k0; k <= k1; ++k) g[k] += c[k] * K(d - (k + 0.5) * dr);
+        }
     }
     // out_x += d/dx sum_k (g_k - g0_k)^2
     void grad(const Model& M, const std::vector<P3>& x, const std::vector<double>& g, const double* g0,
               std::vector<P3>& out_x) const {
-        const double lim = rmax + 5.0 * sb;
-        for (int i = 0; i < M.n; ++i)
-            for (int j = i + 1; j < M.n; ++j) {
-                P3 dv = M.mi(x[i], x[j]);
-                double d = std::sqrt(dv.x * dv.x + dv.y * dv.y + dv.z * dv.z);
-                if (d >= lim || d <= 0.0) continue;
-                double s = 0.0;
-                for (int k = 0; k < nb; ++k) {
-                    double u = d - (k + 0.5) * dr;
-                    s += 2.0 * (g[k] - g0[k]) * c[k] * (-u / (sb * sb)) * K(u);
-                }
-                P3 r{dv.x / d, dv.y / d, dv.z / d};
-                out_x[i].x += s * r.x; out_x[i].y += s * r.y; out_x[i].z += s * r.z;
-                out_x[j].x -= s * r.x; out_x[j].y -= s * r.y; out_x[j].z -= s * r.z;
+        PairList pl;
+        pair_list(M, x, lim(), true, pl);
+        for (size_t m = 0; m < pl.a.size(); ++m) {
+            const int i = pl.a[m], j = pl.b[m];
+            const P3 dv = pl.dv[m];
+            const double d = pl.d[m];
+            if (d <= 0.0) continue;
+            int k0, k1;
+            window(d, k0, k1);
+            double s = 0.0;
+            for (int k = k0; k <= k1; ++k) {
+                double u = d - (k + 0.5) * dr;
+                s += 2.0 * (g[k] - g0[k]) * c[k] * (-u / (sb * sb)) * K(u);
             }
+            P3 r{dv.x / d, dv.y / d, dv.z / d};
+            out_x[i].x += s * r.x; out_x[i].y += s * r.y; out_x[i].z += s * r.z;
+            out_x[j].x -= s * r.x; out_x[j].y -= s * r.y; out_x[j].z -= s * r.z;
+        }
     }
 };
 
