@@ -89,10 +89,10 @@ __global__ __launch_bounds__(BLOCK) void k_integrate_key(KParams P, double4* __r
                                                          const double* __restrict__ lam_ptr,
                                                          uint32_t* __restrict__ key, uint32_t* __restrict__ ticket,
                                                          uint32_t* __restrict__ count) {
-    int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= P.n) return;
-    double4 p = pos4[k];
-    if (KICK_DRIFT) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    const bool valid = k < P.n;
+    double4 p = valid ? pos4[k] : make_double4(0.0, 0.0, 0.0, 0.0);
+    if (KICK_DRIFT && valid) {
         double lam = *lam_ptr;
         double a = vx[k] * lam, b = vy[k] * lam, c = vz[k] * lam;
         a = a + ax[k] * P.half_dt;
@@ -111,9 +111,28 @@ __global__ __launch_bounds__(BLOCK) void k_integrate_key(KParams P, double4* __r
         }
         pos4[k] = p;
     }
-    uint32_t kk = cell_key(P, p.x, p.y, p.z);
-    key[k] = kk;
-    ticket[k] = atomicAdd(&count[kk], 1u);
+    const uint32_t kk = valid ? cell_key(P, p.x, p.y, p.z) : 0xFFFFFFFFu;
+    // The state is cell-sorted, so a wave's atoms fall into a few runs of
+    // equal cells: each run takes its tickets with one atomic (per-lane
+    // atomics on one address would serialise).  The order of the tickets
+    // within a cell is undone by k_sort_gather's per-cell ordering.
+    const int lane = threadIdx.x & 63;
+    const uint32_t prev = __shfl_up(kk, 1, WAVE);
+    const bool head = valid && (lane == 0 || prev != kk);
+    const uint64_t heads = __ballot(head);
+    const int nvalid = __popcll(__ballot(valid));
+    const uint64_t upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    const uint64_t hb = heads & upto;
+    const int rs = hb ? 63 - __builtin_clzll(hb) : 0;  // head of this lane's run
+    const uint64_t ha = heads & ~upto;
+    const int re = ha ? __builtin_ctzll(ha) : nvalid;  // one past this lane's run
+    uint32_t base = 0;
+    if (head) base = atomicAdd(&count[kk], (uint32_t)(re - rs));
+    base = __shfl(base, rs, WAVE);
+    if (valid) {
+        key[k] = kk;
+        ticket[k] = base + (uint32_t)(lane - rs);
+    }
 }
 
 // Exclusive scan of the per-cell histogram into cell_start[0..ncell], and
@@ -2910,7 +2929,6 @@ struct gs_universe {
     double* adj_qs = nullptr;
     double* adj_acc = nullptr;
     double* adj_gatom = nullptr;
-    double* adj_spart = nullptr;  // species-reduction partials, 32 per block
     unsigned long long* adj_hist = nullptr;  // smoothed g(r): fixed-point bin sums
     size_t adj_nbins_cap = 0;
     double* adj_ck = nullptr;
