@@ -123,6 +123,10 @@ _SIGNATURES = {
     "gs_velocity_histogram": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_int32, _U64,
                                              ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), _D]),
     "gs_bond_reset": (ctypes.c_int, [_H]),
+    "gs_snapshot_append": (ctypes.c_int, [_H, ctypes.c_char_p]),
+    "gs_snapshot_load": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_int64]),
+    "gs_set_tick": (ctypes.c_int, [_H, ctypes.c_int64]),
+    "gs_get_tick": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_int64)]),
     "gs_radial_distribution_noalloc": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_int32, ctypes.c_void_p, _D, _D]),
 }
 

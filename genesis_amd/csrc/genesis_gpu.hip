@@ -4390,3 +4390,4 @@ int gs_clusters(gs_universe* u, double bond_factor, uint32_t* labels, gs_cluster
 #include "gs_batch.inc"
 #include "gs_bonds.inc"
 #include "gs_noalloc.inc"
+#include "gs_snapshot.inc"

@@ -315,6 +315,20 @@ class GpuUniverse:
     def bond_reset(self) -> None:
         _lib.check(self.lib.gs_bond_reset(self.h))
 
+    def set_tick(self, tick: int) -> None:
+        """Time::tick of the universe; the next step is tick + 1."""
+        _lib.check(self.lib.gs_set_tick(self.h, int(tick)))
+
+    def tick(self) -> int:
+        t = ctypes.c_int64()
+        _lib.check(self.lib.gs_get_tick(self.h, ctypes.byref(t)))
+        return t.value
+
+    def save(self, path: str, config=None, state=None) -> None:
+        """save_universe: a bincode v4 Snapshot followed by the device image."""
+        from .snapshot import save_universe
+        save_universe(self, path, config, state)
+
 
 def formula(composition) -> str:
     """Stoichiometric formula (src/analysis/mod.rs:417-435): symbols by descending

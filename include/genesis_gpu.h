@@ -325,6 +325,33 @@ int gs_bond_binding(gs_universe* u, double* binding);
  * tick counter. */
 int gs_bond_reset(gs_universe* u);
 
+/* ---- snapshots (save_universe / load_universe, src/serialization/mod.rs:84-120)
+ * The file format is the reference's bincode v4 Snapshot (config, clock,
+ * RNG, stats and every entity's components); the host mirror writes and
+ * reads that part (genesis_amd/snapshot.py).  These two calls add what the
+ * device holds beyond the components, appended after the Snapshot (the
+ * reference's loader ignores trailing bytes): the resident order, the
+ * Verlet lists with their build positions, buckets and flags, the bond
+ * tracker's episodes and counters, and the tick.  A universe created with
+ * the same gs_params and atom count and loaded from the image steps bit for
+ * bit like the one it was taken from on the per-tick cell tiles and the
+ * Verlet lists (pair_mode 0 and 1); the cell-staged lists of pair_mode 2 are
+ * not in the image, so such a universe re-sorts and rebuilds them on its
+ * first tick and continues to within rounding.  Paths are plain file names.
+ * bincode 1 (Cargo.toml: bincode = "1") reads a Snapshot with
+ * `deserialize`, which allows trailing bytes. */
+/* Appends the device image of u to the file (created if absent). */
+int gs_snapshot_append(gs_universe* u, const char* path);
+/* Restores the device image found at byte `offset` of the file; GS_E_INVALID
+ * if it was taken from a universe with other parameters or atom count. */
+int gs_snapshot_load(gs_universe* u, const char* path, int64_t offset);
+/* Time::tick of the universe (restored by load_universe from a file without
+ * a device image, src/serialization/mod.rs:116 and src/universe/mod.rs:129):
+ * the next gs_step tick is tick + 1.  0 <= tick < 2^32 - 1. */
+int gs_set_tick(gs_universe* u, int64_t tick);
+/* Time::tick of the universe: the last tick gs_step ran (or the one set). */
+int gs_get_tick(gs_universe* u, int64_t* tick);
+
 /* ---- differentiable trajectory (no reference counterpart) -------------------
  * theta[30] = per-species sigma[10], epsilon_k[10], charge[10]; keep[n] (u8,
  * slot order) multiplies each atom's charge (0 = neutralised atom).  These
