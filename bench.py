@@ -144,9 +144,13 @@ def ionic_headline(steps: int, warmup: int, lanes: int, skin_frac: float, setup_
     gr0, comp0 = smp.n_gr, smp.n_comp
     first = smp.tick + 1
     u.timing(enable="pair", every=10)
+    # the window is bracketed by two gs_speed_stats calls (their k_speed_parts
+    # launches mark its ends in a kernel trace of this run)
+    k_first = u.speed_stats()[0]
     t0 = time.perf_counter()
     smp.run(steps)
     t1 = time.perf_counter()
+    k_last = u.speed_stats()[0]
     tm = u.timing()
     u.timing(enable=False)
     elapsed = t1 - t0
@@ -204,7 +208,9 @@ def ionic_headline(steps: int, warmup: int, lanes: int, skin_frac: float, setup_
                    "window": {"first_tick": first, "last_tick": smp.tick, "untimed_setup_ticks": setup_ticks,
                               "bonded_pairs_at_start": b0["bonded_pairs"], "bonded_pairs_at_end": b1["bonded_pairs"],
                               "bonded_entities_at_end": b1["bonded_entities"],
-                              "gr_samples_timed": smp.n_gr - gr0, "component_samples_timed": smp.n_comp - comp0},
+                              "gr_samples_timed": smp.n_gr - gr0, "component_samples_timed": smp.n_comp - comp0,
+                              "kinetic_at_ends": [k_first, k_last],
+                              "marked_by": "one gs_speed_stats call (k_speed_parts) at each end"},
                    "parallelism": "single GPU"},
         "roofline": roof,
         "roofline_hbm": roof_hbm,

@@ -93,6 +93,8 @@ _SIGNATURES = {
     "gs_radial_distribution": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_int32, _U64, _D, _D, _D]),
     "gs_radial_distribution_between": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                                       ctypes.c_int32, _U64, _D, _D, _D]),
+    "gs_radial_distribution_full": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                                   ctypes.c_int32, _U64, _D, _D, _D]),
     "gs_get_tuning": (ctypes.c_int, [_H, ctypes.POINTER(Tuning)]),
     "gs_set_tuning": (ctypes.c_int, [_H, ctypes.POINTER(Tuning)]),
     "gs_set_bound_search": (ctypes.c_int, [_H, ctypes.c_int32]),

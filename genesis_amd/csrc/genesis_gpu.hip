@@ -1115,8 +1115,24 @@ struct ThermoArgs {
     double target, tau, kconst;
 };
 
+// The universe's scalar block: [0] lambda, [1] V, [2] K, [3] V_bond, [4] the
+// fault word (u64 bits: 1 a non-finite kinetic energy, 2 potential, 4 bond
+// energy, set by close_tick and cleared at the start of each gs_step call).
+constexpr int SCAL_WORDS = 5;
+constexpr int SCAL_FAULT = 4;
+
+__device__ __forceinline__ unsigned long long* fault_word(double* scal) {
+    return reinterpret_cast<unsigned long long*>(scal + SCAL_FAULT);
+}
+
+// fault: the fault word to mark when a total is non-finite (NULL: none).  A
+// NaN or infinite velocity, acceleration or position reaches K within the
+// tick it appears, so the per-tick totals are where it is caught; one thread
+// of the tick's last workgroup writes it and ticks are ordered on the stream.
 __device__ void close_tick(double K, double V, double Vb, int n, const ThermoArgs& th, double* lam_out,
-                           gs_observables* obs) {
+                           gs_observables* obs, unsigned long long* fault) {
+    if (fault && !(isfinite(K) && isfinite(V) && isfinite(Vb)))
+        *fault |= (isfinite(K) ? 0ull : 1ull) | (isfinite(V) ? 0ull : 2ull) | (isfinite(Vb) ? 0ull : 4ull);
     double lam = 1.0;
     const double T = (2.0 / 3.0) * (K / (double)n) / th.kconst;
     if (th.on && th.target > 0.0 && n > 0) {
@@ -1183,7 +1199,7 @@ __global__ __launch_bounds__(BLOCK) void k_kick_ke(KParams P, double* __restrict
     const double V = vpart ? block_sum_fixed(vpart, nvpart, red) : 0.0;
     const double Vb = vbpart ? block_sum_fixed(vbpart, nvb, red) : 0.0;
     if (tid == 0) {
-        close_tick(K, V, Vb, P.n, ThermoArgs{thermostat, target, tau, kconst}, lam_out, obs);
+        close_tick(K, V, Vb, P.n, ThermoArgs{thermostat, target, tau, kconst}, lam_out, obs, fault_word(lam_out));
         *counter = 0u;
     }
 }
@@ -2165,7 +2181,7 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
     block_sum2_fixed<BLOCK, true>(kpart, vpart, gridDim.x, red, K, V);
     const double Vb = vbpart ? block_sum_fixed(vbpart, nvb, red) : 0.0;
     if (tid == 0) {
-        close_tick(K, V, Vb, (int)N, th, lam_out, obs);
+        close_tick(K, V, Vb, (int)N, th, lam_out, obs, fault_word(lam_out));
         close_list_tick(flags);
     }
 }
@@ -2516,7 +2532,7 @@ __global__ __launch_bounds__(CELL_THREADS) __attribute__((amdgpu_waves_per_eu(5)
     block_sum2_fixed<CELL_THREADS, true>(kpart, vpart, gridDim.x, red, K, V);
     const double Vb = vbpart ? block_sum_fixed<CELL_THREADS>(vbpart, nvb, red) : 0.0;
     if (tid == 0) {
-        close_tick(K, V, Vb, (int)N, th, lam_out, obs);
+        close_tick(K, V, Vb, (int)N, th, lam_out, obs, fault_word(lam_out));
         close_list_tick(flags);
     }
 }
@@ -2856,7 +2872,7 @@ struct gs_universe {
     uint32_t npart = 0;  // slots of vpart and kpart: every pair pass's workgroups
     double* kpart = nullptr;
     uint32_t* counter = nullptr;
-    double* scal = nullptr;  // [0] lambda, [1] V, [2] K, [3] spare
+    double* scal = nullptr;  // SCAL_WORDS: [0] lambda, [1] V, [2] K, [3] V_bond, [4] fault word
     double *pe = nullptr, *sabs = nullptr;
     double* stage = nullptr;  // 9n doubles
     uint32_t* stage_u = nullptr;
@@ -3569,7 +3585,7 @@ int gs_create(const gs_params* params, int64_t n_atoms, gs_universe** out) {
         (rc = dalloc(&u->count, (size_t)P.ncell)) || (rc = dalloc(&u->cstart, (size_t)P.ncell + 1)) ||
         (rc = dalloc(&u->coef, coef.size())) || (rc = dalloc(&u->vpart, (size_t)u->npart)) ||
         (rc = dalloc(&u->kpart, (size_t)u->npart)) || (rc = dalloc(&u->counter, 4)) ||
-        (rc = dalloc(&u->scal, 4)) || (rc = dalloc(&u->pe, n)) || (rc = dalloc(&u->sabs, n)) ||
+        (rc = dalloc(&u->scal, (size_t)SCAL_WORDS)) || (rc = dalloc(&u->pe, n)) || (rc = dalloc(&u->sabs, n)) ||
         (rc = dalloc(&u->stage, 9 * n)) || (rc = dalloc(&u->stage_u, n)) ||
         (rc = scan_reserve(&u->scan_tmp, &u->scan_tmp_cap, P.ncell)))
         return bail(rc);
@@ -3580,6 +3596,7 @@ int gs_create(const gs_params* params, int64_t n_atoms, gs_universe** out) {
             hipSuccess)
         return bail(fail(GS_E_HIP, "device initialisation failed"));
     k_set_scalar<<<1, 1, 0, u->stream>>>(u->scal, 1.0);
+    k_set_scalar<<<1, 1, 0, u->stream>>>(u->scal + SCAL_FAULT, 0.0);
     if (hipStreamSynchronize(u->stream) != hipSuccess) return bail(fail(GS_E_HIP, "device initialisation failed"));
     *out = u;
     return GS_OK;
@@ -3881,11 +3898,20 @@ int gs_speed_stats(gs_universe* u, double* kinetic, double* speed_sum) {
     return GS_OK;
 }
 
-int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
-    int rc = check_state(u);
-    if (rc) return rc;
-    if (n_steps < 0) return fail(GS_E_INVALID, "n_steps < 0");
-    if (n_steps == 0 || u->n == 0) return GS_OK;
+}  // extern "C"
+
+namespace {
+
+// Ticks per segment of a gs_step call.  The call takes its replay snapshot
+// and reads its flags once per segment, so a capacity overflow replays at
+// most one segment, and a long call holds a bounded observables buffer.
+constexpr int64_t STEP_SEGMENT = 1024;
+
+// One segment of gs_step (n_steps <= STEP_SEGMENT): the ticks, then the
+// flags; a replay returns the state, the bonds and the tick to the
+// segment's start.
+int step_segment(gs_universe* u, int64_t n_steps, gs_observables* obs) {
+    int rc;
     if (obs && u->obs_cap < n_steps) {
         (void)hipFree(u->obs);
         u->obs = nullptr;
@@ -3912,6 +3938,7 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
     u->bound_unfused = false;
     for (int attempt = 0;; ++attempt) {
         if (list) HIPCHK(hipMemsetAsync(u->flags, 0, LIST_FLAGS_RESET * sizeof(uint32_t), u->stream));
+        k_set_scalar<<<1, 1, 0, u->stream>>>(u->scal + SCAL_FAULT, 0.0);  // all-zero bits: no fault
         if ((rc = bond_fused_prepare(u, list))) return rc;
         if ((rc = run_ticks(u, n_steps, obs != nullptr, list))) return rc;
         const int c = u->cur;
@@ -3922,8 +3949,9 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
         if (list) HIPCHK(hipMemcpyAsync(fl, u->flags, sizeof(fl), hipMemcpyDeviceToHost, u->stream));
         // the bond state's sticky overflow word: an entity with more than 32
         // bound partners, tracked episodes or bonds
-        unsigned long long bond_ovf = 0;
+        unsigned long long bond_ovf = 0, fault = 0;
         if ((rc = bond_overflow_fetch(u, &bond_ovf))) return rc;
+        HIPCHK(hipMemcpyAsync(&fault, u->scal + SCAL_FAULT, sizeof(fault), hipMemcpyDeviceToHost, u->stream));
         HIPCHK(hipStreamSynchronize(u->stream));
         if (list && (fl[1] || fl[2] || fl[4] || fl[6] || fl[7])) {
             // A list or a cell bucket overflowed its capacity, a stencil
@@ -3969,6 +3997,15 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
             }
             return fail(GS_E_OVERFLOW, "more than 32 bound partners, tracked episodes or bonds for one entity");
         }
+        if (fault) {
+            // the ticks ran to the call's end; the state keeps what they made
+            std::string what;
+            if (fault & 1ull) what += " kinetic energy";
+            if (fault & 2ull) what += std::string(what.empty() ? "" : ",") + " potential energy";
+            if (fault & 4ull) what += std::string(what.empty() ? "" : ",") + " bond energy";
+            return fail(GS_E_NONFINITE, "non-finite" + what + " in ticks " + std::to_string(tick0 + 1) + ".." +
+                                            std::to_string(u->tick) + " (a NaN or infinite velocity, force or position)");
+        }
         break;
     }
     if (obs)
@@ -3991,6 +4028,22 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
             u->tm.step_ms += (double)a + b + c2;
         }
         u->tm.steps += timed;
+    }
+    return GS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs) {
+    int rc = check_state(u);
+    if (rc) return rc;
+    if (n_steps < 0) return fail(GS_E_INVALID, "n_steps < 0");
+    if (n_steps == 0 || u->n == 0) return GS_OK;
+    for (int64_t done = 0; done < n_steps; done += STEP_SEGMENT) {
+        const int64_t k = std::min(STEP_SEGMENT, n_steps - done);
+        if ((rc = step_segment(u, k, obs ? obs + done : nullptr))) return rc;
     }
     return GS_OK;
 }
@@ -4204,8 +4257,9 @@ int lens_sort(gs_universe* u, const AuxGrid& G, const double4* pts, int m) {
     return GS_OK;
 }
 
+// full: every atom (no subsample).
 int radial_impl(gs_universe* u, bool between, int ta, int tb, double r_max_in, int nbins_in, uint64_t* counts,
-                double* g, double* r_max_out, double* dr_out) {
+                double* g, double* r_max_out, double* dr_out, bool full = false) {
     int rc = check_state(u);
     if (rc) return rc;
     if (between && (ta < 0 || ta >= GS_NSPECIES || tb < 0 || tb >= GS_NSPECIES))
@@ -4220,7 +4274,7 @@ int radial_impl(gs_universe* u, bool between, int ta, int tb, double r_max_in, i
     // g_subsample (src/analysis/mod.rs:32-46).
     const uint32_t N = (uint32_t)u->n;
     uint32_t step = 1, m = N;
-    if (N > 8192u) {
+    if (N > 8192u && !full) {
         step = N / 8192u + 1u;
         m = N / step;
     }
@@ -4338,6 +4392,12 @@ int gs_radial_distribution(gs_universe* u, double r_max, int32_t nbins, uint64_t
 int gs_radial_distribution_between(gs_universe* u, int32_t species_a, int32_t species_b, double r_max,
                                    int32_t nbins, uint64_t* counts, double* g, double* r_max_out, double* dr_out) {
     return radial_impl(u, true, species_a, species_b, r_max, nbins, counts, g, r_max_out, dr_out);
+}
+
+int gs_radial_distribution_full(gs_universe* u, int32_t species_a, int32_t species_b, double r_max, int32_t nbins,
+                                uint64_t* counts, double* g, double* r_max_out, double* dr_out) {
+    if (species_a < 0) return radial_impl(u, false, -1, -1, r_max, nbins, counts, g, r_max_out, dr_out, true);
+    return radial_impl(u, true, species_a, species_b, r_max, nbins, counts, g, r_max_out, dr_out, true);
 }
 
 int gs_clusters(gs_universe* u, double bond_factor, uint32_t* labels, gs_cluster_stats* stats) {

@@ -89,6 +89,17 @@ class ThermostatSystem(System):
         ctx.resources["lambda"] = ctx.universe.thermostat()
 
 
+class BondObservationSystem(System):
+    """src/systems/bonds.rs:73-185: bound pairs, the pair tracker and the
+    Bonds adjacency of this tick (read by the next tick's ForceSystem)."""
+
+    def name(self) -> str:
+        return "bond_observation"
+
+    def run(self, ctx: SystemContext) -> None:
+        ctx.universe.bond_observe(ctx.tick)
+
+
 class Scheduler:
     """Sequential scheduler in registration order (src/scheduler/mod.rs:34-89)."""
 
@@ -98,13 +109,18 @@ class Scheduler:
     def add_system(self, s: System) -> None:
         self.systems.append(s)
 
+    def names(self) -> List[str]:
+        return [s.name() for s in self.systems]
+
     def run(self, ctx: SystemContext) -> None:
         for s in self.systems:
             s.run(ctx)
 
 
 def build_schedule(physics: Physics) -> Scheduler:
-    """build_schedule with forces on (src/universe/mod.rs:355-369)."""
+    """build_schedule with forces on (src/universe/mod.rs:355-388), up to the
+    hot path's last system: the observation lenses after it (structure,
+    bond structure, stats) read the state and are separate calls."""
     s = Scheduler()
     s.add_system(VelocityHalfKick())
     s.add_system(PositionDrift())
@@ -114,4 +130,13 @@ def build_schedule(physics: Physics) -> Scheduler:
     s.add_system(VelocityHalfKick())
     if physics.enable_thermostat:
         s.add_system(ThermostatSystem())
+    if physics.enable_bond_observation:
+        s.add_system(BondObservationSystem())
     return s
+
+
+def tick(scheduler: Scheduler, ctx: SystemContext) -> None:
+    """Universe::tick (src/universe/mod.rs:140-159): the clock advances, then
+    the schedule runs once."""
+    ctx.tick += 1
+    scheduler.run(ctx)

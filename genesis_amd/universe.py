@@ -235,6 +235,16 @@ class GpuUniverse:
             ctypes.byref(rm), ctypes.byref(dr)))
         return counts, g, rm.value, dr.value
 
+    def radial_distribution_full(self, r_max: float, nbins: int, ta: int = -1, tb: int = -1):
+        """g(r) (ta < 0) or g_ab(r) over every atom, without the subsample."""
+        counts = np.zeros(max(1, min(nbins, 1 << 16)), dtype=np.uint64)
+        g = np.zeros(counts.shape[0])
+        rm, dr = ctypes.c_double(), ctypes.c_double()
+        _lib.check(self.lib.gs_radial_distribution_full(
+            self.h, ta, tb, r_max, nbins, counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), _dp(g),
+            ctypes.byref(rm), ctypes.byref(dr)))
+        return counts, g, rm.value, dr.value
+
     def clusters(self, bond_factor: float = 1.5):
         labels = np.zeros(self.n, dtype=np.uint32)
         st = _lib.ClusterStats()

@@ -191,11 +191,19 @@ int gs_speed_stats(gs_universe* u, double* kinetic, double* speed_sum);
  * build_schedule (src/universe/mod.rs:355-392): kick, drift, [wrap], forces,
  * kick, [thermostat].  Fused and resident: nothing crosses to the host but the
  * observables.  obs may be NULL, else it receives n_steps records.
+ * The call runs in segments of 1024 ticks, each with its own replay
+ * snapshot: a list or bucket that overflows replays only its segment.
  * With bond observation, an entity with more than 32 bound partners, tracked
  * episodes or bonds (the reference has no cap) makes the call return
  * GS_E_OVERFLOW; on the Verlet-list path the universe is then back at the
- * call's start, on the cell tiles it holds the call's end.  The overflow is
- * sticky until gs_bond_reset. */
+ * start of the segment that overflowed (gs_get_tick tells which), on the
+ * cell tiles it holds that segment's end.  The overflow is sticky until
+ * gs_bond_reset.
+ * Device fault word: the last workgroup of every tick marks a non-finite
+ * kinetic, potential or bond energy (what a NaN or infinite velocity, force
+ * or position turns into within its tick); a call that saw one returns
+ * GS_E_NONFINITE naming the quantities and the segment's ticks, with the
+ * state left at that segment's end.  The reference has no such check. */
 int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs);
 
 int gs_get_tuning(gs_universe* u, gs_tuning* out);
@@ -245,6 +253,13 @@ int gs_velocity_histogram(gs_universe* u, double max_speed, int32_t nbins, uint6
 int gs_radial_distribution(gs_universe* u, double r_max, int32_t nbins,
                            uint64_t* counts, double* g,
                            double* r_max_out, double* dr_out);
+/* g(r) over every atom, without the subsample (no counterpart in the
+ * reference, whose radial_distribution subsamples above 8192 atoms,
+ * src/analysis/mod.rs:28-46): the same bins and normalisation with all n
+ * atoms.  species_a < 0 gives the total g(r), otherwise g_ab(r) as
+ * gs_radial_distribution_between. */
+int gs_radial_distribution_full(gs_universe* u, int32_t species_a, int32_t species_b, double r_max,
+                                int32_t nbins, uint64_t* counts, double* g, double* r_max_out, double* dr_out);
 /* No counterpart in the reference: an allocation-free variant of
  * gs_radial_distribution for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid

@@ -539,14 +539,20 @@ std::vector<size_t> g_subsample_index(size_t n) {
 
 // Pair counting shared by radial_distribution(_between),
 // src/analysis/mod.rs:93-117 and :152-182.
+// full: every point, no subsample (the GPU's full mode; the reference always
+// subsamples above G_SAMPLE_CAP points).
 void gr_counts(const std::vector<V3>& pos, const std::vector<int>& type, V3 size, double r_max_in,
                int nbins_in, int ta, int tb, std::vector<uint64_t>& counts, double* r_max_out, double* dr_out,
-               double* pair_norm) {
+               double* pair_norm, bool full = false) {
     int nbins = std::min(std::max(nbins_in, 1), 1 << 16);
     double half_min = std::min(std::min(size.x, size.y), size.z) * 0.5;
     double r_max = std::min(fmax_r(r_max_in, 1e-9), half_min);
     double dr = r_max / (double)nbins;
-    std::vector<size_t> idx = g_subsample_index(pos.size());
+    std::vector<size_t> idx = g_subsample_index(full ? 0 : pos.size());
+    if (full) {
+        idx.resize(pos.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+    }
     std::vector<V3> sp(idx.size());
     std::vector<int> st(idx.size());
     for (size_t i = 0; i < idx.size(); ++i) {
@@ -1078,7 +1084,7 @@ int64_t or_track(int64_t nticks, const int64_t* counts, const uint32_t* pa, cons
 // for the total g(r); ta < 0 selects the total.
 int or_radial_distribution(int64_t n, const double* x, const double* y, const double* z, const uint8_t* species,
                            const double* size, double r_max, int nbins, int ta, int tb, uint64_t* counts, double* g,
-                           double* r_max_out, double* dr_out) {
+                           double* r_max_out, double* dr_out, int full) {
     std::vector<V3> pos = load_pos(n, x, y, z);
     std::vector<int> type;
     if (species) {
@@ -1088,7 +1094,7 @@ int or_radial_distribution(int64_t n, const double* x, const double* y, const do
     std::vector<uint64_t> c;
     double rm, dr, norm;
     V3 s = v3(size[0], size[1], size[2]);
-    gr_counts(pos, type, s, r_max, nbins, ta, tb, c, &rm, &dr, &norm);
+    gr_counts(pos, type, s, r_max, nbins, ta, tb, c, &rm, &dr, &norm, full != 0);
     if (counts) std::memcpy(counts, c.data(), c.size() * sizeof(uint64_t));
     if (g) gr_normalise(c, dr, s, norm, g);
     if (r_max_out) *r_max_out = rm;

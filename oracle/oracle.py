@@ -62,7 +62,8 @@ def lib():
             "or_thermostat": (ctypes.c_double, [P, ctypes.c_int64, _D, _D, _D, _U8]),
             "or_run": (ctypes.c_int, [P, ctypes.c_int64] + [_D] * 9 + [_U8, _D, ctypes.c_int64, ctypes.c_int, _D]),
             "or_radial_distribution": (ctypes.c_int, [ctypes.c_int64, _D, _D, _D, _U8, _D, ctypes.c_double,
-                                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, _U64, _D, _D, _D]),
+                                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, _U64, _D, _D, _D,
+                                                      ctypes.c_int]),
             "or_clusters": (ctypes.c_int, [ctypes.c_int64, _D, _D, _D, _U8, _D, _D, ctypes.c_double, _U32, _D]),
             "or_pair_derivs": (None, [ctypes.c_double] * 5 + [ctypes.c_int, _D]),
             "or_adj_accel": (ctypes.c_double, [P, ctypes.c_int64, _D, _D, _D, _U8, _U8, _D, _D, _D, _D]),
@@ -389,7 +390,8 @@ def thermostat(physics, state):
                                _p(sp, ctypes.c_uint8))
 
 
-def radial_distribution(x, y, z, size, r_max, nbins, species=None, ta=-1, tb=-1):
+def radial_distribution(x, y, z, size, r_max, nbins, species=None, ta=-1, tb=-1, full=False):
+    """full: every atom (no subsample above 8192 atoms)."""
     n = len(x)
     x, y, z, s = _d(x), _d(y), _d(z), _d(size)
     sp = None if species is None else np.ascontiguousarray(species, dtype=np.uint8)
@@ -397,7 +399,8 @@ def radial_distribution(x, y, z, size, r_max, nbins, species=None, ta=-1, tb=-1)
     counts, g = np.zeros(nb, dtype=np.uint64), np.zeros(nb)
     rm, dr = ctypes.c_double(), ctypes.c_double()
     lib().or_radial_distribution(n, _p(x), _p(y), _p(z), _p(sp, ctypes.c_uint8), _p(s), r_max, nbins, ta, tb,
-                                 _p(counts, ctypes.c_uint64), _p(g), ctypes.byref(rm), ctypes.byref(dr))
+                                 _p(counts, ctypes.c_uint64), _p(g), ctypes.byref(rm), ctypes.byref(dr),
+                                 1 if full else 0)
     return counts, g, rm.value, dr.value
 
 

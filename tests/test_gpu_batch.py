@@ -87,8 +87,9 @@ def test_temperature_sweep_workload_matches_the_oracle():
     # run of that replica alone.
     from genesis_amd.batch import GpuBatch
     from oracle import oracle as O
-    # Replica 1 (651 atoms) is on a 2-cell grid (brute force with the
-    # reference's multiplicity), 2 and 3 (1001 and 2459 atoms) on lists.
+    # Replica 1 (651 atoms) is on a 2-cell grid (brute force; the sweep runs
+    # every replica with min-image semantics, each pair counted once), 2 and
+    # 3 (1001 and 2459 atoms) on lists.
     ws = replica_sweep(n_rep=64, steps=10)
     with GpuBatch([w.physics for w in ws], [w.state for w in ws]) as b:
         assert b.get_tuning()["pair_mode"] == 1

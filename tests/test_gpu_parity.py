@@ -262,6 +262,25 @@ def test_partial_radial_distribution_bit_exact(GU):
             assert np.array_equal(g, g_ref)
 
 
+def test_full_radial_distribution_bit_exact(GU):
+    # every atom of the 32,768-atom mixture (the reference's lens would take
+    # 8,192 of them), total and partial, against the oracle's full mode
+    w = mixture(n=32768, seed=32769, steps=1)
+    ph, st = w.physics, w.state
+    r_max = 0.25 * ph.box[0]
+    say(f"full g(r) n={st.n} r_max={r_max:.3f}")
+    with GU(ph, st) as u:
+        for ta, tb in ((-1, -1), (4, 5), (0, 0)):
+            c_ref, g_ref, rm_ref, dr_ref = O.radial_distribution(st.x, st.y, st.z, np.asarray(ph.box), r_max, 200,
+                                                                 species=st.species, ta=ta, tb=tb, full=True)
+            c, g, rm, dr = u.radial_distribution_full(r_max, 200, ta, tb)
+            assert (rm, dr) == (rm_ref, dr_ref)
+            assert np.array_equal(c, c_ref), (ta, tb)
+            assert np.array_equal(g, g_ref)
+            if ta < 0:
+                assert c.sum() > 16 * O.radial_distribution(st.x, st.y, st.z, np.asarray(ph.box), r_max, 200)[0].sum()
+
+
 @pytest.mark.parametrize("n,density", [(32768, 1.0), (5000, 0.3)])
 def test_cluster_labels_bit_exact(GU, n, density):
     w = mixture(n=n, seed=n + 3, steps=1)
@@ -580,3 +599,31 @@ def test_coulomb_switch_straddling_the_clamp(GU, mode, lanes):
     r, _ = _close_pairs(ph, st, 0.54)
     assert ((r > 0.486) & (r < 0.5)).sum() >= 20 and (r < 0.1).sum() >= 2
     _one_tick_forces_vs_oracle(GU, ph, st, mode, lanes, "switch-across-clamp", skin=0.12 * ph.force_rc())
+
+
+@pytest.mark.parametrize("mode,lanes", [(0, 1), (1, 4)])
+def test_non_finite_energy_raises_from_the_device_fault_word(GU, mode, lanes):
+    # one atom at a finite speed of 1e200: 0.5 m v^2 overflows to +inf on the
+    # first tick (positions stay finite, the wrap folds them back into the
+    # box), so the per-tick kinetic energy is non-finite on every tick
+    from genesis_amd import _lib
+    L = 30.0
+    ph = Physics(box=(L, L, L))
+    st = random_gas(71, 3000, ph.box, species_pool=(0, 4))
+    bad = st.copy()
+    bad.vx[17] = 1e200
+    print(f"[gpu-parity] fault word mode={mode} lanes={lanes}", flush=True)
+    with GU(ph, bad) as u:
+        if mode:
+            u.set_tuning(mode, lanes)
+        with pytest.raises(_lib.GsError) as e:
+            u.step(5)
+        assert e.value.code == _lib.GS_E_NONFINITE
+        assert "kinetic energy" in str(e.value) and "potential" not in str(e.value)
+        assert "ticks 1..5" in str(e.value)
+        got = u.download()
+        assert np.isfinite(got.x).all() and got.vx[17] == 1e200
+        # the word is cleared at every call: a clean state steps without error
+        u.upload(st)
+        obs = u.step(3)
+        assert np.isfinite(obs).all()

@@ -74,6 +74,22 @@ def test_g_r_subsample_above_cap():
     assert np.array_equal(c, brute_hist(x[idx], y[idx], z[idx], size, 4.0, 40))
 
 
+def test_g_r_full_mode_counts_every_atom():
+    # the same 10000 atoms as above without the subsample: brute force over
+    # all of them; below the cap the two modes coincide
+    n, L = 10000, 60.0
+    size = np.array([L] * 3)
+    st = Stream(3, 0)
+    x, y, z = [(st.uniform(n) - 0.5) * L for _ in range(3)]
+    c, g, _, dr = O.radial_distribution(x, y, z, size, 4.0, 40, full=True)
+    assert np.array_equal(c, brute_hist(x, y, z, size, 4.0, 40))
+    assert 0.8 <= g[10:].mean() <= 1.2
+    k = 3000
+    a, _, _, _ = O.radial_distribution(x[:k], y[:k], z[:k], size, 4.0, 40, full=True)
+    b, _, _, _ = O.radial_distribution(x[:k], y[:k], z[:k], size, 4.0, 40)
+    assert np.array_equal(a, b)
+
+
 def test_partial_g_r_isolates_the_requested_species():
     size = np.array([32.0] * 3)
     st = Stream(11, 0)
