@@ -1853,6 +1853,10 @@ __device__ __forceinline__ void close_list_tick(uint32_t* flags) {
 // evaluates anyway.
 constexpr int BOUND_MAXT = 32;  // bound partners kept per owner (the tracker's episodes)
 constexpr int HIT_LANE = 16;    // screened candidates a lane's LDS column holds
+// Bound-now lists by resident slot i, slot-major: word q of slot i at q n + i.
+__host__ __device__ __forceinline__ size_t now_idx(uint32_t q, uint32_t i, size_t n) {
+    return (size_t)q * n + i;
+}
 struct BoundArgs {
     uint32_t* hits = nullptr;   // [h * (N << LOGT) + lane]: the lane's h-th screened list entry
     uint32_t* hinfo = nullptr;  // [lane]: screened entries (clamped at 255) | band mask << 16
@@ -2248,20 +2252,24 @@ __global__ __launch_bounds__(BLOCK) void k_bound_from_hits(KParams P, uint32_t N
                 }
                 if (!acc) continue;
                 if (m < (uint32_t)BOUND_MAXT) {
-                    uint32_t w = m;  // insertion in ascending order
-                    while (w > 0 && rows[w - 1][threadIdx.x] > em[k]) {
+                    // insertion in ascending id order of partner words
+                    // (species << 28 | entity id; the entry carries the species)
+                    uint32_t w = m;
+                    while (w > 0 && (rows[w - 1][threadIdx.x] & 0x0FFFFFFFu) > em[k]) {
                         rows[w][threadIdx.x] = rows[w - 1][threadIdx.x];
                         --w;
                     }
-                    rows[w][threadIdx.x] = em[k];
+                    rows[w][threadIdx.x] = (f[k] & 0xF0000000u) | em[k];
                 }
                 ++m;
             }
         }
     }
+    // by resident slot, so the writes of a wave are coalesced; the tracker
+    // reads them through slot_of
     const uint32_t mm = min(m, (uint32_t)BOUND_MAXT);
-    for (uint32_t r = 0; r < mm; ++r) nowp[(size_t)r * N + e] = rows[r][threadIdx.x];
-    nown[e] = mm;
+    for (uint32_t r = 0; r < mm; ++r) nowp[now_idx(r, i, N)] = rows[r][threadIdx.x];
+    nown[i] = mm;
     if (m > (uint32_t)BOUND_MAXT) atomicOr(cnt + 4, 1ull);
 }
 
@@ -2903,6 +2911,7 @@ struct gs_universe {
     int logT = 2;               // lanes per atom = 1 << logT
     double skin = 0.0;
     int64_t sort_every = 100;   // ticks between full re-sorts of the state in list mode
+    uint64_t order_gen = 0;     // changes whenever the resident order (id[cur]) may have changed
     int64_t list_age = 0;       // ticks since the last full re-sort
     uint32_t nbr_cap = 0;       // list capacity per atom (entries)
     uint32_t nbr_have = 0;      // capacity the per-atom list array holds
@@ -3141,6 +3150,7 @@ int rebuild(gs_universe* u, bool kick_drift, const KParams& P) {
         P.ncell, u->cstart, u->perm, u->pos4[c], u->v[c][0], u->v[c][1], u->v[c][2], u->type[c], u->id[c],
         u->pos4[o], u->v[o][0], u->v[o][1], u->v[o][2], u->type[o], u->id[o]);
     u->cur = o;
+    ++u->order_gen;
     HIPCHK(hipGetLastError());
     return GS_OK;
 }
@@ -3315,6 +3325,7 @@ int snapshot(gs_universe* u, bool restore) {
     }
     HIPCHK(cp(u->type[c], u->snap_type, n));
     HIPCHK(cp(u->id[c], u->snap_id, n * sizeof(uint32_t)));
+    if (restore) ++u->order_gen;
     return GS_OK;
 }
 
@@ -3709,6 +3720,7 @@ int gs_upload_state(gs_universe* u, const double* x, const double* y, const doub
     u->species_h.assign(species, species + n);
     u->charge_h.assign(charge, charge + n);
     std::memcpy(u->present, present, sizeof(present));
+    ++u->order_gen;
     return bond_refresh_species(u, species);
 }
 
