@@ -26,6 +26,7 @@
 #include <mutex>
 #include <set>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace gs;
@@ -455,43 +456,55 @@ __device__ __forceinline__ void pair_law_sp(const KParams& P, double sig_i, doub
     }
 }
 
-// Four pairs of one atom at once, statement by statement (each statement
-// over the four pairs before the next), with the expressions of the list
-// traversal and pair_law_sp: a dependent FP64 operation waits for its
-// operand's result for several issue slots, so the four independent chains
-// are interleaved to fill them.  p: the four neighbour positions; tt: their
-// species + 1 in successive bytes, 0 for a slot past the end of the list.
 // Bound-pair screen of the list pass: k_bind^2 narrowed and widened by
 // 1e-10 relative.
 struct BoundScreen {
     double kb2_lo = 0.0, kb2_hi = 0.0;  // k_bind^2 (1 -+ 1e-10)
 };
 
-// Q0, NQ: the pairs q0 .. q0 + nq - 1 of the group (a group may be evaluated
-// as two halves, which holds fewer values live at once; the sums take the
+// Per-species-pair factors of the list pass, formed once per workgroup in
+// LDS with the expressions pair_law_sp evaluates per pair: sigma_ij =
+// (sigma_i + sigma_j) / 2, 4 eps_ij = a4_i a4_j, its sixfold (the factor of
+// the LJ force), the table's 1 / rcore, and the bound-pair screen's
+// thresholds sigma_ij^2 k_bind^2 (1 -+ 1e-10).  48-byte rows: a wave's
+// lanes reading different rows spread over the LDS banks.
+struct ListCoef {
+    double sig, eps4;
+    double e6, irc;
+    double thlo, thhi;
+};
+
+// Four pairs of one atom at once, statement by statement (each statement
+// over the four pairs before the next), with the expressions of the list
+// traversal and pair_law_sp: a dependent FP64 operation waits for its
+// operand's result for several issue slots, so the four independent chains
+// are interleaved to fill them.  p: the four neighbour positions; sp: their
+// species in successive bytes.  A slot past the end of the list holds the
+// atom's own position (r^2 = 0), which every test below rejects, so no
+// liveness mask is carried.
+// Q0, NQ: the pairs q0 .. q0 + nq - 1 of the group (a group is evaluated as
+// two halves, which holds fewer values live at once; the sums take the
 // pairs in the same order either way).
-// BOUND: bits 0-3 of hitm mark the pairs that are bound pairs for certain
+// INNER: every pair of the group is known to lie inside every switching
+// radius (d < r_on) on this tick, where the quintic switch is exactly 1 with
+// derivative 0: its evaluation is skipped (fma(f, 1, -v * -0) = f, v * 1 = v,
+// so the sums are bit-identical to the switched law's).
+// SCREEN: bits 0-3 of hitm mark the pairs that are bound pairs for certain
 // (r^2 at least 1e-10 relative inside (k_bind sigma_ij)^2, so no rounding of
 // the reference's own expressions can move them out), bits 4-7 the pairs
 // within that margin of the threshold, which the exact test decides.
-template <bool COUL, bool SHARED, bool GRAV, bool BOUND = false, int Q0 = 0, int NQ = 4>
+template <bool COUL, bool SHARED, bool GRAV, bool INNER, bool SCREEN, int Q0, int NQ>
 __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double Ly, double Lz, double iLx, double iLy,
-                                           double iLz, const double4& pi, double kqi, double sig_i, double a4_i,
-                                           const SpCoef* ssp, const double* irc_row, const PairCoef* row,
-                                           const double4 (&p)[4], uint32_t tt, double& fx, double& fy, double& fz,
-                                           double& vl, const BoundScreen& bs = BoundScreen{},
-                                           uint32_t* hitm = nullptr) {
-    bool live[4], ok[4], core[4];
+                                           double iLz, const double4& pi, double kqi, const ListCoef* lrow,
+                                           const PairCoef* row, const double4 (&p)[4], uint32_t sp, double& fx,
+                                           double& fy, double& fz, double& vl, uint32_t& hitm) {
+    bool ok[4], core[4];
     int tj[4];
-    double dx[4], dy[4], dz[4], r2[4], y[4], e[4], inv_d[4], d[4], sig[4], eps4[4], irc[4], r[4], inv_r[4];
-    double traw[4];
+    double dx[4], dy[4], dz[4], r2[4], y[4], e[4], inv_d[4], d[4], sig[4], eps4[4], e6[4], irc[4], r[4], inv_r[4];
+    double traw[4], thlo[4], thhi[4];
     double s6[4], s12[4], vlj[4], flj[4], sw[4], dsw[4], fmag[4], vv[4];
 #pragma unroll
-    for (int q = Q0; q < Q0 + NQ; ++q) {
-        const uint32_t tq = (tt >> (8 * q)) & 0xFFu;
-        live[q] = tq != 0u;
-        tj[q] = live[q] ? (int)tq - 1 : 0;
-    }
+    for (int q = Q0; q < Q0 + NQ; ++q) tj[q] = (int)min((sp >> (8 * q)) & 0xFFu, (uint32_t)(GS_NSPECIES - 1));
 #pragma unroll
     for (int q = Q0; q < Q0 + NQ; ++q) {
         dx[q] = pi.x - p[q].x;
@@ -507,9 +520,9 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
 #pragma unroll
     for (int q = Q0; q < Q0 + NQ; ++q) {
         const double t = fma(dz[q], dz[q], fma(dy[q], dy[q], dx[q] * dx[q]));
-        ok[q] = live[q] && t > P.eps2;
+        ok[q] = t > P.eps2;
         r2[q] = ok[q] ? t : 1.0;
-        if (BOUND) traw[q] = t;
+        if (SCREEN) traw[q] = t;
     }
     // rsqrt_refined, four at a time
 #pragma unroll
@@ -521,24 +534,30 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
 #pragma unroll
     for (int q = Q0; q < Q0 + NQ; ++q) {
         d[q] = r2[q] * inv_d[q];
-        const SpCoef cj = ssp[tj[q]];
-        irc[q] = irc_row[tj[q]];
-        sig[q] = 0.5 * (sig_i + cj.sigma);
-        eps4[q] = a4_i * cj.a4;
+        const ListCoef& c = lrow[tj[q]];
+        sig[q] = c.sig;
+        eps4[q] = c.eps4;
+        e6[q] = c.e6;
+        irc[q] = c.irc;
+        if (SCREEN) {
+            thlo[q] = c.thlo;
+            thhi[q] = c.thhi;
+        }
     }
-    if (BOUND) {
+    if (SCREEN) {
         uint32_t m = 0u;
 #pragma unroll
         for (int q = Q0; q < Q0 + NQ; ++q) {
             // (k_bind sigma_ab <= k_bind max sigma: inside the pair's own
-            // threshold is inside the cutoff)
-            const double t = traw[q], th2 = sig[q] * sig[q];
-            const bool sure = live[q] && t > 1e-24 && t < th2 * bs.kb2_lo;
-            const bool maybe = live[q] && t < th2 * bs.kb2_hi;
-            m |= (sure ? 1u : 0u) << q;
-            m |= (maybe && !sure ? 1u : 0u) << (4 + q);
+            // threshold is inside the cutoff; r^2 = 0 is no pair); the tests
+            // are combined without short-circuit so no branch is formed
+            const double t = traw[q];
+            const uint32_t sure = (uint32_t)(t > 1e-24) & (uint32_t)(t < thlo[q]);
+            const uint32_t maybe = (uint32_t)(t > 0.0) & (uint32_t)(t < thhi[q]);
+            m |= sure << q;
+            m |= (maybe & (sure ^ 1u)) << (4 + q);
         }
-        *hitm |= m;
+        hitm |= m;
     }
 #pragma unroll
     for (int q = Q0; q < Q0 + NQ; ++q) {
@@ -557,10 +576,12 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
 #pragma unroll
     for (int q = Q0; q < Q0 + NQ; ++q) {
         vlj[q] = eps4[q] * (s12[q] - s6[q]);
-        flj[q] = (6.0 * eps4[q] * inv_r[q]) * fma(2.0, s12[q], -s6[q]);
+        flj[q] = (e6[q] * inv_r[q]) * fma(2.0, s12[q], -s6[q]);
     }
+    if (!INNER) {
 #pragma unroll
-    for (int q = Q0; q < Q0 + NQ; ++q) switch_q(r[q], P.lj_ron, P.lj_inv_w, sw[q], dsw[q]);
+        for (int q = Q0; q < Q0 + NQ; ++q) switch_q(r[q], P.lj_ron, P.lj_inv_w, sw[q], dsw[q]);
+    }
     if (COUL && SHARED) {
 #pragma unroll
         for (int q = Q0; q < Q0 + NQ; ++q) {
@@ -569,14 +590,14 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
             const double vc = (kqi * p[q].w) * inv_c;
             const double fc = vc * inv_c;
             const double fs = flj[q] + fc, vs = vlj[q] + vc;
-            fmag[q] = fma(fs, sw[q], -vs * dsw[q]);
-            vv[q] = vs * sw[q];
+            fmag[q] = INNER ? fs : fma(fs, sw[q], -vs * dsw[q]);
+            vv[q] = INNER ? vs : vs * sw[q];
         }
     } else {
 #pragma unroll
         for (int q = Q0; q < Q0 + NQ; ++q) {
-            fmag[q] = fma(flj[q], sw[q], -vlj[q] * dsw[q]);
-            vv[q] = vlj[q] * sw[q];
+            fmag[q] = INNER ? flj[q] : fma(flj[q], sw[q], -vlj[q] * dsw[q]);
+            vv[q] = INNER ? vlj[q] : vlj[q] * sw[q];
         }
         if (COUL) {
 #pragma unroll
@@ -585,10 +606,15 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
                 const double inv_c = cc ? 2.0 : inv_d[q];
                 const double vc = (kqi * p[q].w) * inv_c;
                 const double fc = vc * inv_c;
-                double swc, dswc;
-                switch_q(d[q], P.c_ron, P.c_inv_w, swc, dswc);
-                fmag[q] += fma(fc, swc, -vc * dswc);
-                vv[q] += vc * swc;
+                if (INNER) {
+                    fmag[q] += fc;
+                    vv[q] += vc;
+                } else {
+                    double swc, dswc;
+                    switch_q(d[q], P.c_ron, P.c_inv_w, swc, dswc);
+                    fmag[q] += fma(fc, swc, -vc * dswc);
+                    vv[q] += vc * swc;
+                }
             }
         }
     }
@@ -599,10 +625,15 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
             const double inv_g = cc ? 2.0 : inv_d[q];
             const double vg = -(P.gconst * row[tj[q]].mm) * inv_g;
             const double fg = vg * inv_g;
-            double swg, dswg;
-            switch_q(d[q], P.g_ron, P.g_inv_w, swg, dswg);
-            fmag[q] += fma(fg, swg, -vg * dswg);
-            vv[q] += vg * swg;
+            if (INNER) {
+                fmag[q] += fg;
+                vv[q] += vg;
+            } else {
+                double swg, dswg;
+                switch_q(d[q], P.g_ron, P.g_inv_w, swg, dswg);
+                fmag[q] += fma(fg, swg, -vg * dswg);
+                vv[q] += vg * swg;
+            }
         }
     }
 #pragma unroll
@@ -1347,6 +1378,16 @@ constexpr int LDS_IDX_CAP_LOCAL = 2048;         // the same for the cell-staged 
 constexpr int LDS_IDX_CAP_BATCH = 4096;         // the same for a batch's lists (a build launch of its own)
 constexpr int MAX_LOGT = 4;                     // up to 16 lanes per atom
 constexpr uint32_t HOME_CAP = 256;              // home atoms whose list lengths the build keeps in LDS
+constexpr uint32_t LIST_CAP_MAX = 0xFFFFu;       // entries per atom a list length word holds
+constexpr int SPLIT_CAP = 512;                  // longest list the build writes split
+// List length word nlen[i]: bits 0-15 the entries, bits 16-30 the inner
+// entries of a split list, bit 31 set for a split list.  A split list holds
+// first the entries whose build distance lies within the inner radius
+// (switching radius - skin: inside every switching radius as long as the
+// list is in use), then the others, each class in candidate order.
+constexpr uint32_t NLEN_SPLIT = 0x80000000u;
+__host__ __device__ __forceinline__ uint32_t nlen_entries(uint32_t w) { return w & 0xFFFFu; }
+__host__ __device__ __forceinline__ uint32_t nlen_inner(uint32_t w) { return (w >> 16) & 0x7FFFu; }
 
 // Cell-staged list pass (pair_mode 2): entries are 16 bits,
 // (species << CELL_IDX_BITS) | position in the home cell's ordered
@@ -1534,15 +1575,22 @@ __global__ void k_flag_set(uint32_t* flags, int idx, uint32_t v) { flags[idx] = 
 // BATCH (gs_batch): the cells are the list-grid cells of all replicas; a
 // cell's stencil lies in its own replica's grid (bt), and cells of replicas
 // on brute-force pairs have empty buckets.
-template <bool LOCAL, int NT, bool BATCH = false>
+// SPLIT (the Verlet lists of one universe): with rin2 > 0, a cell whose
+// stencil fits one LDS chunk writes each list split (nlen_inner): the
+// candidates within the inner radius are written as they are found, the
+// others are staged in the wave's LDS row and appended after the sweep.  A
+// list longer than SPLIT_CAP or a cell of several chunks is written unsplit.
+template <bool LOCAL, int NT, bool BATCH = false, bool SPLIT = false>
 __device__ __forceinline__ void build_cells(
     uint32_t b0, uint32_t stride, KParams PL, const uint32_t* __restrict__ bcount,
     const uint32_t* __restrict__ bucket, uint32_t bcap, const double4* __restrict__ pos4,
     const uint8_t* __restrict__ type, uint32_t* __restrict__ nbr, uint32_t* __restrict__ nlen,
     double4* __restrict__ xref, uint32_t N, uint32_t logT, uint32_t cap, double cut2, uint32_t* __restrict__ flags,
     uint32_t* __restrict__ cand_store, uint4* __restrict__ cinfo, uint2* __restrict__ hrec,
-    uint16_t* __restrict__ nbr16, size_t region, const BatchTabs& bt = BatchTabs{}) {
+    uint16_t* __restrict__ nbr16, size_t region, const BatchTabs& bt = BatchTabs{}, double rin2 = 0.0) {
     constexpr int NWV = NT / WAVE;
+    // SPLIT: the outer candidates of the wave's current home atom
+    __shared__ uint32_t outb[SPLIT ? NWV : 1][SPLIT ? SPLIT_CAP : 1];
     // the candidates' indices are staged, unordered, in the memory of the
     // position tile, which is filled only after they have been ordered
     __shared__ BuildCand tile[BUILD_TILE];
@@ -1572,6 +1620,9 @@ __device__ __forceinline__ void build_cells(
     // below 2e-6 relative), a pair up to 5e-6 (rc + skin) farther may pass too;
     // it lies beyond rc, where the switched pair law is exactly zero
     const float cut2f = (float)(cutc * (1.0 + 1e-5));
+    // the inner class with the same error bound the other way: an entry
+    // classed inner lies within the inner radius
+    const float rin2f = (float)(rin2 * (1.0 - 1e-5));
     __syncthreads();  // the previous cell's LDS readers are done
     const uint32_t nh = min(bcount[c], bcap);
     if (nh == 0) {
@@ -1660,6 +1711,7 @@ __device__ __forceinline__ void build_cells(
         if (tid == 0) cinfo[c] = make_uint4(ncand, hs, nh, 0u);
     }
     const int nchunks = (int)((ncand + BUILD_TILE - 1) / BUILD_TILE);
+    const bool split = SPLIT && rin2 > 0.0 && nchunks == 1 && cap <= (uint32_t)SPLIT_CAP;
     for (int ch = 0; ch < nchunks; ++ch) {
         const uint32_t c0 = (uint32_t)ch * BUILD_TILE;
         const uint32_t len = min((uint32_t)BUILD_TILE, ncand - c0);
@@ -1698,11 +1750,13 @@ __device__ __forceinline__ void build_cells(
             const float3 pi = pi_n;
             if (i + NWV < nh) pi_n = home_pos(i + NWV);
             uint32_t cnt = ch == 0 ? 0u : (nh <= HOME_CAP ? hcnt[i] : nlen[gi]);
+            uint32_t cntB = 0u;  // split: outer candidates staged in outb[w]
             // 256 candidates per step, four per lane (independent distance
             // tests); the ballots are taken in candidate order, so the list
-            // is ordered by candidate position as with one per lane
+            // (each class of a split list) is ordered by candidate position
+            // as with one per lane
             for (uint32_t base = 0; base < len; base += 4 * WAVE) {
-                bool in[4];
+                bool in[4], outer[4];
                 uint32_t e[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
@@ -1713,7 +1767,9 @@ __device__ __forceinline__ void build_cells(
                     e[q] = LOCAL ? ((cj.e >> 28) << CELL_IDX_BITS) | (c0 + jc) : cj.e;
                     const float dx = pi.x - cj.x, dy = pi.y - cj.y, dz = pi.z - cj.z;
                     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    in[q] = j < len && r2 < cut2f && jg != gi;
+                    const bool any = j < len && r2 < cut2f && jg != gi;
+                    outer[q] = split && any && !(r2 < rin2f);
+                    in[q] = any && !outer[q];
                 }
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
@@ -1728,7 +1784,34 @@ __device__ __forceinline__ void build_cells(
                         }
                     }
                     cnt += (uint32_t)__popcll(m);
+                    if (SPLIT) {
+                        const uint64_t mo = __ballot(outer[q]);
+                        if (outer[q]) {
+                            const uint32_t kb = cntB + mask_rank(mo);
+                            if (kb < (uint32_t)SPLIT_CAP) outb[w][kb] = e[q];
+                        }
+                        cntB += (uint32_t)__popcll(mo);
+                    }
                 }
+            }
+            uint32_t nA = 0u;
+            if (SPLIT && split) {
+                // the outer class after the inner one (the wave's own LDS
+                // row: its writes above precede these reads in the wave's
+                // LDS order)
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                nA = min(cnt, cap);
+                const uint32_t nb = min(cntB, (uint32_t)SPLIT_CAP);
+                for (uint32_t kb = (uint32_t)lane; kb < nb; kb += WAVE) {
+                    const uint32_t k = cnt + kb;
+                    if (k < cap) nbr[nbr_slot(k, gi, N, logT)] = outb[w][kb];
+                }
+                cnt += cntB;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
             if (lane == 0) {
                 if (ch == nchunks - 1 && cnt > cap) {
@@ -1736,7 +1819,7 @@ __device__ __forceinline__ void build_cells(
                     cnt = cap;
                 }
                 if (nh <= HOME_CAP) hcnt[i] = cnt;
-                if (ch == nchunks - 1 || nh > HOME_CAP) nlen[gi] = cnt;
+                if (ch == nchunks - 1 || nh > HOME_CAP) nlen[gi] = (SPLIT && split) ? (cnt | (nA << 16) | NLEN_SPLIT) : cnt;
                 // LOCAL: home record ((species << 28) | atom, list length)
                 if (LOCAL && ch == nchunks - 1) hrec[(size_t)c * bcap + i] = make_uint2(((uint32_t)type[gi] << 28) | gi, cnt);
             }
@@ -1751,10 +1834,11 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
     const double4* __restrict__ pos4, const uint8_t* __restrict__ type, uint32_t* __restrict__ nbr,
     uint32_t* __restrict__ nlen, double4* __restrict__ xref, uint32_t N, uint32_t logT, uint32_t cap, double cut2,
     uint32_t* __restrict__ flags, uint32_t* __restrict__ cand_store, uint4* __restrict__ cinfo,
-    uint2* __restrict__ hrec, uint16_t* __restrict__ nbr16, size_t region) {
+    uint2* __restrict__ hrec, uint16_t* __restrict__ nbr16, size_t region, double rin2) {
     if (flags[0] == 0u) return;
-    build_cells<LOCAL, BUILD_THREADS>(blockIdx.x, gridDim.x, PL, bcount, bucket, bcap, pos4, type, nbr, nlen, xref, N,
-                                      logT, cap, cut2, flags, cand_store, cinfo, hrec, nbr16, region);
+    build_cells<LOCAL, BUILD_THREADS, false, !LOCAL>(blockIdx.x, gridDim.x, PL, bcount, bucket, bcap, pos4, type,
+                                                      nbr, nlen, xref, N, logT, cap, cut2, flags, cand_store, cinfo,
+                                                      hrec, nbr16, region, BatchTabs{}, rin2);
 }
 
 // Start of a list tick with the build one tick behind the binning
@@ -1774,12 +1858,14 @@ __global__ __launch_bounds__(BLOCK) void k_list_step(
     const double* __restrict__ az, const double* __restrict__ lam_ptr, double4* __restrict__ xref,
     double4* __restrict__ xbin, double half_skin, uint32_t* __restrict__ bcount, uint32_t* __restrict__ bucket,
     uint32_t bcap, uint32_t* __restrict__ flags, uint32_t natom, const uint8_t* __restrict__ type,
-    uint32_t* __restrict__ nbr, uint32_t* __restrict__ nlen, uint32_t N, uint32_t logT, uint32_t cap, double cut2) {
+    uint32_t* __restrict__ nbr, uint32_t* __restrict__ nlen, uint32_t N, uint32_t logT, uint32_t cap, double cut2,
+    double rin2) {
     const bool build_now = flags[11] != 0u;
     if (blockIdx.x >= natom) {
         if (!build_now) return;
-        build_cells<false, BLOCK>(blockIdx.x - natom, gridDim.x - natom, PL, bcount, bucket, bcap, xbin, type, nbr,
-                                  nlen, xref, N, logT, cap, cut2, flags, nullptr, nullptr, nullptr, nullptr, 0);
+        build_cells<false, BLOCK, false, true>(blockIdx.x - natom, gridDim.x - natom, PL, bcount, bucket, bcap, xbin,
+                                               type, nbr, nlen, xref, N, logT, cap, cut2, flags, nullptr, nullptr,
+                                               nullptr, nullptr, 0, BatchTabs{}, rin2);
         return;
     }
     const int tid = threadIdx.x, lane = tid & 63;
@@ -1861,7 +1947,9 @@ struct BoundArgs {
     uint32_t* hits = nullptr;   // [h * (N << LOGT) + lane]: the lane's h-th screened list entry
     uint32_t* hinfo = nullptr;  // [lane]: screened entries (clamped at 255) | band mask << 16
     BoundScreen scr;            // the list pass's screen around k_bind sigma_ab
+    uint32_t screen_all = 1u;   // 0: the outer entries of split lists can hold no bound pair
 };
+
 
 // Per-atom list traversal fused with the second half of the tick: T lanes
 // per atom split its list (entries s, s+T, ...) and sum with shuffles; the
@@ -1872,6 +1960,9 @@ struct BoundArgs {
 // kinetic energy; the last workgroup reduces the per-block kinetic and
 // potential partials in a fixed order, computes the Berendsen lambda and the
 // observables (close_tick) and clears the rebuild flag for the next tick.
+// Split lists: the groups that every lane of the wave has inside its inner
+// class skip the switch; with the bound-pair screen, the groups past every
+// lane's inner class skip the screen when no outer entry can be a bound pair.
 // BATCH (gs_batch): the atoms of many replicas; each atom takes its
 // replica's box from the batch tables and atoms of replicas on brute-force
 // pairs are skipped.  The kernel stores a = F/m and the atom's potential
@@ -1879,9 +1970,6 @@ struct BoundArgs {
 // reductions run in the batch's own kernels.
 #ifndef GS_LIST_WAVES
 #define GS_LIST_WAVES 2  // minimum waves per SIMD the list pair kernel is compiled for
-#endif
-#ifndef GS_PAIR_SPLIT
-#define GS_PAIR_SPLIT 1  // a group's four pairs are evaluated as two halves (0: all four at once)
 #endif
 template <int LOGT, bool COUL, bool SHARED, bool SINGLE, bool GRAV, bool BATCH = false, bool BOUND = false>
 __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, uint32_t N, const double4* __restrict__ pos4,
@@ -1900,9 +1988,8 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
                                                      BatchTabs bt = BatchTabs{}, double* __restrict__ pe_out = nullptr,
                                                      BoundArgs ba = BoundArgs{}) {
     constexpr int T = 1 << LOGT;
-    __shared__ PairCoef sc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];
-    __shared__ SpCoef ssp[GS_NSPECIES];
-    __shared__ double sirc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];  // 1 / rcore, 8-byte stride
+    __shared__ PairCoef sc[(SINGLE || GRAV) ? (SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES) : 1];
+    __shared__ ListCoef lc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];
     __shared__ double wv[BLOCK / WAVE];
     __shared__ double wk[BLOCK / WAVE];
     __shared__ double red[2 * BLOCK / WAVE];
@@ -1940,11 +2027,20 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
     if (SINGLE) {
         if (tid == 0) sc[0] = coef[0];
     } else {
+        const SpCoef* spc = reinterpret_cast<const SpCoef*>(coef + COEF_SP_OFFSET);
         for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += BLOCK) {
-            sc[t] = coef[t];
-            sirc[t] = coef[t].inv_rcore;
+            if (GRAV) sc[t] = coef[t];
+            const SpCoef a = spc[t / GS_NSPECIES], b = spc[t % GS_NSPECIES];
+            ListCoef c;
+            c.sig = 0.5 * (a.sigma + b.sigma);
+            c.eps4 = a.a4 * b.a4;
+            c.e6 = 6.0 * c.eps4;
+            c.irc = coef[t].inv_rcore;
+            const double th2 = c.sig * c.sig;
+            c.thlo = th2 * ba.scr.kb2_lo;
+            c.thhi = th2 * ba.scr.kb2_hi;
+            lc[t] = c;
         }
-        if (tid < GS_NSPECIES) ssp[tid] = reinterpret_cast<const SpCoef*>(coef + COEF_SP_OFFSET)[tid];
     }
     __syncthreads();
     // a tick that rebuilt leaves the buckets empty for the next binning
@@ -1961,12 +2057,9 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
     // lists unwritten: the host replays this call, so no list is read now
     const bool lists_ok = (flags[2] | flags[6]) == 0u;
     bool own = i < N;  // BATCH: and the atom's replica is on lists
+    double Lx = P.box[0], Ly = P.box[1], Lz = P.box[2];
+    double iLx = 0.0, iLy = 0.0, iLz = 0.0;
     if (i < N) {
-        const double4 pi = pi0;
-        ti = ti0;
-        const double kqi = P.ke * pi.w;
-        double Lx = P.box[0], Ly = P.box[1], Lz = P.box[2];
-        double iLx, iLy, iLz;
         if (BATCH) {
             const uint32_t rep = bt.arep[i];
             const double4 bx = bt.box[rep], ib = bt.ibox[rep];
@@ -1982,13 +2075,37 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
             iLy = 1.0 / Ly;
             iLz = 1.0 / Lz;
         }
-        const uint32_t len = lists_ok && own ? nl0 : 0u;
-        const PairCoef* row = SINGLE ? sc : sc + ti * GS_NSPECIES;
-        const double sig_i = SINGLE ? 0.0 : ssp[ti].sigma, a4_i = SINGLE ? 0.0 : ssp[ti].a4;
-        // this lane's entries: s, s+T, ... (cnt of them), four per group;
+    }
+    // this lane's entries: s, s+T, ... (cnt of them), four per group, the
+    // first cntA of them in the list's inner class
+    const uint32_t len = lists_ok && own ? nlen_entries(nl0) : 0u;
+    const bool split = (nl0 & NLEN_SPLIT) != 0u;
+    const uint32_t nA = split ? min(nlen_inner(nl0), len) : 0u;
+    const uint32_t cnt = len > s ? (len - s + T - 1) >> LOGT : 0u;
+    const uint32_t cntA = nA > s ? (nA - s + T - 1) >> LOGT : 0u;
+    // Wave-uniform group ranges: groups [0, gin) hold inner entries only on
+    // every lane of the wave (a lane whose entries are all inner does not
+    // limit it); BOUND: groups from gout on hold no entry the screen needs.
+    uint32_t gin = 0u, gout = 0xFFFFFFFFu;
+    if (!SINGLE) {
+        uint32_t mn = cntA >= cnt ? 0xFFFFFFFFu : cntA;
+        uint32_t mx = (BOUND && (!split || ba.screen_all)) ? cnt : cntA;
+#pragma unroll
+        for (int off = WAVE / 2; off > 0; off >>= 1) {
+            mn = min(mn, (uint32_t)__shfl_xor((int)mn, off, WAVE));
+            mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, WAVE));
+        }
+        gin = __builtin_amdgcn_readfirstlane(mn) >> 2;
+        if (BOUND) gout = (__builtin_amdgcn_readfirstlane(mx) + 3u) >> 2;
+    }
+    if (i < N) {
+        const double4 pi = pi0;
+        ti = ti0;
+        const double kqi = P.ke * pi.w;
+        const PairCoef* row = SINGLE ? sc : sc + ((SINGLE || GRAV) ? ti * GS_NSPECIES : 0);
+        const ListCoef* lrow = lc + (SINGLE ? 0 : ti * GS_NSPECIES);
         // a group's four position gathers are all in flight before its four
         // pairs are evaluated
-        const uint32_t cnt = len > s ? (len - s + T - 1) >> LOGT : 0u;
         const uint4* grp = reinterpret_cast<const uint4*>(nbr) + (((size_t)i << LOGT) + s);
         const size_t gstride = (size_t)N << LOGT;
         // Software pipeline over the lane's entry groups, two groups per
@@ -2024,36 +2141,40 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
                 nhit += h != 0u ? 1u : 0u;
             }
         };
-        auto gather = [&](const uint4& e, uint32_t k, double4 (&p)[4], uint32_t& tt) {
+        // positions of a group's four entries (past the end: the atom's
+        // own) and their species, one byte each
+        auto gather = [&](const uint4& e, uint32_t k, double4 (&p)[4], uint32_t& sp) {
             const uint32_t f4[4] = {e.x, e.y, e.z, e.w};
-            tt = 0u;
+            sp = 0u;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const bool live = 4 * k + q < cnt;
                 p[q] = pos4[live ? (f4[q] & NBR_IDX_MASK) : i];
-                tt |= (live ? (f4[q] >> 28) + 1u : 0u) << (8 * q);  // 0: past the end
+                sp |= (f4[q] >> 28) << (8 * q);
             }
         };
-        auto evaluate = [&](const double4 (&p)[4], uint32_t tt) -> uint32_t {
+        // the pair law over one group (SINGLE: the scalar law of the
+        // one-coefficient table)
+        auto evaluate = [&](auto inner_c, auto screen_c, const double4 (&p)[4], uint32_t sp) -> uint32_t {
+            constexpr bool IN = decltype(inner_c)::value, SC = decltype(screen_c)::value;
             uint32_t hitm = 0u;
             if (SINGLE) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const bool live = ((tt >> (8 * q)) & 0xFFu) != 0u;
                     const double4 pj = p[q];
                     double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                     dx = fma(-Lx, rint(dx * iLx), dx);
                     dy = fma(-Ly, rint(dy * iLy), dy);
                     dz = fma(-Lz, rint(dz * iLz), dz);
                     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                    if (BOUND) {
+                    if (SC) {
                         const double th2 = row[0].sigma * row[0].sigma;
-                        const bool sure = live && r2 > 1e-24 && r2 < th2 * ba.scr.kb2_lo;
-                        const bool maybe = live && r2 < th2 * ba.scr.kb2_hi;
+                        const bool sure = r2 > 1e-24 && r2 < th2 * ba.scr.kb2_lo;
+                        const bool maybe = r2 > 0.0 && r2 < th2 * ba.scr.kb2_hi;
                         hitm |= (sure ? 1u : 0u) << q;
                         hitm |= (maybe && !sure ? 1u : 0u) << (4 + q);
                     }
-                    const bool ok = live && r2 > P.eps2;
+                    const bool ok = r2 > P.eps2;
                     const double r2s = ok ? r2 : 1.0;
                     const double inv_d = rsqrt_refined(r2s);
                     const double d = r2s * inv_d;
@@ -2066,18 +2187,10 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
                     vl += ok ? vv : 0.0;
                 }
             } else {
-#if GS_PAIR_SPLIT
-                pair_eval4<COUL, SHARED, GRAV, BOUND, 0, 2>(P, Lx, Ly, Lz, iLx, iLy, iLz, pi, kqi, sig_i, a4_i, ssp,
-                                                            sirc + ti * GS_NSPECIES, row, p, tt, fx, fy, fz, vl,
-                                                            ba.scr, &hitm);
-                pair_eval4<COUL, SHARED, GRAV, BOUND, 2, 2>(P, Lx, Ly, Lz, iLx, iLy, iLz, pi, kqi, sig_i, a4_i, ssp,
-                                                            sirc + ti * GS_NSPECIES, row, p, tt, fx, fy, fz, vl,
-                                                            ba.scr, &hitm);
-#else
-                pair_eval4<COUL, SHARED, GRAV, BOUND>(P, Lx, Ly, Lz, iLx, iLy, iLz, pi, kqi, sig_i, a4_i, ssp,
-                                                      sirc + ti * GS_NSPECIES, row, p, tt, fx, fy, fz, vl, ba.scr,
-                                                      &hitm);
-#endif
+                pair_eval4<COUL, SHARED, GRAV, IN, SC, 0, 2>(P, Lx, Ly, Lz, iLx, iLy, iLz, pi, kqi, lrow, row, p, sp,
+                                                             fx, fy, fz, vl, hitm);
+                pair_eval4<COUL, SHARED, GRAV, IN, SC, 2, 2>(P, Lx, Ly, Lz, iLx, iLy, iLz, pi, kqi, lrow, row, p, sp,
+                                                             fx, fy, fz, vl, hitm);
             }
             return hitm;
         };
@@ -2091,30 +2204,50 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
         __builtin_amdgcn_sched_barrier(0);
         gather(eA, 0, pA, tA);
         if (BOUND) jA = eA;
-        // no exit between a step's loads and the next step (the compiler
-        // would sink the loads past it): a lane with an odd number of
-        // groups evaluates one group of four masked pairs at its end
-        for (uint32_t k = 0; k < ng; k += 2) {
-            // step A: group k
-            __builtin_amdgcn_sched_barrier(0);
-            eA = load_nt(group_ptr(k + 2));
-            gather(eB, k + 1, pB, tB);
-            if (BOUND) jB = eB;
-            __builtin_amdgcn_sched_barrier(0);
-            {
-                const uint32_t hm = evaluate(pA, tA);
-                if (BOUND) record(hm, jA);
+        // Trips of two groups (k, k + 1) while k < kend, with the law of one
+        // group class.  No exit between a step's loads and the next step (the
+        // compiler would sink the loads past it): a lane with an odd number
+        // of groups evaluates one group of four masked pairs at its end.
+        // The classes run as separate loops, so each is scheduled and its
+        // registers allocated on its own.
+        uint32_t k = 0u;
+        auto trips = [&](auto inner_c, auto screen_c, uint32_t kend) {
+            constexpr bool SC = decltype(screen_c)::value;
+            for (; k < kend; k += 2) {
+                // step A: group k
+                __builtin_amdgcn_sched_barrier(0);
+                eA = load_nt(group_ptr(k + 2));
+                gather(eB, k + 1, pB, tB);
+                if (BOUND) jB = eB;
+                __builtin_amdgcn_sched_barrier(0);
+                {
+                    const uint32_t hm = evaluate(inner_c, screen_c, pA, tA);
+                    if (SC) record(hm, jA);
+                }
+                // step B: group k + 1
+                __builtin_amdgcn_sched_barrier(0);
+                eB = load_nt(group_ptr(k + 3));
+                gather(eA, k + 2, pA, tA);
+                if (BOUND) jA = eA;
+                __builtin_amdgcn_sched_barrier(0);
+                {
+                    const uint32_t hm = evaluate(inner_c, screen_c, pB, tB);
+                    if (SC) record(hm, jB);
+                }
             }
-            // step B: group k + 1
-            __builtin_amdgcn_sched_barrier(0);
-            eB = load_nt(group_ptr(k + 3));
-            gather(eA, k + 2, pA, tA);
-            if (BOUND) jA = eA;
-            __builtin_amdgcn_sched_barrier(0);
-            {
-                const uint32_t hm = evaluate(pB, tB);
-                if (BOUND) record(hm, jB);
-            }
+        };
+        using yes = std::integral_constant<bool, true>;
+        using no = std::integral_constant<bool, false>;
+        using scr = std::integral_constant<bool, BOUND>;
+        if (SINGLE) {
+            trips(no{}, scr{}, ng);
+        } else {
+            // trips of two inner groups, then trips the screen needs, then
+            // the rest (gin rounded down, gout up, to whole trips)
+            const uint32_t kin = gin & ~1u;
+            trips(yes{}, scr{}, min(kin, ng));
+            if (BOUND) trips(no{}, scr{}, min(max(kin, (gout + 1u) & ~1u), ng));
+            trips(no{}, no{}, ng);
         }
     }
 #pragma unroll
@@ -3236,6 +3369,23 @@ bool list_geometry(gs_universe* u, double skin) {
 // Entries per cell of the cell-staged lists: a row of cap per bucket slot.
 size_t cell_region(const gs_universe* u) { return (size_t)u->bcap * (size_t)u->nbr_cap; }
 
+// Inner radius of the split Verlet lists: the smallest switching radius in
+// use minus the skin.  An entry built within it stays inside every switch
+// while its list is valid (each end moves less than skin/2), 0 when none
+// can.
+double list_inner(const gs_universe* u) {
+    const KParams& P = u->kp;
+    double r = P.lj_ron;
+    if (u->coul && !u->shared_switch) r = std::min(r, P.c_ron);
+    if (u->prm.enable_gravity) r = std::min(r, P.g_ron);
+    r -= u->skin;
+    return r > 0.0 ? r : 0.0;
+}
+double list_inner2(const gs_universe* u) {
+    const double r = list_inner(u);
+    return r * r;
+}
+
 int list_alloc(gs_universe* u, uint32_t cap, uint32_t bcap_min = 0) {
     int rc;
     const size_t n = (size_t)std::max(1, u->n);
@@ -3452,7 +3602,7 @@ int list_tick_start(gs_universe* u) {
         k_list_step<<<natom + nbuild, BLOCK, 0, u->stream>>>(
             u->kp, PL, u->pos4[c], u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->scal, u->xref,
             u->xbin, 0.5 * u->skin, u->bcount, u->bucket, u->bcap, u->flags, natom, u->type[c], u->nbr, u->nlen, N,
-            (uint32_t)u->logT, u->nbr_cap, rl * rl);
+            (uint32_t)u->logT, u->nbr_cap, rl * rl, list_inner2(u));
         u->list_age += 1;
         HIPCHK(hipGetLastError());
         return GS_OK;
@@ -3476,11 +3626,11 @@ int list_tick_start(gs_universe* u) {
     if (u->pair_mode == 2)
         k_list_build<true><<<bgrid, BUILD_THREADS, 0, u->stream>>>(
             PL, u->bcount, u->bucket, u->bcap, u->pos4[c], u->type[c], nullptr, u->nlen, u->xref, N, 0u, u->nbr_cap,
-            rl * rl, u->flags, u->cand_store, u->cinfo, u->hrec, u->nbr16, cell_region(u));
+            rl * rl, u->flags, u->cand_store, u->cinfo, u->hrec, u->nbr16, cell_region(u), 0.0);
     else
         k_list_build<false><<<bgrid, BUILD_THREADS, 0, u->stream>>>(
             PL, u->bcount, u->bucket, u->bcap, u->pos4[c], u->type[c], u->nbr, u->nlen, u->xref, N,
-            (uint32_t)u->logT, u->nbr_cap, rl * rl, u->flags, nullptr, nullptr, nullptr, nullptr, 0);
+            (uint32_t)u->logT, u->nbr_cap, rl * rl, u->flags, nullptr, nullptr, nullptr, nullptr, 0, list_inner2(u));
     u->list_age += 1;
     HIPCHK(hipGetLastError());
     return GS_OK;
@@ -3987,7 +4137,7 @@ int step_segment(gs_universe* u, int64_t n_steps, gs_observables* obs) {
                 list = false;
             } else {
                 const uint32_t cap = fl[1] ? u->nbr_cap * 2 : u->nbr_cap;
-                if (cap > (1u << 16)) return fail(GS_E_OVERFLOW, "neighbour list exceeds 65536 entries per atom");
+                if (cap > LIST_CAP_MAX) return fail(GS_E_OVERFLOW, "neighbour list exceeds 65535 entries per atom");
                 if ((rc = list_alloc(u, cap, fl[6] ? 2 * u->bcap : 0))) return rc;
             }
             continue;
