@@ -50,7 +50,7 @@ CPU_SHARE = 16              # host cores one GPU's job may use on the GPU box
 # WRITE_SIZE (KiB per dispatch) of `bench.py --leg ionic` at the default setting, scaled by
 # what the same counters read for a 256 MiB device copy (`bench.py --copy-probe`).  Only
 # reported for the setting it was measured at.
-PAIR_TRAFFIC = {"setting": {"lanes": 2, "skin_frac": 0.04}, "fetch_kib": 420947.8, "write_kib": 119883.7,
+PAIR_TRAFFIC = {"setting": {"lanes": 2, "skin_frac": 0.04}, "fetch_kib": 414164.1, "write_kib": 119668.3,
                 "fetch_scale": 2.0, "write_scale": 1.0,
                 "source": "profiles/k_pair_list_bound__ionic_melt1048576_pmc.md (ticks 12001..12500)"}
 # the melt is a solid in the timed window: its resident order keeps its locality, so a full
@@ -267,9 +267,11 @@ def mixture_leg(steps: int = 2000, warmup: int = 100) -> dict:
         smp = Sampler(u, 0, w.extra["gr_rmax"], w.extra["gr_bins"], comp_every=0)
         smp.run(warmup)
         u.timing(enable="pair", every=10)
+        u.speed_stats()  # window marker (k_speed_parts) for kernel traces and counters
         t0 = time.perf_counter()
         smp.run(steps)
         el = time.perf_counter() - t0
+        u.speed_stats()
         tm = u.timing()
         u.timing(enable=False)
         fin = u.download()
@@ -286,25 +288,37 @@ def mixture_leg(steps: int = 2000, warmup: int = 100) -> dict:
             "autotune": {k: t[k] for k in ("pair_mode", "lanes_per_atom", "skin", "ms_per_tick")}}
 
 
-def batched_leg(ticks: int = 200, segment: int = 100) -> dict:
+def batched_leg(ticks: int = 200, segment: int = 100, lanes: int = 0) -> dict:
     """Temperature sweep: 64 replicas of log-uniform sizes in [512, 16384]
     (20 K .. 120 K), stepped (a) as one batch (one launch per phase, hipGraph
     segments of up to 100 ticks, autotuned pair path: Verlet lists or cell
     tiles) and (b) one universe after another through gs_step with its own
-    default setting (the per-replica-launch plain path)."""
+    default setting (the per-replica-launch plain path).  lanes > 0 fixes the
+    batch on Verlet lists with that many lanes per atom instead of autotuning
+    (for profiles of one setting)."""
     from genesis_amd.batch import GpuBatch
     from genesis_amd.inputs import replica_sweep
     from genesis_amd.universe import GpuUniverse
     ws = replica_sweep()
     total = sum(w.state.n for w in ws)
+    # window markers for kernel traces and counters: k_speed_parts of a
+    # small universe before and after the timed batch steps
+    marker = GpuUniverse(ws[0].physics, ws[0].state)
     with GpuBatch([w.physics for w in ws], [w.state for w in ws]) as b:
-        tune = b.autotune(20)
+        if lanes > 0:
+            b.set_tuning(1, lanes)
+            tune = {"cell_scale": 1.0}
+        else:
+            tune = b.autotune(20)
         b.step(segment, graph_segment=segment)
         builds0 = b.get_tuning()["list_builds"]
+        marker.speed_stats()
         t0 = time.perf_counter()
         b.step(ticks, graph_segment=segment)
         tb = time.perf_counter() - t0
+        marker.speed_stats()
         btun = b.get_tuning()
+    marker.close()
     us = [GpuUniverse(w.physics, w.state) for w in ws]
     for u in us:
         u.step(20, observe=False)
@@ -344,12 +358,14 @@ def adjoint_leg(steps: int = 1000, ckpt: int = 50) -> dict:
         _, g0 = u.adjoint_forward(th0, keep, steps, ckpt, win, gr=gr)
         th = th0 * (1.0 + 0.005 * np.cos(np.arange(30)))
         u.upload(st)
+        u.speed_stats()  # window marker (k_speed_parts) for kernel traces and counters
         t0 = time.perf_counter()
         loss, _ = u.adjoint_forward(th, keep, steps, ckpt, win, gr=gr, g0=g0)
         tf = time.perf_counter() - t0
         t0 = time.perf_counter()
         grad = u.adjoint_backward()
         tb = time.perf_counter() - t0
+        u.speed_stats()
     with GpuUniverse(ph, st) as u:
         u.step(50, observe=False)
         t0 = time.perf_counter()
@@ -451,6 +467,8 @@ def main() -> int:
                     help="only run a 256 MiB device copy (HBM counter calibration) and exit")
     ap.add_argument("--leg", choices=("ionic", "mixture", "batched", "adjoint"), default=None,
                     help="run only that leg and print its JSON record, for profiling")
+    ap.add_argument("--batch-lanes", type=int, default=0,
+                    help="batched leg: Verlet lists with this many lanes per atom instead of the autotuner's pick")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe()
@@ -462,7 +480,8 @@ def main() -> int:
         print(json.dumps(mixture_leg(steps=args.steps, warmup=args.warmup)), flush=True)
         return 0
     if args.leg == "batched":
-        print(json.dumps(batched_leg(ticks=args.steps, segment=min(100, args.steps))), flush=True)
+        print(json.dumps(batched_leg(ticks=args.steps, segment=min(100, args.steps), lanes=args.batch_lanes)),
+              flush=True)
         return 0
     if args.leg == "adjoint":
         print(json.dumps(adjoint_leg(steps=args.steps, ckpt=min(50, args.steps))), flush=True)
