@@ -1380,14 +1380,25 @@ constexpr int MAX_LOGT = 4;                     // up to 16 lanes per atom
 constexpr uint32_t HOME_CAP = 256;              // home atoms whose list lengths the build keeps in LDS
 constexpr uint32_t LIST_CAP_MAX = 0xFFFFu;       // entries per atom a list length word holds
 constexpr int SPLIT_CAP = 512;                  // longest list the build writes split
-// List length word nlen[i]: bits 0-15 the entries, bits 16-30 the inner
-// entries of a split list, bit 31 set for a split list.  A split list holds
-// first the entries whose build distance lies within the inner radius
-// (switching radius - skin: inside every switching radius as long as the
-// list is in use), then the others, each class in candidate order.
+// List length word nlen[i].  An unsplit list: the entries (bits 0-15).  A
+// split list (bit 31 set): the entries (bits 0-9), the inner entries (bits
+// 10-19) and, among those, the bind entries (bits 20-29).  A split list holds
+// first the entries whose build distance lies within the bind radius (the
+// bond cutoff + skin: the only pairs that can be bound while the list is in
+// use), then the other entries within the inner radius (the smallest
+// switching radius - skin: inside every switching radius while the list is
+// in use), then the rest, each class in candidate order.
 constexpr uint32_t NLEN_SPLIT = 0x80000000u;
-__host__ __device__ __forceinline__ uint32_t nlen_entries(uint32_t w) { return w & 0xFFFFu; }
-__host__ __device__ __forceinline__ uint32_t nlen_inner(uint32_t w) { return (w >> 16) & 0x7FFFu; }
+static_assert(SPLIT_CAP < 1024, "a split list's counts fit ten bits");
+__host__ __device__ __forceinline__ uint32_t nlen_entries(uint32_t w) {
+    return (w & NLEN_SPLIT) ? (w & 0x3FFu) : (w & 0xFFFFu);
+}
+__host__ __device__ __forceinline__ uint32_t nlen_inner(uint32_t w) {
+    return (w & NLEN_SPLIT) ? ((w >> 10) & 0x3FFu) : 0u;
+}
+__host__ __device__ __forceinline__ uint32_t nlen_bind(uint32_t w) {
+    return (w & NLEN_SPLIT) ? ((w >> 20) & 0x3FFu) : 0u;
+}
 
 // Cell-staged list pass (pair_mode 2): entries are 16 bits,
 // (species << CELL_IDX_BITS) | position in the home cell's ordered
@@ -1576,10 +1587,12 @@ __global__ void k_flag_set(uint32_t* flags, int idx, uint32_t v) { flags[idx] = 
 // cell's stencil lies in its own replica's grid (bt), and cells of replicas
 // on brute-force pairs have empty buckets.
 // SPLIT (the Verlet lists of one universe): with rin2 > 0, a cell whose
-// stencil fits one LDS chunk writes each list split (nlen_inner): the
-// candidates within the inner radius are written as they are found, the
-// others are staged in the wave's LDS row and appended after the sweep.  A
-// list longer than SPLIT_CAP or a cell of several chunks is written unsplit.
+// stencil fits one LDS chunk writes each list split (nlen_inner, nlen_bind):
+// its accepted candidates are staged in the wave's LDS row with their class
+// (within the bind radius, rb2 being its square and at most rin2; within
+// the inner radius; the rest) and written out class by class after the
+// sweep.  A list longer than SPLIT_CAP or a cell of several chunks is
+// written unsplit.
 template <bool LOCAL, int NT, bool BATCH = false, bool SPLIT = false>
 __device__ __forceinline__ void build_cells(
     uint32_t b0, uint32_t stride, KParams PL, const uint32_t* __restrict__ bcount,
@@ -1587,10 +1600,12 @@ __device__ __forceinline__ void build_cells(
     const uint8_t* __restrict__ type, uint32_t* __restrict__ nbr, uint32_t* __restrict__ nlen,
     double4* __restrict__ xref, uint32_t N, uint32_t logT, uint32_t cap, double cut2, uint32_t* __restrict__ flags,
     uint32_t* __restrict__ cand_store, uint4* __restrict__ cinfo, uint2* __restrict__ hrec,
-    uint16_t* __restrict__ nbr16, size_t region, const BatchTabs& bt = BatchTabs{}, double rin2 = 0.0) {
+    uint16_t* __restrict__ nbr16, size_t region, const BatchTabs& bt = BatchTabs{}, double rin2 = 0.0,
+    double rb2 = 0.0) {
     constexpr int NWV = NT / WAVE;
     // SPLIT: the outer candidates of the wave's current home atom
     __shared__ uint32_t outb[SPLIT ? NWV : 1][SPLIT ? SPLIT_CAP : 1];
+    __shared__ uint8_t outc[SPLIT ? NWV : 1][SPLIT ? SPLIT_CAP : 1];
     // the candidates' indices are staged, unordered, in the memory of the
     // position tile, which is filled only after they have been ordered
     __shared__ BuildCand tile[BUILD_TILE];
@@ -1621,8 +1636,10 @@ __device__ __forceinline__ void build_cells(
     // it lies beyond rc, where the switched pair law is exactly zero
     const float cut2f = (float)(cutc * (1.0 + 1e-5));
     // the inner class with the same error bound the other way: an entry
-    // classed inner lies within the inner radius
+    // classed inner lies within the inner radius; the bind class takes every
+    // candidate that may lie within the bind radius
     const float rin2f = (float)(rin2 * (1.0 - 1e-5));
+    const float rb2f = (float)(rb2 * (1.0 + 1e-5));
     __syncthreads();  // the previous cell's LDS readers are done
     const uint32_t nh = min(bcount[c], bcap);
     if (nh == 0) {
@@ -1750,14 +1767,15 @@ __device__ __forceinline__ void build_cells(
             const float3 pi = pi_n;
             if (i + NWV < nh) pi_n = home_pos(i + NWV);
             uint32_t cnt = ch == 0 ? 0u : (nh <= HOME_CAP ? hcnt[i] : nlen[gi]);
-            uint32_t cntB = 0u;  // split: outer candidates staged in outb[w]
             // 256 candidates per step, four per lane (independent distance
             // tests); the ballots are taken in candidate order, so the list
             // (each class of a split list) is ordered by candidate position
-            // as with one per lane
+            // as with one per lane.  Split: every accepted candidate is
+            // staged in the wave's LDS row with its class (0 bind, 1 other
+            // inner, 2 the rest) and the row is written out class by class.
             for (uint32_t base = 0; base < len; base += 4 * WAVE) {
-                bool in[4], outer[4];
-                uint32_t e[4];
+                bool in[4];
+                uint32_t e[4], cls[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const uint32_t j = base + (uint32_t)(q * WAVE + lane);
@@ -1767,16 +1785,20 @@ __device__ __forceinline__ void build_cells(
                     e[q] = LOCAL ? ((cj.e >> 28) << CELL_IDX_BITS) | (c0 + jc) : cj.e;
                     const float dx = pi.x - cj.x, dy = pi.y - cj.y, dz = pi.z - cj.z;
                     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    const bool any = j < len && r2 < cut2f && jg != gi;
-                    outer[q] = split && any && !(r2 < rin2f);
-                    in[q] = any && !outer[q];
+                    in[q] = j < len && r2 < cut2f && jg != gi;
+                    cls[q] = r2 < rb2f ? 0u : (r2 < rin2f ? 1u : 2u);
                 }
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const uint64_t m = __ballot(in[q]);
                     if (in[q]) {
                         const uint32_t k = cnt + mask_rank(m);
-                        if (k < cap) {
+                        if (SPLIT && split) {
+                            if (k < (uint32_t)SPLIT_CAP) {
+                                outb[w][k] = e[q];
+                                outc[w][k] = (uint8_t)cls[q];
+                            }
+                        } else if (k < cap) {
                             if (LOCAL)
                                 reg[(size_t)i * cap + k] = (uint16_t)e[q];
                             else
@@ -1784,42 +1806,46 @@ __device__ __forceinline__ void build_cells(
                         }
                     }
                     cnt += (uint32_t)__popcll(m);
-                    if (SPLIT) {
-                        const uint64_t mo = __ballot(outer[q]);
-                        if (outer[q]) {
-                            const uint32_t kb = cntB + mask_rank(mo);
-                            if (kb < (uint32_t)SPLIT_CAP) outb[w][kb] = e[q];
-                        }
-                        cntB += (uint32_t)__popcll(mo);
-                    }
                 }
             }
-            uint32_t nA = 0u;
-            if (SPLIT && split) {
-                // the outer class after the inner one (the wave's own LDS
-                // row: its writes above precede these reads in the wave's
-                // LDS order)
+            uint32_t nA = 0u, nB = 0u;
+            // more accepted candidates than the LDS row holds: the list
+            // exceeds its capacity (cap <= SPLIT_CAP) and is written empty,
+            // the call being replayed
+            const bool lost = SPLIT && split && cnt > (uint32_t)SPLIT_CAP;
+            if (SPLIT && split && !lost) {
+                // the staged row, class by class (the wave's own LDS row: its
+                // writes above precede these reads in the wave's LDS order)
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                nA = min(cnt, cap);
-                const uint32_t nb = min(cntB, (uint32_t)SPLIT_CAP);
-                for (uint32_t kb = (uint32_t)lane; kb < nb; kb += WAVE) {
-                    const uint32_t k = cnt + kb;
-                    if (k < cap) nbr[nbr_slot(k, gi, N, logT)] = outb[w][kb];
+                uint32_t at = 0u;
+                for (uint32_t c = 0; c < 3u; ++c) {
+                    for (uint32_t k0 = 0; k0 < cnt; k0 += WAVE) {
+                        const uint32_t k = k0 + (uint32_t)lane;
+                        const bool mine = k < cnt && outc[w][k] == c;
+                        const uint64_t mc = __ballot(mine);
+                        if (mine) {
+                            const uint32_t pos = at + mask_rank(mc);
+                            if (pos < cap) nbr[nbr_slot(pos, gi, N, logT)] = outb[w][k];
+                        }
+                        at += (uint32_t)__popcll(mc);
+                    }
+                    if (c == 0u) nB = min(at, cap);
+                    if (c == 1u) nA = min(at, cap);
                 }
-                cnt += cntB;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
             if (lane == 0) {
-                if (ch == nchunks - 1 && cnt > cap) {
+                if (ch == nchunks - 1 && (cnt > cap || lost)) {
                     atomicOr(&flags[1], 1u);
-                    cnt = cap;
+                    cnt = lost ? 0u : cap;
                 }
                 if (nh <= HOME_CAP) hcnt[i] = cnt;
-                if (ch == nchunks - 1 || nh > HOME_CAP) nlen[gi] = (SPLIT && split) ? (cnt | (nA << 16) | NLEN_SPLIT) : cnt;
+                if (ch == nchunks - 1 || nh > HOME_CAP)
+                    nlen[gi] = (SPLIT && split) ? (cnt | (nA << 10) | (nB << 20) | NLEN_SPLIT) : cnt;
                 // LOCAL: home record ((species << 28) | atom, list length)
                 if (LOCAL && ch == nchunks - 1) hrec[(size_t)c * bcap + i] = make_uint2(((uint32_t)type[gi] << 28) | gi, cnt);
             }
@@ -1834,11 +1860,11 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
     const double4* __restrict__ pos4, const uint8_t* __restrict__ type, uint32_t* __restrict__ nbr,
     uint32_t* __restrict__ nlen, double4* __restrict__ xref, uint32_t N, uint32_t logT, uint32_t cap, double cut2,
     uint32_t* __restrict__ flags, uint32_t* __restrict__ cand_store, uint4* __restrict__ cinfo,
-    uint2* __restrict__ hrec, uint16_t* __restrict__ nbr16, size_t region, double rin2) {
+    uint2* __restrict__ hrec, uint16_t* __restrict__ nbr16, size_t region, double rin2, double rb2) {
     if (flags[0] == 0u) return;
     build_cells<LOCAL, BUILD_THREADS, false, !LOCAL>(blockIdx.x, gridDim.x, PL, bcount, bucket, bcap, pos4, type,
                                                       nbr, nlen, xref, N, logT, cap, cut2, flags, cand_store, cinfo,
-                                                      hrec, nbr16, region, BatchTabs{}, rin2);
+                                                      hrec, nbr16, region, BatchTabs{}, rin2, rb2);
 }
 
 // Start of a list tick with the build one tick behind the binning
@@ -1859,13 +1885,13 @@ __global__ __launch_bounds__(BLOCK) void k_list_step(
     double4* __restrict__ xbin, double half_skin, uint32_t* __restrict__ bcount, uint32_t* __restrict__ bucket,
     uint32_t bcap, uint32_t* __restrict__ flags, uint32_t natom, const uint8_t* __restrict__ type,
     uint32_t* __restrict__ nbr, uint32_t* __restrict__ nlen, uint32_t N, uint32_t logT, uint32_t cap, double cut2,
-    double rin2) {
+    double rin2, double rb2) {
     const bool build_now = flags[11] != 0u;
     if (blockIdx.x >= natom) {
         if (!build_now) return;
         build_cells<false, BLOCK, false, true>(blockIdx.x - natom, gridDim.x - natom, PL, bcount, bucket, bcap, xbin,
                                                type, nbr, nlen, xref, N, logT, cap, cut2, flags, nullptr, nullptr,
-                                               nullptr, nullptr, 0, BatchTabs{}, rin2);
+                                               nullptr, nullptr, 0, BatchTabs{}, rin2, rb2);
         return;
     }
     const int tid = threadIdx.x, lane = tid & 63;
@@ -2077,19 +2103,22 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
         }
     }
     // this lane's entries: s, s+T, ... (cnt of them), four per group, the
-    // first cntA of them in the list's inner class
+    // first cntA of them in the list's inner class, the first cntB of those
+    // in its bind class
     const uint32_t len = lists_ok && own ? nlen_entries(nl0) : 0u;
     const bool split = (nl0 & NLEN_SPLIT) != 0u;
-    const uint32_t nA = split ? min(nlen_inner(nl0), len) : 0u;
+    const uint32_t nA = min(nlen_inner(nl0), len), nB = min(nlen_bind(nl0), nA);
     const uint32_t cnt = len > s ? (len - s + T - 1) >> LOGT : 0u;
     const uint32_t cntA = nA > s ? (nA - s + T - 1) >> LOGT : 0u;
+    const uint32_t cntB = nB > s ? (nB - s + T - 1) >> LOGT : 0u;
     // Wave-uniform group ranges: groups [0, gin) hold inner entries only on
     // every lane of the wave (a lane whose entries are all inner does not
-    // limit it); BOUND: groups from gout on hold no entry the screen needs.
+    // limit it); BOUND: groups from gout on hold no entry the screen needs
+    // (of a split list, only its bind class can hold bound pairs).
     uint32_t gin = 0u, gout = 0xFFFFFFFFu;
     if (!SINGLE) {
         uint32_t mn = cntA >= cnt ? 0xFFFFFFFFu : cntA;
-        uint32_t mx = (BOUND && (!split || ba.screen_all)) ? cnt : cntA;
+        uint32_t mx = (BOUND && (!split || ba.screen_all)) ? cnt : cntB;
 #pragma unroll
         for (int off = WAVE / 2; off > 0; off >>= 1) {
             mn = min(mn, (uint32_t)__shfl_xor((int)mn, off, WAVE));
@@ -2242,11 +2271,12 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
         if (SINGLE) {
             trips(no{}, scr{}, ng);
         } else {
-            // trips of two inner groups, then trips the screen needs, then
-            // the rest (gin rounded down, gout up, to whole trips)
-            const uint32_t kin = gin & ~1u;
-            trips(yes{}, scr{}, min(kin, ng));
-            if (BOUND) trips(no{}, scr{}, min(max(kin, (gout + 1u) & ~1u), ng));
+            // trips of two groups by class (gin rounded down, gout up, to
+            // whole trips): inner and screened, inner, screened, the rest
+            const uint32_t kin = gin & ~1u, kout = BOUND ? (gout + 1u) & ~1u : 0u;
+            if (BOUND) trips(yes{}, scr{}, min(min(kin, kout), ng));
+            trips(yes{}, no{}, min(kin, ng));
+            if (BOUND) trips(no{}, scr{}, min(kout, ng));
             trips(no{}, no{}, ng);
         }
     }
@@ -3385,6 +3415,16 @@ double list_inner2(const gs_universe* u) {
     const double r = list_inner(u);
     return r * r;
 }
+// Bind radius of the split lists: the bond cutoff k_bind max(sigma) plus the
+// skin (a pair built farther cannot be bound while its list is valid), at
+// most the inner radius; 0 without bond observation.
+double list_bind2(const gs_universe* u) {
+    if (!u->prm.enable_bond_observation) return 0.0;
+    double max_sigma = 0.0;
+    for (int s = 0; s < GS_NSPECIES; ++s) max_sigma = std::fmax(max_sigma, u->prm.sigma[s]);
+    const double rb = std::fmin(u->prm.bond_k_bind * max_sigma * (1.0 + 1e-9) + u->skin, list_inner(u));
+    return rb * rb;
+}
 
 int list_alloc(gs_universe* u, uint32_t cap, uint32_t bcap_min = 0) {
     int rc;
@@ -3602,7 +3642,7 @@ int list_tick_start(gs_universe* u) {
         k_list_step<<<natom + nbuild, BLOCK, 0, u->stream>>>(
             u->kp, PL, u->pos4[c], u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->scal, u->xref,
             u->xbin, 0.5 * u->skin, u->bcount, u->bucket, u->bcap, u->flags, natom, u->type[c], u->nbr, u->nlen, N,
-            (uint32_t)u->logT, u->nbr_cap, rl * rl, list_inner2(u));
+            (uint32_t)u->logT, u->nbr_cap, rl * rl, list_inner2(u), list_bind2(u));
         u->list_age += 1;
         HIPCHK(hipGetLastError());
         return GS_OK;
@@ -3626,11 +3666,12 @@ int list_tick_start(gs_universe* u) {
     if (u->pair_mode == 2)
         k_list_build<true><<<bgrid, BUILD_THREADS, 0, u->stream>>>(
             PL, u->bcount, u->bucket, u->bcap, u->pos4[c], u->type[c], nullptr, u->nlen, u->xref, N, 0u, u->nbr_cap,
-            rl * rl, u->flags, u->cand_store, u->cinfo, u->hrec, u->nbr16, cell_region(u), 0.0);
+            rl * rl, u->flags, u->cand_store, u->cinfo, u->hrec, u->nbr16, cell_region(u), 0.0, 0.0);
     else
         k_list_build<false><<<bgrid, BUILD_THREADS, 0, u->stream>>>(
             PL, u->bcount, u->bucket, u->bcap, u->pos4[c], u->type[c], u->nbr, u->nlen, u->xref, N,
-            (uint32_t)u->logT, u->nbr_cap, rl * rl, u->flags, nullptr, nullptr, nullptr, nullptr, 0, list_inner2(u));
+            (uint32_t)u->logT, u->nbr_cap, rl * rl, u->flags, nullptr, nullptr, nullptr, nullptr, 0, list_inner2(u),
+            list_bind2(u));
     u->list_age += 1;
     HIPCHK(hipGetLastError());
     return GS_OK;
