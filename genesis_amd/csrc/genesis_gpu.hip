@@ -1589,9 +1589,9 @@ __global__ void k_flag_set(uint32_t* flags, int idx, uint32_t v) { flags[idx] = 
 // SPLIT (the Verlet lists of one universe): with rin2 > 0, a cell whose
 // stencil fits one LDS chunk writes each list split (nlen_inner, nlen_bind):
 // its accepted candidates are staged in the wave's LDS row with their class
-// (within the bind radius, rb2 being its square and at most rin2; within
-// the inner radius; the rest) and written out class by class after the
-// sweep.  A list longer than SPLIT_CAP or a cell of several chunks is
+// and written out class by class after the sweep: within the inner radius
+// and the species pair's bind radius (rbt: its square widened by the FP32
+// margin, per species pair), within the inner radius, the rest.  A list longer than SPLIT_CAP or a cell of several chunks is
 // written unsplit.
 template <bool LOCAL, int NT, bool BATCH = false, bool SPLIT = false>
 __device__ __forceinline__ void build_cells(
@@ -1601,18 +1601,23 @@ __device__ __forceinline__ void build_cells(
     double4* __restrict__ xref, uint32_t N, uint32_t logT, uint32_t cap, double cut2, uint32_t* __restrict__ flags,
     uint32_t* __restrict__ cand_store, uint4* __restrict__ cinfo, uint2* __restrict__ hrec,
     uint16_t* __restrict__ nbr16, size_t region, const BatchTabs& bt = BatchTabs{}, double rin2 = 0.0,
-    double rb2 = 0.0) {
+    const float* __restrict__ rbt = nullptr) {
     constexpr int NWV = NT / WAVE;
     // SPLIT: the outer candidates of the wave's current home atom
     __shared__ uint32_t outb[SPLIT ? NWV : 1][SPLIT ? SPLIT_CAP : 1];
-    __shared__ uint8_t outc[SPLIT ? NWV : 1][SPLIT ? SPLIT_CAP : 1];
+    __shared__ uint8_t outc[SPLIT ? NWV : 1][SPLIT ? SPLIT_CAP : 1];  // their classes
+    __shared__ float s_rbt[SPLIT ? GS_NSPECIES * GS_NSPECIES : 1];    // bind radius^2 per species pair
     // the candidates' indices are staged, unordered, in the memory of the
     // position tile, which is filled only after they have been ordered
     __shared__ BuildCand tile[BUILD_TILE];
     __shared__ double org[3];  // centre of the home cell
     constexpr int IDX_CAP = LOCAL ? LDS_IDX_CAP_LOCAL : (BATCH ? LDS_IDX_CAP_BATCH : LDS_IDX_CAP);
     __shared__ uint32_t ord[IDX_CAP];
-    __shared__ uint32_t hcnt[HOME_CAP];  // running list lengths between chunks
+    // running list lengths between chunks: a cell of several chunks is built
+    // unsplit, so SPLIT keeps them in the staging rows
+    static_assert(!SPLIT || NWV * SPLIT_CAP >= (int)HOME_CAP, "the staging rows hold the list lengths");
+    __shared__ uint32_t hcnt_own[SPLIT ? 1 : HOME_CAP];
+    uint32_t* const hcnt = SPLIT ? &outb[0][0] : hcnt_own;
     __shared__ uint32_t seg_base[27];
     __shared__ uint32_t seg_pref[28];
     __shared__ double seg_sh[27][3];
@@ -1620,6 +1625,8 @@ __device__ __forceinline__ void build_cells(
     uint32_t* cand = reinterpret_cast<uint32_t*>(tile);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (b0 == 0 && tid == 0) flags[3] += 1u;  // rebuilds done on the device
+    if (SPLIT)
+        for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += NT) s_rbt[t] = rbt ? rbt[t] : 0.0f;
     const uint32_t ncell = BATCH ? bt.ncell : PL.ncell;
     for (uint32_t c = b0; c < ncell; c += stride) {
     // BATCH: a cell of a replica that does not build now is skipped (a
@@ -1639,7 +1646,6 @@ __device__ __forceinline__ void build_cells(
     // classed inner lies within the inner radius; the bind class takes every
     // candidate that may lie within the bind radius
     const float rin2f = (float)(rin2 * (1.0 - 1e-5));
-    const float rb2f = (float)(rb2 * (1.0 + 1e-5));
     __syncthreads();  // the previous cell's LDS readers are done
     const uint32_t nh = min(bcount[c], bcap);
     if (nh == 0) {
@@ -1767,12 +1773,16 @@ __device__ __forceinline__ void build_cells(
             const float3 pi = pi_n;
             if (i + NWV < nh) pi_n = home_pos(i + NWV);
             uint32_t cnt = ch == 0 ? 0u : (nh <= HOME_CAP ? hcnt[i] : nlen[gi]);
+            // the bind radii^2 of the home atom's species (split lists)
+            const float* rbrow =
+                s_rbt + (SPLIT ? min((uint32_t)type[gi], (uint32_t)GS_NSPECIES - 1u) * GS_NSPECIES : 0u);
             // 256 candidates per step, four per lane (independent distance
             // tests); the ballots are taken in candidate order, so the list
             // (each class of a split list) is ordered by candidate position
             // as with one per lane.  Split: every accepted candidate is
-            // staged in the wave's LDS row with its class (0 bind, 1 other
-            // inner, 2 the rest) and the row is written out class by class.
+            // staged in the wave's LDS row with its class (0 within the inner
+            // radius and the species pair's bind radius, 1 other inner, 2 the
+            // rest) and the row is written out class by class.
             for (uint32_t base = 0; base < len; base += 4 * WAVE) {
                 bool in[4];
                 uint32_t e[4], cls[4];
@@ -1786,7 +1796,10 @@ __device__ __forceinline__ void build_cells(
                     const float dx = pi.x - cj.x, dy = pi.y - cj.y, dz = pi.z - cj.z;
                     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                     in[q] = j < len && r2 < cut2f && jg != gi;
-                    cls[q] = r2 < rb2f ? 0u : (r2 < rin2f ? 1u : 2u);
+                    if (SPLIT) {
+                        const uint32_t sj = min(cj.e >> 28, (uint32_t)GS_NSPECIES - 1u);
+                        cls[q] = r2 < rin2f ? (r2 < rbrow[sj] ? 0u : 1u) : 2u;
+                    }
                 }
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
@@ -1843,7 +1856,7 @@ __device__ __forceinline__ void build_cells(
                     atomicOr(&flags[1], 1u);
                     cnt = lost ? 0u : cap;
                 }
-                if (nh <= HOME_CAP) hcnt[i] = cnt;
+                if (nchunks > 1 && nh <= HOME_CAP) hcnt[i] = cnt;
                 if (ch == nchunks - 1 || nh > HOME_CAP)
                     nlen[gi] = (SPLIT && split) ? (cnt | (nA << 10) | (nB << 20) | NLEN_SPLIT) : cnt;
                 // LOCAL: home record ((species << 28) | atom, list length)
@@ -1860,11 +1873,12 @@ __global__ __launch_bounds__(BUILD_THREADS) void k_list_build(
     const double4* __restrict__ pos4, const uint8_t* __restrict__ type, uint32_t* __restrict__ nbr,
     uint32_t* __restrict__ nlen, double4* __restrict__ xref, uint32_t N, uint32_t logT, uint32_t cap, double cut2,
     uint32_t* __restrict__ flags, uint32_t* __restrict__ cand_store, uint4* __restrict__ cinfo,
-    uint2* __restrict__ hrec, uint16_t* __restrict__ nbr16, size_t region, double rin2, double rb2) {
+    uint2* __restrict__ hrec, uint16_t* __restrict__ nbr16, size_t region, double rin2,
+    const float* __restrict__ rbt) {
     if (flags[0] == 0u) return;
     build_cells<LOCAL, BUILD_THREADS, false, !LOCAL>(blockIdx.x, gridDim.x, PL, bcount, bucket, bcap, pos4, type,
                                                       nbr, nlen, xref, N, logT, cap, cut2, flags, cand_store, cinfo,
-                                                      hrec, nbr16, region, BatchTabs{}, rin2, rb2);
+                                                      hrec, nbr16, region, BatchTabs{}, rin2, rbt);
 }
 
 // Start of a list tick with the build one tick behind the binning
@@ -1885,13 +1899,13 @@ __global__ __launch_bounds__(BLOCK) void k_list_step(
     double4* __restrict__ xbin, double half_skin, uint32_t* __restrict__ bcount, uint32_t* __restrict__ bucket,
     uint32_t bcap, uint32_t* __restrict__ flags, uint32_t natom, const uint8_t* __restrict__ type,
     uint32_t* __restrict__ nbr, uint32_t* __restrict__ nlen, uint32_t N, uint32_t logT, uint32_t cap, double cut2,
-    double rin2, double rb2) {
+    double rin2, const float* __restrict__ rbt) {
     const bool build_now = flags[11] != 0u;
     if (blockIdx.x >= natom) {
         if (!build_now) return;
         build_cells<false, BLOCK, false, true>(blockIdx.x - natom, gridDim.x - natom, PL, bcount, bucket, bcap, xbin,
                                                type, nbr, nlen, xref, N, logT, cap, cut2, flags, nullptr, nullptr,
-                                               nullptr, nullptr, 0, BatchTabs{}, rin2, rb2);
+                                               nullptr, nullptr, 0, BatchTabs{}, rin2, rbt);
         return;
     }
     const int tid = threadIdx.x, lane = tid & 63;
@@ -3073,6 +3087,8 @@ struct gs_universe {
     bool prebin = true;         // predictive binning in the list integrate kernel
     int logT = 2;               // lanes per atom = 1 << logT
     double skin = 0.0;
+    float* list_rbt = nullptr;  // squared bind radii per species pair (split lists)
+    float list_rbt_host[GS_NSPECIES * GS_NSPECIES] = {};
     int64_t sort_every = 100;   // ticks between full re-sorts of the state in list mode
     uint64_t order_gen = 0;     // changes whenever the resident order (id[cur]) may have changed
     int64_t list_age = 0;       // ticks since the last full re-sort
@@ -3415,15 +3431,27 @@ double list_inner2(const gs_universe* u) {
     const double r = list_inner(u);
     return r * r;
 }
-// Bind radius of the split lists: the bond cutoff k_bind max(sigma) plus the
-// skin (a pair built farther cannot be bound while its list is valid), at
-// most the inner radius; 0 without bond observation.
-double list_bind2(const gs_universe* u) {
-    if (!u->prm.enable_bond_observation) return 0.0;
-    double max_sigma = 0.0;
-    for (int s = 0; s < GS_NSPECIES; ++s) max_sigma = std::fmax(max_sigma, u->prm.sigma[s]);
-    const double rb = std::fmin(u->prm.bond_k_bind * max_sigma * (1.0 + 1e-9) + u->skin, list_inner(u));
-    return rb * rb;
+// Bind radii of the split lists per species pair: the pair's bond cutoff
+// k_bind sigma_ab (src/analysis/pairs.rs:132-174) plus the skin (a pair built
+// farther cannot be bound while its list is valid), squared and widened by
+// the build's FP32 margin; 0 without bond observation.  The device copy
+// list_rbt is uploaded whenever the table differs from the last upload.
+int list_bind_table(gs_universe* u) {
+    float t[GS_NSPECIES * GS_NSPECIES] = {};
+    if (u->prm.enable_bond_observation)
+        for (int a = 0; a < GS_NSPECIES; ++a)
+            for (int b = 0; b < GS_NSPECIES; ++b) {
+                const double rb = u->prm.bond_k_bind * (0.5 * (u->prm.sigma[a] + u->prm.sigma[b])) * (1.0 + 1e-9) +
+                                  u->skin;
+                t[a * GS_NSPECIES + b] = (float)(rb * rb * (1.0 + 1e-5));
+            }
+    if (u->list_rbt && std::memcmp(t, u->list_rbt_host, sizeof(t)) == 0) return GS_OK;
+    int rc;
+    if (!u->list_rbt && (rc = dalloc(&u->list_rbt, GS_NSPECIES * GS_NSPECIES))) return rc;
+    std::memcpy(u->list_rbt_host, t, sizeof(t));
+    HIPCHK(hipMemcpyAsync(u->list_rbt, u->list_rbt_host, sizeof(t), hipMemcpyHostToDevice, u->stream));
+    HIPCHK(hipStreamSynchronize(u->stream));
+    return GS_OK;
 }
 
 int list_alloc(gs_universe* u, uint32_t cap, uint32_t bcap_min = 0) {
@@ -3625,6 +3653,7 @@ int list_tick_start(gs_universe* u) {
     const double rl = u->kp.rc + u->skin;
     const KParams& PL = u->kpl;
     int rc;
+    if (u->pair_mode == 1 && (rc = list_bind_table(u))) return rc;
     if (!u->list_valid || u->list_age >= u->sort_every) {
         if ((rc = rebuild(u, true, PL))) return rc;
         k_list_fill_buckets<<<nblocks(PL.ncell, BLOCK / WAVE), BLOCK, 0, u->stream>>>(PL.ncell, u->cstart, u->bcount,
@@ -3642,7 +3671,7 @@ int list_tick_start(gs_universe* u) {
         k_list_step<<<natom + nbuild, BLOCK, 0, u->stream>>>(
             u->kp, PL, u->pos4[c], u->v[c][0], u->v[c][1], u->v[c][2], u->a[0], u->a[1], u->a[2], u->scal, u->xref,
             u->xbin, 0.5 * u->skin, u->bcount, u->bucket, u->bcap, u->flags, natom, u->type[c], u->nbr, u->nlen, N,
-            (uint32_t)u->logT, u->nbr_cap, rl * rl, list_inner2(u), list_bind2(u));
+            (uint32_t)u->logT, u->nbr_cap, rl * rl, list_inner2(u), u->list_rbt);
         u->list_age += 1;
         HIPCHK(hipGetLastError());
         return GS_OK;
@@ -3666,12 +3695,12 @@ int list_tick_start(gs_universe* u) {
     if (u->pair_mode == 2)
         k_list_build<true><<<bgrid, BUILD_THREADS, 0, u->stream>>>(
             PL, u->bcount, u->bucket, u->bcap, u->pos4[c], u->type[c], nullptr, u->nlen, u->xref, N, 0u, u->nbr_cap,
-            rl * rl, u->flags, u->cand_store, u->cinfo, u->hrec, u->nbr16, cell_region(u), 0.0, 0.0);
+            rl * rl, u->flags, u->cand_store, u->cinfo, u->hrec, u->nbr16, cell_region(u), 0.0, nullptr);
     else
         k_list_build<false><<<bgrid, BUILD_THREADS, 0, u->stream>>>(
             PL, u->bcount, u->bucket, u->bcap, u->pos4[c], u->type[c], u->nbr, u->nlen, u->xref, N,
             (uint32_t)u->logT, u->nbr_cap, rl * rl, u->flags, nullptr, nullptr, nullptr, nullptr, 0, list_inner2(u),
-            list_bind2(u));
+            u->list_rbt);
     u->list_age += 1;
     HIPCHK(hipGetLastError());
     return GS_OK;
@@ -3847,6 +3876,7 @@ int gs_destroy(gs_universe* u) {
     for (hipEvent_t e : u->ev) (void)hipEventDestroy(e);
     (void)hipFree(u->nbr);
     (void)hipFree(u->nbr16);
+    (void)hipFree(u->list_rbt);
     (void)hipFree(u->cand_store);
     (void)hipFree(u->cinfo);
     (void)hipFree(u->hrec);
