@@ -465,23 +465,28 @@ struct BoundScreen {
 // Per-species-pair factors of the list pass, formed once per workgroup in
 // LDS with the expressions pair_law_sp evaluates per pair: sigma_ij =
 // (sigma_i + sigma_j) / 2, 4 eps_ij = a4_i a4_j, its sixfold (the factor of
-// the LJ force), the table's 1 / rcore, and the bound-pair screen's
-// thresholds sigma_ij^2 k_bind^2 (1 -+ 1e-10).  48-byte rows: a wave's
-// lanes reading different rows spread over the LDS banks.
+// the LJ force), the table's 1 / rcore, the core radius sigma_ij / 2, and
+// the bound-pair screen's thresholds sigma_ij^2 k_bind^2 (1 -+ 1e-10).
+// The table has LIST_COLS columns per row: a list entry's four species bits
+// index it directly (columns past the last species repeat it), so a pair's
+// row address is its entry shifted, with no clamp.
 struct ListCoef {
     double sig, eps4;
     double e6, irc;
-    double thlo, thhi;
+    double rcore, thlo;
+    double thhi, pad;
 };
+constexpr int LIST_COLS = 16;
 
 // Four pairs of one atom at once, statement by statement (each statement
 // over the four pairs before the next), with the expressions of the list
 // traversal and pair_law_sp: a dependent FP64 operation waits for its
 // operand's result for several issue slots, so the four independent chains
-// are interleaved to fill them.  p: the four neighbour positions; sp: their
-// species in successive bytes.  A slot past the end of the list holds the
-// atom's own position (r^2 = 0), which every test below rejects, so no
-// liveness mask is carried.
+// are interleaved to fill them.  p: the four neighbour positions; ent:
+// their list entries (species in bits 28-31, the column of lrow).  A slot
+// past the end of the list holds the atom's own entry (r^2 = 0), which the
+// tests below reject: such a pair (and a coincident one, r^2 <= eps2) is
+// evaluated at r^2 = eps2 and scaled by 0, so no liveness mask is carried.
 // Q0, NQ: the pairs q0 .. q0 + nq - 1 of the group (a group is evaluated as
 // two halves, which holds fewer values live at once; the sums take the
 // pairs in the same order either way).
@@ -496,15 +501,18 @@ struct ListCoef {
 template <bool COUL, bool SHARED, bool GRAV, bool INNER, bool SCREEN, int Q0, int NQ>
 __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double Ly, double Lz, double iLx, double iLy,
                                            double iLz, const double4& pi, double kqi, const ListCoef* lrow,
-                                           const PairCoef* row, const double4 (&p)[4], uint32_t sp, double& fx,
+                                           const PairCoef* row, const double4 (&p)[4], const uint4& ent, double& fx,
                                            double& fy, double& fz, double& vl, uint32_t& hitm) {
-    bool ok[4], core[4];
-    int tj[4];
-    double dx[4], dy[4], dz[4], r2[4], y[4], e[4], inv_d[4], d[4], sig[4], eps4[4], e6[4], irc[4], r[4], inv_r[4];
-    double traw[4], thlo[4], thhi[4];
+    bool core[4];
+    uint32_t tj[4];
+    double dx[4], dy[4], dz[4], r2[4], y[4], e[4], inv_d[4], d[4], sig[4], eps4[4], e6[4], irc[4], rcore[4], r[4];
+    double inv_r[4], okf[4], traw[4], thlo[4], thhi[4];
     double s6[4], s12[4], vlj[4], flj[4], sw[4], dsw[4], fmag[4], vv[4];
+    {
+        const uint32_t f4[4] = {ent.x, ent.y, ent.z, ent.w};
 #pragma unroll
-    for (int q = Q0; q < Q0 + NQ; ++q) tj[q] = (int)min((sp >> (8 * q)) & 0xFFu, (uint32_t)(GS_NSPECIES - 1));
+        for (int q = Q0; q < Q0 + NQ; ++q) tj[q] = f4[q] >> 28;
+    }
 #pragma unroll
     for (int q = Q0; q < Q0 + NQ; ++q) {
         dx[q] = pi.x - p[q].x;
@@ -519,9 +527,12 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
     }
 #pragma unroll
     for (int q = Q0; q < Q0 + NQ; ++q) {
+        // a masked pair (r^2 <= eps2: the atom itself past the end of its
+        // list, or a coincident pair) is evaluated at r^2 = eps2, where every
+        // term is finite, and its force and energy are scaled by okf = 0
         const double t = fma(dz[q], dz[q], fma(dy[q], dy[q], dx[q] * dx[q]));
-        ok[q] = t > P.eps2;
-        r2[q] = ok[q] ? t : 1.0;
+        okf[q] = t > P.eps2 ? 1.0 : 0.0;
+        r2[q] = fmax(t, P.eps2);
         if (SCREEN) traw[q] = t;
     }
     // rsqrt_refined, four at a time
@@ -539,6 +550,7 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
         eps4[q] = c.eps4;
         e6[q] = c.e6;
         irc[q] = c.irc;
+        rcore[q] = c.rcore;
         if (SCREEN) {
             thlo[q] = c.thlo;
             thhi[q] = c.thhi;
@@ -561,9 +573,8 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
     }
 #pragma unroll
     for (int q = Q0; q < Q0 + NQ; ++q) {
-        const double rcore = 0.5 * sig[q];
-        core[q] = d[q] < rcore;
-        r[q] = core[q] ? rcore : d[q];
+        core[q] = d[q] < rcore[q];
+        r[q] = core[q] ? rcore[q] : d[q];
         inv_r[q] = core[q] ? irc[q] : inv_d[q];
     }
 #pragma unroll
@@ -623,7 +634,7 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
         for (int q = Q0; q < Q0 + NQ; ++q) {
             const bool cc = d[q] < 0.5;
             const double inv_g = cc ? 2.0 : inv_d[q];
-            const double vg = -(P.gconst * row[tj[q]].mm) * inv_g;
+            const double vg = -(P.gconst * row[min(tj[q], (uint32_t)GS_NSPECIES - 1u)].mm) * inv_g;
             const double fg = vg * inv_g;
             if (INNER) {
                 fmag[q] += fg;
@@ -638,11 +649,11 @@ __device__ __forceinline__ void pair_eval4(const KParams& P, double Lx, double L
     }
 #pragma unroll
     for (int q = Q0; q < Q0 + NQ; ++q) {
-        const double fs = ok[q] ? fmag[q] * inv_d[q] : 0.0;
+        const double fs = (fmag[q] * inv_d[q]) * okf[q];
         fx = fma(fs, dx[q], fx);
         fy = fma(fs, dy[q], fy);
         fz = fma(fs, dz[q], fz);
-        vl += ok[q] ? vv[q] : 0.0;
+        vl = fma(vv[q], okf[q], vl);
     }
 }
 
@@ -1363,6 +1374,10 @@ __global__ __launch_bounds__(BLOCK) void k_cells_entity(KParams P, const uint32_
 // every tick.
 
 constexpr uint32_t NBR_IDX_MASK = 0x0FFFFFFFu;  // entry = (species << 28) | sorted index
+// Atoms the Verlet-list pass holds: below 2^27 an entry shifted left by 5 is
+// its atom's byte offset in the 32-byte position array (the species bits
+// and bit 27 leave the word), which the gathers use directly.
+constexpr uint32_t LIST_ATOMS_MAX = 1u << 27;
 // A candidate as the list build stages it in LDS: its position relative to
 // the centre of the home cell in FP32 (a Verlet list may hold a superset of
 // the pairs within rc + skin, so the build's distance test need not be FP64)
@@ -1851,6 +1866,17 @@ __device__ __forceinline__ void build_cells(
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
+            if (!LOCAL && ch == nchunks - 1) {
+                // each lane's groups are padded with the atom's own entry
+                // (r^2 = 0: no pair) to the end of its last group and one
+                // group beyond it, which the pair pass reads past the end of
+                // the lane's entries (entries [len, 4T (ceil(len / 4T) + 1)))
+                const uint32_t lenf = (cnt > cap || lost) ? (lost ? 0u : cap) : cnt;
+                const uint32_t T4 = 4u << logT;
+                const uint32_t kend = ((lenf + T4 - 1u) / T4 + 1u) * T4;
+                const uint32_t own = ((uint32_t)type[gi] << 28) | gi;
+                for (uint32_t k = lenf + (uint32_t)lane; k < kend; k += WAVE) nbr[nbr_slot(k, gi, N, logT)] = own;
+            }
             if (lane == 0) {
                 if (ch == nchunks - 1 && (cnt > cap || lost)) {
                     atomicOr(&flags[1], 1u);
@@ -2029,7 +2055,7 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
                                                      BoundArgs ba = BoundArgs{}) {
     constexpr int T = 1 << LOGT;
     __shared__ PairCoef sc[(SINGLE || GRAV) ? (SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES) : 1];
-    __shared__ ListCoef lc[SINGLE ? 1 : GS_NSPECIES * GS_NSPECIES];
+    __shared__ ListCoef lc[SINGLE ? 1 : GS_NSPECIES * LIST_COLS];
     __shared__ double wv[BLOCK / WAVE];
     __shared__ double wk[BLOCK / WAVE];
     __shared__ double red[2 * BLOCK / WAVE];
@@ -2068,17 +2094,21 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
         if (tid == 0) sc[0] = coef[0];
     } else {
         const SpCoef* spc = reinterpret_cast<const SpCoef*>(coef + COEF_SP_OFFSET);
-        for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += BLOCK) {
-            if (GRAV) sc[t] = coef[t];
-            const SpCoef a = spc[t / GS_NSPECIES], b = spc[t % GS_NSPECIES];
+        if (GRAV)
+            for (int t = tid; t < GS_NSPECIES * GS_NSPECIES; t += BLOCK) sc[t] = coef[t];
+        for (int t = tid; t < GS_NSPECIES * LIST_COLS; t += BLOCK) {
+            const int ta = t / LIST_COLS, tb = min(t % LIST_COLS, GS_NSPECIES - 1);
+            const SpCoef a = spc[ta], b = spc[tb];
             ListCoef c;
             c.sig = 0.5 * (a.sigma + b.sigma);
             c.eps4 = a.a4 * b.a4;
             c.e6 = 6.0 * c.eps4;
-            c.irc = coef[t].inv_rcore;
+            c.irc = coef[ta * GS_NSPECIES + tb].inv_rcore;
+            c.rcore = 0.5 * c.sig;
             const double th2 = c.sig * c.sig;
             c.thlo = th2 * ba.scr.kb2_lo;
             c.thhi = th2 * ba.scr.kb2_hi;
+            c.pad = 0.0;
             lc[t] = c;
         }
     }
@@ -2141,31 +2171,34 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
         gin = __builtin_amdgcn_readfirstlane(mn) >> 2;
         if (BOUND) gout = (__builtin_amdgcn_readfirstlane(mx) + 3u) >> 2;
     }
-    if (i < N) {
+    if (i < N) ti = ti0;
+    // (lists of a build that did not complete are not walked: they may hold
+    // unwritten groups)
+    if (i < N && lists_ok && own) {
         const double4 pi = pi0;
-        ti = ti0;
         const double kqi = P.ke * pi.w;
         const PairCoef* row = SINGLE ? sc : sc + ((SINGLE || GRAV) ? ti * GS_NSPECIES : 0);
-        const ListCoef* lrow = lc + (SINGLE ? 0 : ti * GS_NSPECIES);
+        const ListCoef* lrow = lc + (SINGLE ? 0 : min(ti, GS_NSPECIES - 1) * LIST_COLS);
         // a group's four position gathers are all in flight before its four
         // pairs are evaluated
         const uint4* grp = reinterpret_cast<const uint4*>(nbr) + (((size_t)i << LOGT) + s);
-        const size_t gstride = (size_t)N << LOGT;
+        const uint32_t gstride = N << LOGT;  // < 2^31: N <= LIST_ATOMS_MAX
         // Software pipeline over the lane's entry groups, two groups per
         // loop trip with the registers of the two named A and B (so no
         // register receiving a load in flight is ever copied, which would
         // make the wave wait for it): a step for group k loads the entries of
         // group k+2, issues the four position gathers of group k+1 and then
         // evaluates group k, whose gathers have had the previous step's
-        // arithmetic to arrive in.  Loads are unconditional (address
-        // selects: past the end a lane re-reads its last group and gathers
+        // arithmetic to arrive in.  Loads are unconditional (past the end a
+        // lane reads its padding group, the atom's own entry, and gathers
         // its own position), so the waits count them statically.  The
         // entries stream from HBM once per tick and carry a non-temporal
         // hint, so they do not displace the positions the gathers reuse
         // from L2.
         const uint32_t ng = (cnt + 3) >> 2;  // groups of this lane
-        const uint4* glast = grp + (size_t)(ng ? ng - 1 : 0) * gstride;
-        auto group_ptr = [&](uint32_t k) { return k < ng ? grp + (size_t)k * gstride : glast; };
+        // past its last group a lane reads group ng, which holds the atom's
+        // own entry only (the build pads every lane's list with it)
+        auto group_ptr = [&](uint32_t k) { return grp + (uint64_t)min(k, ng) * (uint64_t)gstride; };
         // BOUND: the pairs the pair law's screen marks (hitm: bound for
         // certain, or within 1e-10 of a threshold) join the lane's column;
         // bandm marks the column entries the exact test must decide
@@ -2184,21 +2217,18 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
                 nhit += h != 0u ? 1u : 0u;
             }
         };
-        // positions of a group's four entries (past the end: the atom's
-        // own) and their species, one byte each
-        auto gather = [&](const uint4& e, uint32_t k, double4 (&p)[4], uint32_t& sp) {
+        // positions of a group's four entries (past the end of the lane's
+        // entries: the atom's own); entry << 5 is the byte offset of its
+        // atom (N <= LIST_ATOMS_MAX)
+        auto gather = [&](const uint4& e, double4 (&p)[4]) {
             const uint32_t f4[4] = {e.x, e.y, e.z, e.w};
-            sp = 0u;
+            const char* base = reinterpret_cast<const char*>(pos4);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const bool live = 4 * k + q < cnt;
-                p[q] = pos4[live ? (f4[q] & NBR_IDX_MASK) : i];
-                sp |= (f4[q] >> 28) << (8 * q);
-            }
+            for (int q = 0; q < 4; ++q) p[q] = *reinterpret_cast<const double4*>(base + (f4[q] << 5));
         };
         // the pair law over one group (SINGLE: the scalar law of the
         // one-coefficient table)
-        auto evaluate = [&](auto inner_c, auto screen_c, const double4 (&p)[4], uint32_t sp) -> uint32_t {
+        auto evaluate = [&](auto inner_c, auto screen_c, const double4 (&p)[4], const uint4& ent) -> uint32_t {
             constexpr bool IN = decltype(inner_c)::value, SC = decltype(screen_c)::value;
             uint32_t hitm = 0u;
             if (SINGLE) {
@@ -2230,9 +2260,9 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
                     vl += ok ? vv : 0.0;
                 }
             } else {
-                pair_eval4<COUL, SHARED, GRAV, IN, SC, 0, 2>(P, Lx, Ly, Lz, iLx, iLy, iLz, pi, kqi, lrow, row, p, sp,
+                pair_eval4<COUL, SHARED, GRAV, IN, SC, 0, 2>(P, Lx, Ly, Lz, iLx, iLy, iLz, pi, kqi, lrow, row, p, ent,
                                                              fx, fy, fz, vl, hitm);
-                pair_eval4<COUL, SHARED, GRAV, IN, SC, 2, 2>(P, Lx, Ly, Lz, iLx, iLy, iLz, pi, kqi, lrow, row, p, sp,
+                pair_eval4<COUL, SHARED, GRAV, IN, SC, 2, 2>(P, Lx, Ly, Lz, iLx, iLy, iLz, pi, kqi, lrow, row, p, ent,
                                                              fx, fy, fz, vl, hitm);
             }
             return hitm;
@@ -2240,13 +2270,12 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
         uint4 eA = load_nt(group_ptr(0));
         uint4 eB = load_nt(group_ptr(1));
         double4 pA[4], pB[4];
-        uint32_t tA = 0u, tB = 0u;
-        uint4 jA = make_uint4(0u, 0u, 0u, 0u), jB = jA;  // BOUND: a group's entries, kept for its screen
+        uint4 jA = make_uint4(0u, 0u, 0u, 0u), jB = jA;  // a group's entries, kept for its law and screen
         // group 1's entries are issued before group 0's gathers, the order
         // every later step keeps
         __builtin_amdgcn_sched_barrier(0);
-        gather(eA, 0, pA, tA);
-        if (BOUND) jA = eA;
+        gather(eA, pA);
+        jA = eA;
         // Trips of two groups (k, k + 1) while k < kend, with the law of one
         // group class.  No exit between a step's loads and the next step (the
         // compiler would sink the loads past it): a lane with an odd number
@@ -2260,21 +2289,21 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
                 // step A: group k
                 __builtin_amdgcn_sched_barrier(0);
                 eA = load_nt(group_ptr(k + 2));
-                gather(eB, k + 1, pB, tB);
-                if (BOUND) jB = eB;
+                gather(eB, pB);
+                jB = eB;
                 __builtin_amdgcn_sched_barrier(0);
                 {
-                    const uint32_t hm = evaluate(inner_c, screen_c, pA, tA);
+                    const uint32_t hm = evaluate(inner_c, screen_c, pA, jA);
                     if (SC) record(hm, jA);
                 }
                 // step B: group k + 1
                 __builtin_amdgcn_sched_barrier(0);
                 eB = load_nt(group_ptr(k + 3));
-                gather(eA, k + 2, pA, tA);
-                if (BOUND) jA = eA;
+                gather(eA, pA);
+                jA = eA;
                 __builtin_amdgcn_sched_barrier(0);
                 {
-                    const uint32_t hm = evaluate(inner_c, screen_c, pB, tB);
+                    const uint32_t hm = evaluate(inner_c, screen_c, pB, jB);
                     if (SC) record(hm, jB);
                 }
             }
@@ -3521,9 +3550,10 @@ int list_alloc(gs_universe* u, uint32_t cap, uint32_t bcap_min = 0) {
         (void)hipFree(u->nbr);
         u->nbr = nullptr;
         u->nbr_have = 0;
-        // groups of 4 entries per lane: (cap + 4 T) entries per atom cover any
+        // groups of 4 entries per lane: (cap + 8 T) entries per atom hold
+        // the padded lists (each lane's last group and one more) of any
         // lanes-per-atom setting up to 16
-        if ((rc = dalloc(&u->nbr, ((size_t)cap + (4u << MAX_LOGT)) * n))) return rc;
+        if ((rc = dalloc(&u->nbr, ((size_t)cap + (8u << MAX_LOGT)) * n))) return rc;
         u->nbr_have = cap;
     }
     return GS_OK;
@@ -3784,7 +3814,7 @@ int gs_create(const gs_params* params, int64_t n_atoms, gs_universe** out) {
     std::memcpy(&u->prm, params, sizeof(gs_params));
     std::vector<PairCoef> coef = derive(u);
     // Default list skin: 6% of the cutoff (adapted lifetime keeps it valid).
-    u->list_ok = !u->small_box && list_geometry(u, 0.06 * u->kp.rc);
+    u->list_ok = !u->small_box && n_atoms <= (int64_t)LIST_ATOMS_MAX && list_geometry(u, 0.06 * u->kp.rc);
     if (!u->list_ok) u->pair_mode = 0;
     (void)hipGetDevice(&u->dev);
     if ((rc = noalloc_scratch_setup(u->dev))) {
@@ -4309,6 +4339,8 @@ int gs_set_tuning(gs_universe* u, const gs_tuning* t) {
             default: return fail(GS_E_INVALID, "lanes_per_atom must be 1, 2, 4, 8 or 16");
         }
         if (!(t->skin >= 0.0) || !std::isfinite(t->skin)) return fail(GS_E_INVALID, "skin must be >= 0");
+        if (t->pair_mode == 1 && (uint64_t)u->n > LIST_ATOMS_MAX)
+            return fail(GS_E_INVALID, "Verlet-list mode holds at most 2^27 atoms");
         if (u->small_box || !list_geometry(u, t->skin))
             return fail(GS_E_INVALID, "list mode needs >= 3 cells of edge rc + skin on every axis");
         u->list_ok = true;
