@@ -1603,11 +1603,11 @@ __global__ void k_flag_set(uint32_t* flags, int idx, uint32_t v) { flags[idx] = 
 // on brute-force pairs have empty buckets.
 // SPLIT (the Verlet lists of one universe): with rin2 > 0, a cell whose
 // stencil fits one LDS chunk writes each list split (nlen_inner, nlen_bind):
-// its accepted candidates are staged in the wave's LDS row with their class
-// and written out class by class after the sweep: within the inner radius
+// its accepted candidates are staged in the wave's LDS row and, after the
+// sweep, classed and written out class by class: within the inner radius
 // and the species pair's bind radius (rbt: its square widened by the FP32
-// margin, per species pair), within the inner radius, the rest.  A list longer than SPLIT_CAP or a cell of several chunks is
-// written unsplit.
+// margin, per species pair), within the inner radius, the rest.  A list
+// longer than SPLIT_CAP or a cell of several chunks is written unsplit.
 template <bool LOCAL, int NT, bool BATCH = false, bool SPLIT = false>
 __device__ __forceinline__ void build_cells(
     uint32_t b0, uint32_t stride, KParams PL, const uint32_t* __restrict__ bcount,
@@ -1795,12 +1795,13 @@ __device__ __forceinline__ void build_cells(
             // tests); the ballots are taken in candidate order, so the list
             // (each class of a split list) is ordered by candidate position
             // as with one per lane.  Split: every accepted candidate is
-            // staged in the wave's LDS row with its class (0 within the inner
-            // radius and the species pair's bind radius, 1 other inner, 2 the
-            // rest) and the row is written out class by class.
+            // staged in the wave's LDS row (its tile position); after the
+            // sweep each gets its class (0 within the inner radius and the
+            // species pair's bind radius, 1 other inner, 2 the rest) and the
+            // row is written out class by class.
             for (uint32_t base = 0; base < len; base += 4 * WAVE) {
                 bool in[4];
-                uint32_t e[4], cls[4];
+                uint32_t e[4], jt[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const uint32_t j = base + (uint32_t)(q * WAVE + lane);
@@ -1808,13 +1809,10 @@ __device__ __forceinline__ void build_cells(
                     const BuildCand cj = tile[jc];
                     const uint32_t jg = cj.e & NBR_IDX_MASK;
                     e[q] = LOCAL ? ((cj.e >> 28) << CELL_IDX_BITS) | (c0 + jc) : cj.e;
+                    jt[q] = jc;
                     const float dx = pi.x - cj.x, dy = pi.y - cj.y, dz = pi.z - cj.z;
                     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                     in[q] = j < len && r2 < cut2f && jg != gi;
-                    if (SPLIT) {
-                        const uint32_t sj = min(cj.e >> 28, (uint32_t)GS_NSPECIES - 1u);
-                        cls[q] = r2 < rin2f ? (r2 < rbrow[sj] ? 0u : 1u) : 2u;
-                    }
                 }
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
@@ -1822,10 +1820,8 @@ __device__ __forceinline__ void build_cells(
                     if (in[q]) {
                         const uint32_t k = cnt + mask_rank(m);
                         if (SPLIT && split) {
-                            if (k < (uint32_t)SPLIT_CAP) {
-                                outb[w][k] = e[q];
-                                outc[w][k] = (uint8_t)cls[q];
-                            }
+                            // (the tile position; the flush classes it)
+                            if (k < (uint32_t)SPLIT_CAP) outb[w][k] = jt[q];
                         } else if (k < cap) {
                             if (LOCAL)
                                 reg[(size_t)i * cap + k] = (uint16_t)e[q];
@@ -1844,6 +1840,22 @@ __device__ __forceinline__ void build_cells(
             if (SPLIT && split && !lost) {
                 // the staged row, class by class (the wave's own LDS row: its
                 // writes above precede these reads in the wave's LDS order)
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                // each staged candidate's class, from its tile entry with the
+                // sweep's FP32 distance, and its list entry
+                for (uint32_t k0 = 0; k0 < cnt; k0 += WAVE) {
+                    const uint32_t k = k0 + (uint32_t)lane;
+                    if (k < cnt) {
+                        const BuildCand cj = tile[outb[w][k]];
+                        const float dx = pi.x - cj.x, dy = pi.y - cj.y, dz = pi.z - cj.z;
+                        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        const uint32_t sj = min(cj.e >> 28, (uint32_t)GS_NSPECIES - 1u);
+                        outc[w][k] = (uint8_t)(r2 < rin2f ? (r2 < rbrow[sj] ? 0u : 1u) : 2u);
+                        outb[w][k] = cj.e;
+                    }
+                }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
