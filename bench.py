@@ -50,12 +50,13 @@ CPU_SHARE = 16              # host cores one GPU's job may use on the GPU box
 # WRITE_SIZE (KiB per dispatch) of `bench.py --leg ionic` at the default setting, scaled by
 # what the same counters read for a 256 MiB device copy (`bench.py --copy-probe`).  Only
 # reported for the setting it was measured at.
-PAIR_TRAFFIC = {"setting": {"lanes": 2, "skin_frac": 0.04}, "fetch_kib": 414164.1, "write_kib": 119668.3,
+PAIR_TRAFFIC = {"setting": {"lanes": 2, "skin_frac": 0.035}, "fetch_kib": 409979.6, "write_kib": 112370.6,
                 "fetch_scale": 2.0, "write_scale": 1.0,
                 "source": "profiles/k_pair_list_bound__ionic_melt1048576_pmc.md (ticks 12001..12500)"}
 # the melt is a solid in the timed window: its resident order keeps its locality, so a full
 # re-sort every 300 ticks (1.256e9 atom-steps/s against 1.241e9 at 100 and 1.255e9 at 1000)
-DEFAULT_LANES, DEFAULT_SKIN_FRAC, DEFAULT_SORT_EVERY = 2, 0.04, 300
+# skin 0.035 rc: as fast per tick as 0.04 (more rebuilds, fewer list entries; 0.03 is slower)
+DEFAULT_LANES, DEFAULT_SKIN_FRAC, DEFAULT_SORT_EVERY = 2, 0.035, 300
 GR_EVERY, COMP_EVERY = 100, 500
 
 
