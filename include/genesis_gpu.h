@@ -208,7 +208,8 @@ int gs_step(gs_universe* u, int64_t n_steps, gs_observables* obs);
 
 int gs_get_tuning(gs_universe* u, gs_tuning* out);
 /* Validates and applies; GS_E_INVALID if the setting is impossible for this
- * box (list mode needs >= 3 cells of edge rc + skin per axis). */
+ * box (list mode needs >= 3 cells of edge rc + skin per axis) or universe
+ * (the Verlet list, pair_mode 1, holds at most 2^27 atoms). */
 int gs_set_tuning(gs_universe* u, const gs_tuning* t);
 /* No counterpart in the reference (where gs_step finds the bound pairs of
  * bond observation).  0 (default): on the Verlet-list path with 2 or 4 lanes
@@ -416,6 +417,7 @@ int gs_adjoint_backward(gs_universe* u, double* dloss_dtheta);
  * Bond observation and the bond interaction are not part of the batched
  * path: gs_batch_create rejects params with either enabled (GS_E_INVALID). */
 typedef struct gs_batch gs_batch;
+/* (at most 2^27 atoms over all replicas) */
 int gs_batch_create(int32_t n_replicas, const gs_params* params, const int64_t* n_atoms, gs_batch** out);
 int gs_batch_destroy(gs_batch* b);
 /* State of replica r (slot order within the replica); every replica must be
