@@ -2016,6 +2016,14 @@ __device__ __forceinline__ void close_list_tick(uint32_t* flags) {
 // bound_pairs_with_grid (src/analysis/pairs.rs:62-85) among the entries it
 // evaluates anyway.
 constexpr int BOUND_MAXT = 32;  // bound partners kept per owner (the tracker's episodes)
+// Bond counters: 5 words (formed bonds, lifetime sum, bonded pairs, bonded
+// entities, the overflow flag) and the species matrix, kept in 64 stripes:
+// workgroup b adds to stripe b mod 64, so thousands of workgroups do not
+// serialise on a few addresses; readers sum the stripes (the overflow word,
+// [4], lives in stripe 0 only).
+constexpr int BOND_CNT_WORDS = 5 + GS_NSPECIES * GS_NSPECIES;
+constexpr int BOND_STRIPES = 64;
+constexpr int BOND_CNT_ALL = BOND_STRIPES * BOND_CNT_WORDS;
 constexpr int HIT_LANE = 16;    // screened candidates a lane's LDS column holds
 // Bound-now lists by resident slot i, slot-major: word q of slot i at q n + i.
 __host__ __device__ __forceinline__ size_t now_idx(uint32_t q, uint32_t i, size_t n) {
@@ -2424,10 +2432,23 @@ __global__ __launch_bounds__(BLOCK) void k_bound_from_hits(KParams P, uint32_t N
                                                            const uint32_t* __restrict__ hinfo,
                                                            const double* __restrict__ thr, double cd2,
                                                            uint32_t* __restrict__ nowp, uint32_t* __restrict__ nown,
-                                                           unsigned long long* __restrict__ cnt) {
+                                                           unsigned long long* __restrict__ cnt,
+                                                           uint32_t* __restrict__ clr_deg,
+                                                           uint32_t* __restrict__ clr_bdeg) {
     constexpr int T = 1 << LOGT;
     __shared__ uint32_t rows[BOUND_MAXT][BLOCK];
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    // the tracker's inputs cleared for the tick (k_bond_clear's work: the
+    // next set's degrees, the degrees by slot, the per-tick counter words;
+    // the grid covers max(N, the counter words))
+    if (i < N) {
+        clr_deg[i] = 0u;
+        clr_bdeg[i] = 0u;
+    }
+    if (i < (uint32_t)BOND_CNT_ALL) {
+        const uint32_t w = i % (uint32_t)BOND_CNT_WORDS;
+        if (w == 2u || w == 3u || w >= 5u) cnt[i] = 0ull;
+    }
     if (i >= N) return;
     const size_t NT = (size_t)N << LOGT;
     const uint32_t e = id[i];
