@@ -2459,19 +2459,39 @@ __global__ __launch_bounds__(BLOCK) void k_bound_from_hits(KParams P, uint32_t N
     uint32_t info[T];
 #pragma unroll
     for (int s = 0; s < T; ++s) info[s] = hinfo[(i << LOGT) + (uint32_t)s];
+    // four hits at a time: their entry loads, then their id gathers, are in
+    // flight together; the first four of every lane's column are all loaded
+    // before any is considered (past a column's end: the atom's own slot)
+    uint32_t f0[T][4], em0[T][4];
+#pragma unroll
+    for (int s = 0; s < T; ++s) {
+        const uint32_t c = info[s] & 0xFFu, nh = c > (uint32_t)HIT_LANE ? 0u : c;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) f0[s][k] = (uint32_t)k < nh ? hits[(size_t)k * NT + (i << LOGT) + (uint32_t)s] : i;
+    }
+#pragma unroll
+    for (int s = 0; s < T; ++s)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) em0[s][k] = id[f0[s][k] & NBR_IDX_MASK];
 #pragma unroll
     for (int s = 0; s < T; ++s) {
         const uint32_t g = (i << LOGT) + (uint32_t)s;
         const uint32_t c = info[s] & 0xFFu, band = info[s] >> 16;
         const uint32_t nh = c > (uint32_t)HIT_LANE ? 0u : c;  // an overflowed column is replayed
-        // four hits at a time: their entry loads, then their id gathers, are
-        // in flight together
         for (uint32_t h0 = 0; h0 < nh; h0 += 4) {
             uint32_t f[4], em[4];
+            if (h0 == 0) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) f[k] = hits[(size_t)min(h0 + (uint32_t)k, nh - 1u) * NT + g];
+                for (int k = 0; k < 4; ++k) {
+                    f[k] = f0[s][k];
+                    em[k] = em0[s][k];
+                }
+            } else {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) em[k] = id[f[k] & NBR_IDX_MASK];
+                for (int k = 0; k < 4; ++k) f[k] = hits[(size_t)min(h0 + (uint32_t)k, nh - 1u) * NT + g];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) em[k] = id[f[k] & NBR_IDX_MASK];
+            }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const uint32_t h = h0 + (uint32_t)k;
