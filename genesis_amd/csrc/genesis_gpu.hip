@@ -2303,9 +2303,14 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
         // The classes run as separate loops, so each is scheduled and its
         // registers allocated on its own.
         uint32_t k = 0u;
-        auto trips = [&](auto inner_c, auto screen_c, uint32_t kend) {
+        // (LAST: the lane's final phase; its trips stop while a group is
+        // left, and the final trip, with one or two groups, issues no
+        // gather past the lane's last group: a gathered lane costs the load
+        // path the same whether its data is used or not)
+        auto trips = [&](auto inner_c, auto screen_c, uint32_t kend, auto last_c) {
             constexpr bool SC = decltype(screen_c)::value;
-            for (; k < kend; k += 2) {
+            constexpr bool LAST = decltype(last_c)::value;
+            for (; LAST ? k + 2u < kend : k < kend; k += 2) {
                 // step A: group k
                 __builtin_amdgcn_sched_barrier(0);
                 eA = load_nt(group_ptr(k + 2));
@@ -2327,20 +2332,36 @@ __global__ __launch_bounds__(BLOCK, GS_LIST_WAVES) void k_pair_list(KParams P, u
                     if (SC) record(hm, jB);
                 }
             }
+            if (LAST && k < kend) {
+                // groups k and (if it exists) k + 1; k's gathers are in
+                // flight, k + 1's entries loaded
+                const bool two = k + 1u < kend;
+                if (two) gather(eB, pB);
+                jB = eB;
+                {
+                    const uint32_t hm = evaluate(inner_c, screen_c, pA, jA);
+                    if (SC) record(hm, jA);
+                }
+                if (two) {
+                    const uint32_t hm = evaluate(inner_c, screen_c, pB, jB);
+                    if (SC) record(hm, jB);
+                }
+                k += 2;
+            }
         };
         using yes = std::integral_constant<bool, true>;
         using no = std::integral_constant<bool, false>;
         using scr = std::integral_constant<bool, BOUND>;
         if (SINGLE) {
-            trips(no{}, scr{}, ng);
+            trips(no{}, scr{}, ng, yes{});
         } else {
             // trips of two groups by class (gin rounded down, gout up, to
             // whole trips): inner and screened, inner, screened, the rest
             const uint32_t kin = gin & ~1u, kout = BOUND ? (gout + 1u) & ~1u : 0u;
-            if (BOUND) trips(yes{}, scr{}, min(min(kin, kout), ng));
-            trips(yes{}, no{}, min(kin, ng));
-            if (BOUND) trips(no{}, scr{}, min(kout, ng));
-            trips(no{}, no{}, ng);
+            if (BOUND) trips(yes{}, scr{}, min(min(kin, kout), ng), no{});
+            trips(yes{}, no{}, min(kin, ng), no{});
+            if (BOUND) trips(no{}, scr{}, min(kout, ng), no{});
+            trips(no{}, no{}, ng, yes{});
         }
     }
 #pragma unroll
