@@ -50,7 +50,7 @@ CPU_SHARE = 16              # host cores one GPU's job may use on the GPU box
 # WRITE_SIZE (KiB per dispatch) of `bench.py --leg ionic` at the default setting, scaled by
 # what the same counters read for a 256 MiB device copy (`bench.py --copy-probe`).  Only
 # reported for the setting it was measured at.
-PAIR_TRAFFIC = {"setting": {"lanes": 2, "skin_frac": 0.035}, "fetch_kib": 409979.6, "write_kib": 112370.6,
+PAIR_TRAFFIC = {"setting": {"lanes": 2, "skin_frac": 0.035}, "fetch_kib": 376322.7, "write_kib": 112371.0,
                 "fetch_scale": 2.0, "write_scale": 1.0,
                 "source": "profiles/k_pair_list_bound__ionic_melt1048576_pmc.md (ticks 12001..12500)"}
 # the melt is a solid in the timed window: its resident order keeps its locality, so a full
